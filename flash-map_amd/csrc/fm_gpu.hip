@@ -1,0 +1,729 @@
+// fm_gpu.hip -- GPU backend (MI355X / gfx950) of the FlashMap C ABI.
+//
+// Replaces the reference's GpuFlashMap (src/gpu.rs:32-712): no NVRTC JIT,
+// the kernels are compiled ahead of time for gfx950; device memory is sized
+// once per map; per-call scratch and staging buffers grow geometrically and
+// are reused, so steady-state calls allocate nothing.  All work for a map is
+// ordered on one stream (default priority).
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "fm_internal.h"
+#include "fm_kernels.h"
+
+namespace fm {
+namespace {
+
+using dev::Counters;
+using dev::Table;
+
+constexpr uint64_t kMaxGpuCapacity = 1ull << 31;  // u32 slot ids in scratch
+
+// ---------------------------------------------------------------------------
+// Per-(K,V) launchers.  Specialised sizes get register-resident keys and
+// vector loads; everything else runs the generic byte path (K,V <= 256).
+// ---------------------------------------------------------------------------
+struct Ops {
+  hipError_t (*get)(const Table&, const uint8_t*, uint64_t, uint8_t*, uint8_t*, uint32_t, uint32_t, hipStream_t);
+  hipError_t (*ins_dry)(const Table&, const uint8_t*, uint64_t, unsigned long long*, uint32_t*, Counters*, uint32_t,
+                        hipStream_t);
+  hipError_t (*ins_replay)(const Table&, const uint8_t*, const uint8_t*, uint64_t, const uint32_t*, const uint32_t*,
+                           Counters*, uint32_t, hipStream_t);
+  hipError_t (*rem_probe)(const Table&, const uint8_t*, uint64_t, unsigned long long*, uint32_t, uint32_t, hipStream_t);
+  hipError_t (*rem_runs)(const Table&, const uint8_t*, uint64_t, uint32_t*, uint32_t*, uint32_t, hipStream_t);
+  hipError_t (*rem_replay)(const Table&, const uint8_t*, uint64_t, const uint32_t*, const uint32_t*, Counters*,
+                           uint32_t, hipStream_t);
+};
+
+inline uint32_t grid_for(uint64_t threads, uint32_t block) {
+  const uint64_t g = (threads + block - 1) / block;
+  return static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(g, 1u << 20)));
+}
+
+template <int KS, int VS, int G>
+hipError_t launch_get_g(const Table& t, const uint8_t* q, uint64_t n, uint8_t* ov, uint8_t* of, uint32_t block,
+                        hipStream_t s) {
+  hipLaunchKernelGGL((dev::k_get<KS, VS, G>), dim3(grid_for(n * G, block)), dim3(block), 0, s, t, q, n, ov, of);
+  return hipGetLastError();
+}
+
+template <int KS, int VS>
+hipError_t launch_get(const Table& t, const uint8_t* q, uint64_t n, uint8_t* ov, uint8_t* of, uint32_t G,
+                      uint32_t block, hipStream_t s) {
+  switch (G) {
+    case 1: return launch_get_g<KS, VS, 1>(t, q, n, ov, of, block, s);
+    case 2: return launch_get_g<KS, VS, 2>(t, q, n, ov, of, block, s);
+    case 4: return launch_get_g<KS, VS, 4>(t, q, n, ov, of, block, s);
+    case 8: return launch_get_g<KS, VS, 8>(t, q, n, ov, of, block, s);
+    default: return launch_get_g<KS, VS, 16>(t, q, n, ov, of, block, s);
+  }
+}
+
+template <int KS>
+hipError_t launch_ins_dry(const Table& t, const uint8_t* k, uint64_t n, unsigned long long* Q, uint32_t* home,
+                          Counters* c, uint32_t block, hipStream_t s) {
+  hipLaunchKernelGGL((dev::k_ins_dry<KS>), dim3(grid_for(n, block)), dim3(block), 0, s, t, k, n, Q, home, c);
+  return hipGetLastError();
+}
+
+template <int KS, int VS>
+hipError_t launch_ins_replay(const Table& t, const uint8_t* k, const uint8_t* v, uint64_t n, const uint32_t* cid,
+                             const uint32_t* idx, Counters* c, uint32_t block, hipStream_t s) {
+  hipLaunchKernelGGL((dev::k_ins_replay<KS, VS>), dim3(grid_for(n, block)), dim3(block), 0, s, t, k, v, n, cid, idx,
+                     c);
+  return hipGetLastError();
+}
+
+template <int KS, int G>
+hipError_t launch_rem_probe_g(const Table& t, const uint8_t* q, uint64_t n, unsigned long long* ev, uint32_t block,
+                              hipStream_t s) {
+  hipLaunchKernelGGL((dev::k_rem_probe<KS, G>), dim3(grid_for(n * G, block)), dim3(block), 0, s, t, q, n, ev);
+  return hipGetLastError();
+}
+
+template <int KS>
+hipError_t launch_rem_probe(const Table& t, const uint8_t* q, uint64_t n, unsigned long long* ev, uint32_t G,
+                            uint32_t block, hipStream_t s) {
+  switch (G) {
+    case 1: return launch_rem_probe_g<KS, 1>(t, q, n, ev, block, s);
+    case 2: return launch_rem_probe_g<KS, 2>(t, q, n, ev, block, s);
+    case 4: return launch_rem_probe_g<KS, 4>(t, q, n, ev, block, s);
+    case 8: return launch_rem_probe_g<KS, 8>(t, q, n, ev, block, s);
+    default: return launch_rem_probe_g<KS, 16>(t, q, n, ev, block, s);
+  }
+}
+
+template <int KS>
+hipError_t launch_rem_runs(const Table& t, const uint8_t* q, uint64_t n, uint32_t* cid, uint32_t* idx, uint32_t block,
+                           hipStream_t s) {
+  hipLaunchKernelGGL((dev::k_rem_runs<KS>), dim3(grid_for(n, block)), dim3(block), 0, s, t, q, n, cid, idx);
+  return hipGetLastError();
+}
+
+template <int KS>
+hipError_t launch_rem_replay(const Table& t, const uint8_t* q, uint64_t n, const uint32_t* cid, const uint32_t* idx,
+                             Counters* c, uint32_t block, hipStream_t s) {
+  hipLaunchKernelGGL((dev::k_rem_replay<KS>), dim3(grid_for(n, block)), dim3(block), 0, s, t, q, n, cid, idx, c);
+  return hipGetLastError();
+}
+
+template <int KS, int VS>
+Ops make_ops() {
+  Ops o;
+  o.get = &launch_get<KS, VS>;
+  o.ins_dry = &launch_ins_dry<KS>;
+  o.ins_replay = &launch_ins_replay<KS, VS>;
+  o.rem_probe = &launch_rem_probe<KS>;
+  o.rem_runs = &launch_rem_runs<KS>;
+  o.rem_replay = &launch_rem_replay<KS>;
+  return o;
+}
+
+// Sizes with register-resident fast paths: the five benchmark shapes
+// (u64->u64, [u8;32]->[u8;128], [u8;32]->u64, u64->[u8;32]) and the
+// reference test shapes ([u8;32]->[u8;64]).
+Ops select_ops(uint32_t K, uint32_t V) {
+#define FM_SIZE(k, v) \
+  if (K == k && V == v) return make_ops<k, v>();
+  FM_SIZE(8, 8)
+  FM_SIZE(32, 128)
+  FM_SIZE(32, 8)
+  FM_SIZE(8, 32)
+  FM_SIZE(32, 64)
+  FM_SIZE(32, 32)
+  FM_SIZE(16, 16)
+  FM_SIZE(8, 16)
+#undef FM_SIZE
+  return make_ops<0, 0>();
+}
+
+uint32_t vec_align(uint32_t size) {
+  if (size % 16 == 0) return 16;
+  if (size % 8 == 0) return 8;
+  if (size % 4 == 0) return 4;
+  return 1;
+}
+
+std::string hip_msg(const char* what, hipError_t e) { return std::string(what) + ": " + hipGetErrorString(e); }
+
+class GpuBackend final : public Backend {
+ public:
+  GpuBackend() = default;
+  ~GpuBackend() override { release(); }
+
+  fm_status init(const fm_builder& b, std::string* err);
+
+  fm_backend_kind kind() const override { return FM_BACKEND_GPU; }
+  uint64_t capacity() const override { return t_.cap; }
+  uint64_t len() const override { return len_; }
+  uint32_t key_size() const override { return t_.K; }
+  uint32_t value_size() const override { return t_.V; }
+
+  fm_status bulk_get(const void* keys, uint64_t n, void* out_values, uint8_t* out_found) override;
+  fm_status bulk_insert(const void* keys, const void* values, uint64_t n, uint64_t* out_new) override;
+  fm_status bulk_remove(const void* keys, uint64_t n, uint64_t* out_removed) override;
+  fm_status clear() override;
+  fm_status export_slots(uint32_t* flags, void* keys, void* values) override;
+  fm_status import_slots(const uint32_t* flags, const void* keys, const void* values, uint64_t len) override;
+
+  void* device_stream() override { return stream_; }
+  int32_t device_id() const override { return dev_; }
+  fm_status alloc_device(uint64_t bytes, void** out) override;
+  fm_status free_device(void* p) override;
+  fm_status upload(const void* host, uint64_t bytes, void** out) override;
+  fm_status download(const void* dptr, uint64_t bytes, void* host) override;
+  fm_status bulk_get_device(const void* d_keys, uint64_t n, void* d_out, uint8_t* d_found) override;
+  fm_status bulk_insert_device(const void* d_keys, const void* d_vals, uint64_t n, uint64_t* out_new,
+                               bool counted) override;
+  fm_status bulk_remove_device(const void* d_keys, uint64_t n, uint64_t* out_removed) override;
+  fm_status recount(uint64_t* out) override;
+  fm_status synchronize() override;
+  fm_status get_tuning(fm_tuning* out) const override {
+    *out = tune_;
+    return FM_OK;
+  }
+  fm_status set_tuning(const fm_tuning* t) override;
+
+ private:
+  fm_status fail(fm_status code, const std::string& msg) {
+    set_error(code, msg);
+    return code;
+  }
+  fm_status check_launch(hipError_t e, const char* what) {
+    return e == hipSuccess ? FM_OK : fail(FM_ERR_KERNEL_LAUNCH, hip_msg(what, e));
+  }
+  fm_status use_device() {
+    hipError_t e = hipSetDevice(dev_);
+    return e == hipSuccess ? FM_OK : fail(FM_ERR_DEVICE_INIT, hip_msg("hipSetDevice", e));
+  }
+  fm_status ensure_scratch(uint64_t n);
+  fm_status ensure_staging(uint64_t bytes_keys, uint64_t bytes_vals, uint64_t bytes_out, uint64_t bytes_found);
+  fm_status read_counters(Counters* out);
+  fm_status zero_counters();
+  fm_status check_device_ptr(const void* p, uint32_t align, const char* what);
+  fm_status check_insert_status(const Counters& c) {
+    if (c.status & dev::kStatusBudget)
+      return fail(FM_ERR_KERNEL_LAUNCH, "insert walk found no free slot in a full table (uncounted inserts overfilled it)");
+    return FM_OK;
+  }
+
+  // Device pipelines (inputs on the device, ordered on stream_).
+  fm_status run_get(const uint8_t* d_keys, uint64_t n, uint8_t* d_out, uint8_t* d_found);
+  fm_status run_insert(const uint8_t* d_keys, const uint8_t* d_vals, uint64_t n);  // leaves count in ctr
+  fm_status run_remove(const uint8_t* d_keys, uint64_t n, uint64_t* removed);
+
+  void release();
+
+  int dev_ = -1;
+  hipStream_t stream_ = nullptr;
+  Table t_{};
+  uint64_t len_ = 0;
+  Ops ops_{};
+  fm_tuning tune_{4, 256, 4, 256};
+
+  // scratch (grown geometrically, reused)
+  uint64_t scratch_n_ = 0;
+  uint32_t* home_ = nullptr;
+  uint32_t* cid_ = nullptr;
+  uint32_t* idx_ = nullptr;
+  uint32_t* cid_s_ = nullptr;
+  uint32_t* idx_s_ = nullptr;
+  unsigned long long* ev_ = nullptr;
+  uint8_t* action_ = nullptr;
+  void* sort_tmp_ = nullptr;
+  size_t sort_tmp_bytes_ = 0;
+  unsigned long long* Q_ = nullptr;
+  Counters* ctr_ = nullptr;
+  Counters* h_ctr_ = nullptr;  // pinned
+  int sort_end_bit_ = 1;
+
+  // host-API staging
+  uint8_t* st_keys_ = nullptr;
+  uint64_t st_keys_bytes_ = 0;
+  uint8_t* st_vals_ = nullptr;
+  uint64_t st_vals_bytes_ = 0;
+  uint8_t* st_out_ = nullptr;
+  uint64_t st_out_bytes_ = 0;
+  uint8_t* st_found_ = nullptr;
+  uint64_t st_found_bytes_ = 0;
+};
+
+template <typename T>
+void dfree(T*& p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+void GpuBackend::release() {
+  if (dev_ < 0) return;
+  (void)hipSetDevice(dev_);
+  if (stream_) (void)hipStreamSynchronize(stream_);
+  dfree(t_.keys);
+  dfree(t_.vals);
+  dfree(t_.flags);
+  dfree(t_.freebits);
+  dfree(t_.emptybits);
+  dfree(t_.owner);
+  dfree(home_);
+  dfree(cid_);
+  dfree(idx_);
+  dfree(cid_s_);
+  dfree(idx_s_);
+  dfree(ev_);
+  dfree(action_);
+  if (sort_tmp_) (void)hipFree(sort_tmp_);
+  sort_tmp_ = nullptr;
+  dfree(Q_);
+  dfree(ctr_);
+  if (h_ctr_) (void)hipHostFree(h_ctr_);
+  h_ctr_ = nullptr;
+  dfree(st_keys_);
+  dfree(st_vals_);
+  dfree(st_out_);
+  dfree(st_found_);
+  if (stream_) (void)hipStreamDestroy(stream_);
+  stream_ = nullptr;
+  dev_ = -1;
+}
+
+fm_status GpuBackend::init(const fm_builder& b, std::string* err) {
+  // GpuFlashMap::new (gpu.rs:45-105)
+  if (b.capacity == 0) {
+    *err = "capacity must be positive";
+    return FM_ERR_ZERO_CAPACITY;
+  }
+  if (b.key_size == 0 || b.key_size > dev::kMaxGenericSize || b.value_size == 0 ||
+      b.value_size > dev::kMaxGenericSize) {
+    *err = "key/value size outside 1..256 bytes";
+    return FM_ERR_INVALID_ARGUMENT;
+  }
+  const uint64_t cap = round_capacity(b.capacity);
+  if (cap > kMaxGpuCapacity) {
+    *err = "capacity above 2^31 slots";
+    return FM_ERR_INVALID_ARGUMENT;
+  }
+  int count = 0;
+  hipError_t e = hipGetDeviceCount(&count);
+  if (e != hipSuccess || count <= 0) {
+    *err = e != hipSuccess ? hip_msg("hipGetDeviceCount", e) : std::string("no HIP device");
+    return FM_ERR_DEVICE_INIT;
+  }
+  if (b.device_id < 0 || b.device_id >= count) {
+    *err = "device ordinal " + std::to_string(b.device_id) + " out of range";
+    return FM_ERR_DEVICE_INIT;
+  }
+  dev_ = b.device_id;
+  if ((e = hipSetDevice(dev_)) != hipSuccess) {
+    *err = hip_msg("hipSetDevice", e);
+    return FM_ERR_DEVICE_INIT;
+  }
+  if ((e = hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking)) != hipSuccess) {
+    *err = hip_msg("hipStreamCreate", e);
+    return FM_ERR_DEVICE_INIT;
+  }
+  t_.cap = cap;
+  t_.mask = cap - 1;
+  t_.K = b.key_size;
+  t_.V = b.value_size;
+  t_.mode = b.hash_strategy == FM_HASH_MURMUR3 ? 1u : 0u;
+  const uint64_t words = (cap + 63) / 64;
+  // Align per-slot arrays for 16-byte vector access.
+  if ((e = hipMalloc(&t_.keys, cap * t_.K + 16)) != hipSuccess ||
+      (e = hipMalloc(&t_.vals, cap * t_.V + 16)) != hipSuccess ||
+      (e = hipMalloc(&t_.flags, cap * sizeof(uint32_t))) != hipSuccess ||
+      (e = hipMalloc(&t_.freebits, words * 8)) != hipSuccess ||
+      (e = hipMalloc(&t_.emptybits, words * 8)) != hipSuccess ||
+      (e = hipMalloc(&t_.owner, cap * sizeof(uint32_t))) != hipSuccess ||
+      (e = hipMalloc(&ctr_, sizeof(Counters))) != hipSuccess ||
+      (e = hipHostMalloc(&h_ctr_, sizeof(Counters), hipHostMallocDefault)) != hipSuccess) {
+    *err = hip_msg("hipMalloc (table)", e);
+    return FM_ERR_GPU_ALLOC;
+  }
+  ops_ = select_ops(t_.K, t_.V);
+  sort_end_bit_ = 1;
+  while ((1ull << sort_end_bit_) <= cap) ++sort_end_bit_;  // ids in [0, cap]
+  hipLaunchKernelGGL(dev::k_clear, dim3(grid_for(cap, 256)), dim3(256), 0, stream_, t_);
+  hipLaunchKernelGGL(dev::k_fill_u32, dim3(grid_for(cap, 256)), dim3(256), 0, stream_, t_.owner, cap, dev::kNoOwner);
+  if ((e = hipGetLastError()) != hipSuccess || (e = hipStreamSynchronize(stream_)) != hipSuccess) {
+    *err = hip_msg("table init", e);
+    return FM_ERR_DEVICE_INIT;
+  }
+  len_ = 0;
+  return FM_OK;
+}
+
+fm_status GpuBackend::set_tuning(const fm_tuning* t) {
+  auto pow2 = [](uint32_t g) { return g >= 1 && g <= 16 && (g & (g - 1)) == 0; };
+  auto blk = [](uint32_t b) { return b >= 64 && b <= 1024 && b % 64 == 0; };
+  if (!t || !pow2(t->get_group) || !pow2(t->remove_group) || !blk(t->get_block) || !blk(t->insert_block))
+    return fail(FM_ERR_INVALID_ARGUMENT, "tuning: group must be 1,2,4,8,16; block a multiple of 64 in 64..1024");
+  tune_ = *t;
+  return FM_OK;
+}
+
+fm_status GpuBackend::ensure_scratch(uint64_t n) {
+  if (n <= scratch_n_) return FM_OK;
+  uint64_t want = std::max<uint64_t>(n, std::max<uint64_t>(1024, scratch_n_ * 2));
+  if (want > 0x7FFFFFFFull) want = std::max<uint64_t>(n, 0x7FFFFFFFull);
+  if (n > 0x7FFFFFFFull) return fail(FM_ERR_INVALID_ARGUMENT, "batch above 2^31-1 keys");
+  (void)hipStreamSynchronize(stream_);
+  dfree(home_);
+  dfree(cid_);
+  dfree(idx_);
+  dfree(cid_s_);
+  dfree(idx_s_);
+  dfree(ev_);
+  dfree(action_);
+  if (sort_tmp_) (void)hipFree(sort_tmp_);
+  sort_tmp_ = nullptr;
+  scratch_n_ = 0;
+  hipError_t e;
+  if ((e = hipMalloc(&home_, want * 4)) != hipSuccess || (e = hipMalloc(&cid_, want * 4)) != hipSuccess ||
+      (e = hipMalloc(&idx_, want * 4)) != hipSuccess || (e = hipMalloc(&cid_s_, want * 4)) != hipSuccess ||
+      (e = hipMalloc(&idx_s_, want * 4)) != hipSuccess || (e = hipMalloc(&ev_, want * 8)) != hipSuccess ||
+      (e = hipMalloc(&action_, want)) != hipSuccess)
+    return fail(FM_ERR_GPU_ALLOC, hip_msg("hipMalloc (scratch)", e));
+  size_t bytes = 0;
+  e = hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, cid_, cid_s_, idx_, idx_s_, static_cast<int>(want), 0,
+                                         sort_end_bit_, stream_);
+  if (e != hipSuccess) return fail(FM_ERR_KERNEL_LAUNCH, hip_msg("radix sort sizing", e));
+  if ((e = hipMalloc(&sort_tmp_, bytes + 256)) != hipSuccess)
+    return fail(FM_ERR_GPU_ALLOC, hip_msg("hipMalloc (sort scratch)", e));
+  sort_tmp_bytes_ = bytes + 256;
+  if (!Q_) {
+    const uint64_t words = (t_.cap + 63) / 64;
+    if ((e = hipMalloc(&Q_, words * 8)) != hipSuccess) return fail(FM_ERR_GPU_ALLOC, hip_msg("hipMalloc (Q)", e));
+  }
+  scratch_n_ = want;
+  return FM_OK;
+}
+
+fm_status GpuBackend::ensure_staging(uint64_t bk, uint64_t bv, uint64_t bo, uint64_t bf) {
+  auto grow = [&](uint8_t*& p, uint64_t& have, uint64_t need) -> hipError_t {
+    if (need <= have) return hipSuccess;
+    (void)hipStreamSynchronize(stream_);
+    dfree(p);
+    have = 0;
+    const uint64_t want = std::max<uint64_t>(need, 4096);
+    hipError_t e = hipMalloc(&p, want);
+    if (e == hipSuccess) have = want;
+    return e;
+  };
+  hipError_t e;
+  if ((e = grow(st_keys_, st_keys_bytes_, bk)) != hipSuccess || (e = grow(st_vals_, st_vals_bytes_, bv)) != hipSuccess ||
+      (e = grow(st_out_, st_out_bytes_, bo)) != hipSuccess || (e = grow(st_found_, st_found_bytes_, bf)) != hipSuccess)
+    return fail(FM_ERR_GPU_ALLOC, hip_msg("hipMalloc (staging)", e));
+  return FM_OK;
+}
+
+fm_status GpuBackend::zero_counters() {
+  hipError_t e = hipMemsetAsync(ctr_, 0, sizeof(Counters), stream_);
+  return e == hipSuccess ? FM_OK : fail(FM_ERR_KERNEL_LAUNCH, hip_msg("hipMemsetAsync", e));
+}
+
+fm_status GpuBackend::read_counters(Counters* out) {
+  hipError_t e = hipMemcpyAsync(h_ctr_, ctr_, sizeof(Counters), hipMemcpyDeviceToHost, stream_);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream_);
+  if (e != hipSuccess) return fail(FM_ERR_TRANSFER, hip_msg("counter readback", e));
+  *out = *h_ctr_;
+  return FM_OK;
+}
+
+fm_status GpuBackend::check_device_ptr(const void* p, uint32_t align, const char* what) {
+  if (!p) return fail(FM_ERR_INVALID_ARGUMENT, std::string(what) + ": null device pointer");
+  if (reinterpret_cast<uintptr_t>(p) % align != 0)
+    return fail(FM_ERR_INVALID_ARGUMENT, std::string(what) + ": device buffer not " + std::to_string(align) +
+                                             "-byte aligned");
+  return FM_OK;
+}
+
+fm_status GpuBackend::run_get(const uint8_t* d_keys, uint64_t n, uint8_t* d_out, uint8_t* d_found) {
+  return check_launch(ops_.get(t_, d_keys, n, d_out, d_found, tune_.get_group, tune_.get_block, stream_), "k_get");
+}
+
+fm_status GpuBackend::run_insert(const uint8_t* d_keys, const uint8_t* d_vals, uint64_t n) {
+  fm_status st;
+  if ((st = ensure_scratch(n)) != FM_OK) return st;
+  if ((st = zero_counters()) != FM_OK) return st;
+  const uint64_t words = (t_.cap + 63) / 64;
+  hipError_t e = hipMemcpyAsync(Q_, t_.freebits, words * 8, hipMemcpyDeviceToDevice, stream_);
+  if (e != hipSuccess) return fail(FM_ERR_TRANSFER, hip_msg("bitmap copy", e));
+  const uint32_t blk = tune_.insert_block;
+  if ((st = check_launch(ops_.ins_dry(t_, d_keys, n, Q_, home_, ctr_, blk, stream_), "k_ins_dry")) != FM_OK) return st;
+  hipLaunchKernelGGL(dev::k_ins_closure, dim3(grid_for(n, blk)), dim3(blk), 0, stream_, t_.cap, Q_, home_, n, cid_,
+                     idx_, ctr_);
+  if ((st = check_launch(hipGetLastError(), "k_ins_closure")) != FM_OK) return st;
+  size_t tmp = sort_tmp_bytes_;
+  e = hipcub::DeviceRadixSort::SortPairs(sort_tmp_, tmp, cid_, cid_s_, idx_, idx_s_, static_cast<int>(n), 0,
+                                         sort_end_bit_, stream_);
+  if (e != hipSuccess) return fail(FM_ERR_KERNEL_LAUNCH, hip_msg("radix sort", e));
+  return check_launch(ops_.ins_replay(t_, d_keys, d_vals, n, cid_s_, idx_s_, ctr_, blk, stream_), "k_ins_replay");
+}
+
+fm_status GpuBackend::run_remove(const uint8_t* d_keys, uint64_t n, uint64_t* removed) {
+  fm_status st;
+  if ((st = ensure_scratch(n)) != FM_OK) return st;
+  if ((st = zero_counters()) != FM_OK) return st;
+  const uint32_t blk = tune_.insert_block;
+  if ((st = check_launch(ops_.rem_probe(t_, d_keys, n, ev_, tune_.remove_group, tune_.get_block, stream_),
+                         "k_rem_probe")) != FM_OK)
+    return st;
+  hipLaunchKernelGGL(dev::k_rem_check, dim3(grid_for(n, blk)), dim3(blk), 0, stream_, t_, ev_, n, action_, ctr_);
+  if ((st = check_launch(hipGetLastError(), "k_rem_check")) != FM_OK) return st;
+  hipLaunchKernelGGL(dev::k_rem_apply, dim3(grid_for(n, blk)), dim3(blk), 0, stream_, t_, ev_, action_, n, ctr_);
+  if ((st = check_launch(hipGetLastError(), "k_rem_apply")) != FM_OK) return st;
+  Counters c;
+  if ((st = read_counters(&c)) != FM_OK) return st;
+  if (c.status & dev::kStatusConflict) {
+    // Exact replay grouped by EMPTY-delimited runs, from the untouched
+    // pre-batch state.
+    if ((st = zero_counters()) != FM_OK) return st;
+    if ((st = check_launch(ops_.rem_runs(t_, d_keys, n, cid_, idx_, blk, stream_), "k_rem_runs")) != FM_OK) return st;
+    size_t tmp = sort_tmp_bytes_;
+    hipError_t e = hipcub::DeviceRadixSort::SortPairs(sort_tmp_, tmp, cid_, cid_s_, idx_, idx_s_, static_cast<int>(n),
+                                                      0, sort_end_bit_, stream_);
+    if (e != hipSuccess) return fail(FM_ERR_KERNEL_LAUNCH, hip_msg("radix sort", e));
+    if ((st = check_launch(ops_.rem_replay(t_, d_keys, n, cid_s_, idx_s_, ctr_, blk, stream_), "k_rem_replay")) !=
+        FM_OK)
+      return st;
+    if ((st = read_counters(&c)) != FM_OK) return st;
+  }
+  *removed = c.count;
+  return FM_OK;
+}
+
+// ---- host-facing API -------------------------------------------------------
+
+fm_status GpuBackend::bulk_get(const void* keys, uint64_t n, void* out_values, uint8_t* out_found) {
+  if (n == 0) return FM_OK;  // gpu.rs:112-114
+  fm_status st;
+  if ((st = use_device()) != FM_OK) return st;
+  if ((st = ensure_staging(n * t_.K, 0, n * t_.V, n)) != FM_OK) return st;
+  hipError_t e = hipMemcpyAsync(st_keys_, keys, n * t_.K, hipMemcpyHostToDevice, stream_);
+  if (e != hipSuccess) return fail(FM_ERR_TRANSFER, hip_msg("H2D keys", e));
+  if ((st = run_get(st_keys_, n, st_out_, st_found_)) != FM_OK) return st;
+  if ((e = hipMemcpyAsync(out_values, st_out_, n * t_.V, hipMemcpyDeviceToHost, stream_)) != hipSuccess ||
+      (e = hipMemcpyAsync(out_found, st_found_, n, hipMemcpyDeviceToHost, stream_)) != hipSuccess ||
+      (e = hipStreamSynchronize(stream_)) != hipSuccess)
+    return fail(FM_ERR_TRANSFER, hip_msg("D2H results", e));
+  return FM_OK;
+}
+
+fm_status GpuBackend::bulk_insert(const void* keys, const void* values, uint64_t n, uint64_t* out_new) {
+  if (n == 0) {  // gpu.rs:194-196
+    *out_new = 0;
+    return FM_OK;
+  }
+  if (len_ + n > t_.cap) {  // gpu.rs:198-206, MAX_LOAD_FACTOR = 1.0
+    set_table_full(len_, t_.cap);
+    return FM_ERR_TABLE_FULL;
+  }
+  fm_status st;
+  if ((st = use_device()) != FM_OK) return st;
+  if ((st = ensure_staging(n * t_.K, n * t_.V, 0, 0)) != FM_OK) return st;
+  hipError_t e;
+  if ((e = hipMemcpyAsync(st_keys_, keys, n * t_.K, hipMemcpyHostToDevice, stream_)) != hipSuccess ||
+      (e = hipMemcpyAsync(st_vals_, values, n * t_.V, hipMemcpyHostToDevice, stream_)) != hipSuccess)
+    return fail(FM_ERR_TRANSFER, hip_msg("H2D pairs", e));
+  if ((st = run_insert(st_keys_, st_vals_, n)) != FM_OK) return st;
+  Counters c;
+  if ((st = read_counters(&c)) != FM_OK) return st;
+  if ((st = check_insert_status(c)) != FM_OK) return st;
+  len_ += c.count;
+  *out_new = c.count;
+  return FM_OK;
+}
+
+fm_status GpuBackend::bulk_remove(const void* keys, uint64_t n, uint64_t* out_removed) {
+  if (n == 0) {  // gpu.rs:276-278
+    *out_removed = 0;
+    return FM_OK;
+  }
+  fm_status st;
+  if ((st = use_device()) != FM_OK) return st;
+  if ((st = ensure_staging(n * t_.K, 0, 0, 0)) != FM_OK) return st;
+  hipError_t e = hipMemcpyAsync(st_keys_, keys, n * t_.K, hipMemcpyHostToDevice, stream_);
+  if (e != hipSuccess) return fail(FM_ERR_TRANSFER, hip_msg("H2D keys", e));
+  uint64_t removed = 0;
+  if ((st = run_remove(st_keys_, n, &removed)) != FM_OK) return st;
+  len_ -= std::min(len_, removed);
+  *out_removed = removed;
+  return FM_OK;
+}
+
+fm_status GpuBackend::clear() {
+  fm_status st;
+  if ((st = use_device()) != FM_OK) return st;
+  hipLaunchKernelGGL(dev::k_clear, dim3(grid_for(t_.cap, 256)), dim3(256), 0, stream_, t_);
+  if ((st = check_launch(hipGetLastError(), "k_clear")) != FM_OK) return st;
+  hipError_t e = hipStreamSynchronize(stream_);
+  if (e != hipSuccess) return fail(FM_ERR_KERNEL_LAUNCH, hip_msg("clear", e));
+  len_ = 0;
+  return FM_OK;
+}
+
+fm_status GpuBackend::export_slots(uint32_t* flags, void* keys, void* values) {
+  fm_status st;
+  if ((st = use_device()) != FM_OK) return st;
+  hipError_t e;
+  if ((e = hipMemcpyAsync(flags, t_.flags, t_.cap * 4, hipMemcpyDeviceToHost, stream_)) != hipSuccess ||
+      (keys && (e = hipMemcpyAsync(keys, t_.keys, t_.cap * t_.K, hipMemcpyDeviceToHost, stream_)) != hipSuccess) ||
+      (values && (e = hipMemcpyAsync(values, t_.vals, t_.cap * t_.V, hipMemcpyDeviceToHost, stream_)) != hipSuccess) ||
+      (e = hipStreamSynchronize(stream_)) != hipSuccess)
+    return fail(FM_ERR_TRANSFER, hip_msg("export", e));
+  // Stale bytes of non-OCCUPIED slots are unobservable; zero them so that
+  // exports compare bytewise.
+  for (uint64_t i = 0; i < t_.cap; ++i) {
+    if ((flags[i] & 0xFu) == dev::kOccupied) continue;
+    if (keys) std::memset(static_cast<uint8_t*>(keys) + i * t_.K, 0, t_.K);
+    if (values) std::memset(static_cast<uint8_t*>(values) + i * t_.V, 0, t_.V);
+  }
+  return FM_OK;
+}
+
+fm_status GpuBackend::import_slots(const uint32_t* flags, const void* keys, const void* values, uint64_t len) {
+  fm_status st;
+  if ((st = use_device()) != FM_OK) return st;
+  hipError_t e;
+  if ((e = hipMemcpyAsync(t_.flags, flags, t_.cap * 4, hipMemcpyHostToDevice, stream_)) != hipSuccess ||
+      (e = hipMemcpyAsync(t_.keys, keys, t_.cap * t_.K, hipMemcpyHostToDevice, stream_)) != hipSuccess ||
+      (e = hipMemcpyAsync(t_.vals, values, t_.cap * t_.V, hipMemcpyHostToDevice, stream_)) != hipSuccess)
+    return fail(FM_ERR_TRANSFER, hip_msg("import", e));
+  hipLaunchKernelGGL(dev::k_rebuild_bits, dim3(grid_for(t_.cap, 256)), dim3(256), 0, stream_, t_);
+  if ((st = check_launch(hipGetLastError(), "k_rebuild_bits")) != FM_OK) return st;
+  if ((e = hipStreamSynchronize(stream_)) != hipSuccess) return fail(FM_ERR_TRANSFER, hip_msg("import", e));
+  len_ = len;
+  return FM_OK;
+}
+
+// ---- device-resident API ---------------------------------------------------
+
+fm_status GpuBackend::alloc_device(uint64_t bytes, void** out) {
+  fm_status st;
+  if ((st = use_device()) != FM_OK) return st;
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, std::max<uint64_t>(bytes, 1));
+  if (e != hipSuccess) return fail(FM_ERR_GPU_ALLOC, hip_msg("hipMalloc", e));
+  if ((e = hipMemsetAsync(p, 0, std::max<uint64_t>(bytes, 1), stream_)) != hipSuccess ||
+      (e = hipStreamSynchronize(stream_)) != hipSuccess) {
+    (void)hipFree(p);
+    return fail(FM_ERR_GPU_ALLOC, hip_msg("zero fill", e));
+  }
+  *out = p;
+  return FM_OK;
+}
+
+fm_status GpuBackend::free_device(void* p) {
+  fm_status st;
+  if ((st = use_device()) != FM_OK) return st;
+  (void)hipStreamSynchronize(stream_);
+  hipError_t e = hipFree(p);
+  return e == hipSuccess ? FM_OK : fail(FM_ERR_GPU_ALLOC, hip_msg("hipFree", e));
+}
+
+fm_status GpuBackend::upload(const void* host, uint64_t bytes, void** out) {
+  fm_status st;
+  if ((st = alloc_device(bytes, out)) != FM_OK) return st;
+  hipError_t e = hipMemcpyAsync(*out, host, bytes, hipMemcpyHostToDevice, stream_);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream_);
+  if (e != hipSuccess) return fail(FM_ERR_TRANSFER, hip_msg("upload", e));
+  return FM_OK;
+}
+
+fm_status GpuBackend::download(const void* dptr, uint64_t bytes, void* host) {
+  fm_status st;
+  if ((st = use_device()) != FM_OK) return st;
+  hipError_t e = hipMemcpyAsync(host, dptr, bytes, hipMemcpyDeviceToHost, stream_);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream_);
+  return e == hipSuccess ? FM_OK : fail(FM_ERR_TRANSFER, hip_msg("download", e));
+}
+
+fm_status GpuBackend::bulk_get_device(const void* d_keys, uint64_t n, void* d_out, uint8_t* d_found) {
+  if (n == 0) return FM_OK;
+  fm_status st;
+  if ((st = use_device()) != FM_OK) return st;
+  if ((st = check_device_ptr(d_keys, vec_align(t_.K), "keys")) != FM_OK ||
+      (st = check_device_ptr(d_out, vec_align(t_.V), "out_values")) != FM_OK)
+    return st;
+  return run_get(static_cast<const uint8_t*>(d_keys), n, static_cast<uint8_t*>(d_out), d_found);
+}
+
+fm_status GpuBackend::bulk_insert_device(const void* d_keys, const void* d_vals, uint64_t n, uint64_t* out_new,
+                                         bool counted) {
+  if (n == 0) {
+    if (out_new) *out_new = 0;
+    return FM_OK;
+  }
+  if (counted && len_ + n > t_.cap) {  // gpu.rs:551-559
+    set_table_full(len_, t_.cap);
+    return FM_ERR_TABLE_FULL;
+  }
+  fm_status st;
+  if ((st = use_device()) != FM_OK) return st;
+  if ((st = check_device_ptr(d_keys, vec_align(t_.K), "keys")) != FM_OK ||
+      (st = check_device_ptr(d_vals, vec_align(t_.V), "values")) != FM_OK)
+    return st;
+  if ((st = run_insert(static_cast<const uint8_t*>(d_keys), static_cast<const uint8_t*>(d_vals), n)) != FM_OK)
+    return st;
+  if (!counted) {  // gpu.rs:670: assumes every key new
+    len_ += n;
+    return FM_OK;
+  }
+  Counters c;
+  if ((st = read_counters(&c)) != FM_OK) return st;
+  if ((st = check_insert_status(c)) != FM_OK) return st;
+  len_ += c.count;
+  if (out_new) *out_new = c.count;
+  return FM_OK;
+}
+
+fm_status GpuBackend::bulk_remove_device(const void* d_keys, uint64_t n, uint64_t* out_removed) {
+  if (n == 0) {
+    if (out_removed) *out_removed = 0;
+    return FM_OK;
+  }
+  fm_status st;
+  if ((st = use_device()) != FM_OK) return st;
+  if ((st = check_device_ptr(d_keys, vec_align(t_.K), "keys")) != FM_OK) return st;
+  uint64_t removed = 0;
+  if ((st = run_remove(static_cast<const uint8_t*>(d_keys), n, &removed)) != FM_OK) return st;
+  len_ -= std::min(len_, removed);
+  if (out_removed) *out_removed = removed;
+  return FM_OK;
+}
+
+fm_status GpuBackend::recount(uint64_t* out) {
+  fm_status st;
+  if ((st = use_device()) != FM_OK) return st;
+  if ((st = zero_counters()) != FM_OK) return st;
+  hipLaunchKernelGGL(dev::k_count, dim3(grid_for(t_.cap, 256)), dim3(256), 0, stream_, t_, ctr_);
+  if ((st = check_launch(hipGetLastError(), "k_count")) != FM_OK) return st;
+  Counters c;
+  if ((st = read_counters(&c)) != FM_OK) return st;
+  len_ = c.count;
+  *out = c.count;
+  return FM_OK;
+}
+
+fm_status GpuBackend::synchronize() {
+  fm_status st;
+  if ((st = use_device()) != FM_OK) return st;
+  hipError_t e = hipStreamSynchronize(stream_);
+  return e == hipSuccess ? FM_OK : fail(FM_ERR_KERNEL_LAUNCH, hip_msg("synchronize", e));
+}
+
+}  // namespace
+
+std::unique_ptr<Backend> make_gpu_backend(const fm_builder& b, fm_status* st, std::string* err) {
+  std::unique_ptr<GpuBackend> g(new GpuBackend());
+  *st = g->init(b, err);
+  if (*st != FM_OK) return nullptr;
+  return g;
+}
+
+}  // namespace fm

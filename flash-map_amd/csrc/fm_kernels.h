@@ -1,0 +1,614 @@
+// fm_kernels.h -- gfx950 device code for FlashMap bulk ops.
+//
+// Table layout in HBM (structure of arrays, one set per map):
+//   flags[cap]      u32   dist<<4 | state  (EMPTY 0, OCCUPIED 1, TOMBSTONE 2;
+//                         the same word as src/kernels.cu:13-30)
+//   keys[cap*K]     bytes
+//   vals[cap*V]     bytes
+//   freebits[cap/64]  u64  bit set <=> slot is EMPTY or TOMBSTONE
+//   emptybits[cap/64] u64  bit set <=> slot is EMPTY
+//   owner[cap]      u32   per-slot scratch for remove conflict detection,
+//                         0xFFFFFFFF between ops
+//
+// Exactness contract: every op produces exactly the logical slot array that
+// the reference's CPU path (src/rayon_cpu.rs) produces when it applies the
+// batch one key at a time in index order.
+//
+//  * get: read-only; any schedule that takes the first match before the first
+//    terminator is exact (rayon_cpu.rs:86-121).
+//  * insert: a sequential insert touches only slots h..e, where e is at or
+//    before the first free slot at/after its home h (walks move forward one
+//    slot at a time; an eviction restarts the carried key at the next slot,
+//    rayon_cpu.rs:211-216).  A "dry" parking pass in which every batch key
+//    claims the first free bit at/after its home yields, order-independently,
+//    a superset of the slots the real inserts consume.  Slots still free after
+//    it ("barriers") therefore stay free through the whole sequential batch,
+//    and cut the ring into closures that no sequential walk crosses.  Keys
+//    are grouped by closure (stable radix sort on the closure's barrier) and
+//    each closure is replayed serially in batch order by one lane.
+//  * remove: removes never create EMPTY or OCCUPIED slots.  Each key's first
+//    event (match / terminator) on the pre-batch table is final unless an
+//    earlier key of the batch tombstones that very slot (a duplicate key, or
+//    the terminator of a later key).  Such conflicts are detected with a
+//    per-slot atomicMin of the matching key index; conflicting batches are
+//    replayed exactly, grouped by EMPTY-delimited runs.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fm_hash.h"
+
+namespace fm {
+namespace dev {
+
+constexpr uint32_t kEmpty = 0u;
+constexpr uint32_t kOccupied = 1u;
+constexpr uint32_t kTombstone = 2u;
+constexpr uint32_t kMaxGenericSize = 256u;  // FM_MAX_KV_SIZE, kernels.cu:32
+constexpr uint32_t kNoOwner = 0xFFFFFFFFu;
+
+// Status bits written by kernels.
+constexpr uint32_t kStatusDryFail = 1u;   // dry pass found no free slot
+constexpr uint32_t kStatusConflict = 2u;  // remove batch needs exact replay
+constexpr uint32_t kStatusBudget = 4u;    // an insert walk exceeded cap+2 steps
+
+// Event kinds of a probe (get / remove).
+constexpr uint32_t kEvNone = 0u;      // probed all cap slots
+constexpr uint32_t kEvMatch = 1u;     // OCCUPIED, equal key
+constexpr uint32_t kEvTermOcc = 2u;   // OCCUPIED with probe > dist (RH exit)
+constexpr uint32_t kEvTermEmpty = 3u; // EMPTY
+
+struct Table {
+  uint8_t* keys;
+  uint8_t* vals;
+  uint32_t* flags;
+  unsigned long long* freebits;
+  unsigned long long* emptybits;
+  uint32_t* owner;
+  uint64_t cap;
+  uint64_t mask;
+  uint32_t K;
+  uint32_t V;
+  uint32_t mode;
+};
+
+struct Counters {
+  uint32_t status;
+  uint32_t pad;
+  unsigned long long count;
+};
+
+__device__ __forceinline__ uint32_t make_flag(uint32_t state, uint64_t dist) {
+  return (static_cast<uint32_t>(dist) << 4) | (state & 0xFu);
+}
+
+// ---------------------------------------------------------------------------
+// Fixed-size byte blobs held in registers (keys and values).
+// N > 0: N % 4 == 0, vector loads of the widest width that divides N
+// (buffers are aligned to that width).  N == 0: runtime size, bytes.
+// ---------------------------------------------------------------------------
+template <int N>
+struct Blob {
+  static_assert(N % 4 == 0, "static blob sizes are multiples of 4");
+  static constexpr int W = N / 4;
+  uint32_t w[W];
+
+  __device__ __forceinline__ void load(const uint8_t* __restrict__ p, uint32_t) {
+    if constexpr (N % 16 == 0) {
+#pragma unroll
+      for (int i = 0; i < N / 16; ++i) {
+        const uint4 v = reinterpret_cast<const uint4*>(p)[i];
+        w[4 * i] = v.x; w[4 * i + 1] = v.y; w[4 * i + 2] = v.z; w[4 * i + 3] = v.w;
+      }
+    } else if constexpr (N % 8 == 0) {
+#pragma unroll
+      for (int i = 0; i < N / 8; ++i) {
+        const uint2 v = reinterpret_cast<const uint2*>(p)[i];
+        w[2 * i] = v.x; w[2 * i + 1] = v.y;
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < W; ++i) w[i] = reinterpret_cast<const uint32_t*>(p)[i];
+    }
+  }
+  __device__ __forceinline__ void store(uint8_t* __restrict__ p, uint32_t) const {
+    if constexpr (N % 16 == 0) {
+#pragma unroll
+      for (int i = 0; i < N / 16; ++i)
+        reinterpret_cast<uint4*>(p)[i] = make_uint4(w[4 * i], w[4 * i + 1], w[4 * i + 2], w[4 * i + 3]);
+    } else if constexpr (N % 8 == 0) {
+#pragma unroll
+      for (int i = 0; i < N / 8; ++i) reinterpret_cast<uint2*>(p)[i] = make_uint2(w[2 * i], w[2 * i + 1]);
+    } else {
+#pragma unroll
+      for (int i = 0; i < W; ++i) reinterpret_cast<uint32_t*>(p)[i] = w[i];
+    }
+  }
+  __device__ __forceinline__ bool eq(const uint8_t* __restrict__ p, uint32_t n) const {
+    Blob<N> o;
+    o.load(p, n);
+    uint32_t diff = 0;
+#pragma unroll
+    for (int i = 0; i < W; ++i) diff |= w[i] ^ o.w[i];
+    return diff == 0;
+  }
+  __device__ __forceinline__ uint64_t hash(uint32_t, uint32_t mode) const {
+    if (mode == 0) {
+      if constexpr (N >= 8) return static_cast<uint64_t>(w[0]) | (static_cast<uint64_t>(w[1]) << 32);
+      else return static_cast<uint64_t>(w[0]);
+    }
+    return murmur3_words(w, N);
+  }
+};
+
+template <>
+struct Blob<0> {
+  uint8_t b[kMaxGenericSize];
+  __device__ __forceinline__ void load(const uint8_t* __restrict__ p, uint32_t n) {
+    for (uint32_t i = 0; i < n; ++i) b[i] = p[i];
+  }
+  __device__ __forceinline__ void store(uint8_t* __restrict__ p, uint32_t n) const {
+    for (uint32_t i = 0; i < n; ++i) p[i] = b[i];
+  }
+  __device__ __forceinline__ bool eq(const uint8_t* __restrict__ p, uint32_t n) const {
+    for (uint32_t i = 0; i < n; ++i)
+      if (b[i] != p[i]) return false;
+    return true;
+  }
+  __device__ __forceinline__ uint64_t hash(uint32_t n, uint32_t mode) const { return hash_key(b, n, mode); }
+};
+
+// ---------------------------------------------------------------------------
+// Bitmap helpers.  Bits at positions >= cap in the last word are always 0.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ unsigned long long load_word(const unsigned long long* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// First set bit at or after `start`, cyclically; cap if there is none.
+__device__ inline uint64_t bm_next_set(const unsigned long long* bm, uint64_t start, uint64_t cap) {
+  const uint64_t words = (cap + 63) >> 6;
+  uint64_t w = start >> 6;
+  unsigned long long word = bm[w] & (~0ull << (start & 63));
+  for (uint64_t it = 0; it <= words; ++it) {
+    if (word) return (w << 6) + static_cast<uint64_t>(__ffsll(word) - 1);
+    w = (w + 1 == words) ? 0 : w + 1;
+    word = bm[w];
+  }
+  return cap;
+}
+
+// Dry-pass claim: atomically clear the first set bit at/after `start`
+// (cyclically) and return its position.  Linearisable: bits are only ever
+// cleared while the pass runs, so the bit won is the first free one at the
+// instant of the successful atomic.  `budget` bounds the loop.
+__device__ inline bool bm_claim(unsigned long long* bm, uint64_t start, uint64_t cap, uint64_t budget,
+                                uint64_t* out) {
+  const uint64_t words = (cap + 63) >> 6;
+  uint64_t w = start >> 6;
+  unsigned long long m = ~0ull << (start & 63);
+  for (uint64_t it = 0; it < budget; ++it) {
+    unsigned long long cur = load_word(&bm[w]) & m;
+    while (cur) {
+      const int b = __ffsll(cur) - 1;
+      const unsigned long long bit = 1ull << b;
+      const unsigned long long old = atomicAnd(&bm[w], ~bit);
+      if (old & bit) {
+        *out = (w << 6) + static_cast<uint64_t>(b);
+        return true;
+      }
+      cur = old & m;
+      if (++it >= budget) return false;
+    }
+    m = ~0ull;
+    w = (w + 1 == words) ? 0 : w + 1;
+  }
+  return false;
+}
+
+// ---------------------------------------------------------------------------
+// Cooperative probe by a group of G lanes (G power of two, 1..64): lane l of
+// the group inspects slot home+base+l; wave-wide ballots find the first
+// event (match or terminator) of each group.  Semantics: rayon_cpu.rs:91-120
+// (probe at most cap slots; EMPTY ends; OCCUPIED with p > dist ends; an
+// equal key at OCCUPIED matches; tombstones and other keys continue).
+// Home filter: an equal key sits at displacement p == dist (stored distances
+// equal the displacement mod cap), so keys are compared only there.
+// ---------------------------------------------------------------------------
+template <int KS, int G>
+__device__ __forceinline__ void group_probe(const Table& t, const Blob<KS>& q, uint32_t K, uint64_t home,
+                                            uint32_t gl, uint32_t gshift, uint32_t* kind, uint64_t* slot) {
+  constexpr unsigned long long kGroupMask = (G == 64) ? ~0ull : ((1ull << G) - 1ull);
+  for (uint64_t base = 0;; base += G) {
+    const uint64_t p = base + gl;
+    const bool valid = p <= t.mask;
+    const uint64_t idx = (home + p) & t.mask;
+    const uint32_t f = valid ? t.flags[idx] : 0u;
+    const uint32_t st = f & 0xFu;
+    const uint64_t d = f >> 4;
+    const bool occ = st == kOccupied;
+    bool match = false;
+    if (valid && occ && d == p) match = q.eq(t.keys + idx * K, K);
+    const bool term = !valid || st == kEmpty || (occ && p > d);
+    const uint32_t my_kind = match ? kEvMatch : (!valid ? kEvNone : (st == kEmpty ? kEvTermEmpty : kEvTermOcc));
+    if constexpr (G == 1) {
+      if (match || term) {
+        *kind = my_kind;
+        *slot = idx;
+        return;
+      }
+    } else {
+      const unsigned long long ev = ((__ballot(match) | __ballot(term)) >> gshift) & kGroupMask;
+      if (ev) {
+        const int first = __ffsll(ev) - 1;
+        *kind = static_cast<uint32_t>(__shfl(static_cast<int>(my_kind), first, G));
+        *slot = static_cast<uint64_t>(__shfl(static_cast<long long>(idx), first, G));
+        return;
+      }
+    }
+  }
+}
+
+// Group-cooperative copy of V bytes (src == nullptr writes zeros).
+template <int VS, int G>
+__device__ __forceinline__ void group_copy(uint8_t* __restrict__ dst, const uint8_t* __restrict__ src, uint32_t V,
+                                           uint32_t gl) {
+  if constexpr (VS != 0 && VS % 16 == 0) {
+#pragma unroll
+    for (uint32_t c = gl; c < VS / 16; c += G)
+      reinterpret_cast<uint4*>(dst)[c] = src ? reinterpret_cast<const uint4*>(src)[c] : make_uint4(0, 0, 0, 0);
+  } else if constexpr (VS != 0 && VS % 8 == 0) {
+#pragma unroll
+    for (uint32_t c = gl; c < VS / 8; c += G)
+      reinterpret_cast<uint2*>(dst)[c] = src ? reinterpret_cast<const uint2*>(src)[c] : make_uint2(0, 0);
+  } else if constexpr (VS != 0 && VS % 4 == 0) {
+#pragma unroll
+    for (uint32_t c = gl; c < VS / 4; c += G)
+      reinterpret_cast<uint32_t*>(dst)[c] = src ? reinterpret_cast<const uint32_t*>(src)[c] : 0u;
+  } else {
+    for (uint32_t c = gl; c < V; c += G) dst[c] = src ? src[c] : 0;
+  }
+}
+
+// Adds per-thread counts into *dst with one atomic per wave.
+__device__ __forceinline__ void wave_add(unsigned long long* dst, uint32_t v) {
+  unsigned long long sum = v;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off, 64);
+  if ((threadIdx.x & 63) == 0 && sum) atomicAdd(dst, sum);
+}
+
+// ---------------------------------------------------------------------------
+// bulk_get (rayon_cpu.rs:82-126).  One group of G lanes per query.
+// out_found may be null (bulk_get_values_device, kernels.cu:481-556).
+// ---------------------------------------------------------------------------
+template <int KS, int VS, int G>
+__global__ void __launch_bounds__(1024)
+k_get(Table t, const uint8_t* __restrict__ qkeys, uint64_t n, uint8_t* __restrict__ out_vals,
+      uint8_t* __restrict__ out_found) {
+  const uint32_t K = KS ? KS : t.K;
+  const uint32_t V = VS ? VS : t.V;
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t gl = lane & (G - 1);
+  const uint32_t gshift = lane & ~(G - 1u);
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x / G;
+  for (uint64_t qi = (static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / G; qi < n; qi += stride) {
+    Blob<KS> q;
+    q.load(qkeys + qi * K, K);
+    const uint64_t home = q.hash(K, t.mode) & t.mask;
+    uint32_t kind;
+    uint64_t slot;
+    group_probe<KS, G>(t, q, K, home, gl, gshift, &kind, &slot);
+    const bool hit = kind == kEvMatch;
+    group_copy<VS, G>(out_vals + qi * V, hit ? t.vals + slot * V : nullptr, V, gl);
+    if (out_found && gl == 0) out_found[qi] = hit ? 1 : 0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// bulk_insert, exact (see header).
+// ---------------------------------------------------------------------------
+
+// Pass 1: hash every key and dry-claim one free bit of Q (a copy of
+// freebits) at/after its home.
+template <int KS>
+__global__ void __launch_bounds__(1024)
+k_ins_dry(Table t, const uint8_t* __restrict__ in_keys, uint64_t n, unsigned long long* __restrict__ Q,
+          uint32_t* __restrict__ home_out, Counters* ctr) {
+  const uint32_t K = KS ? KS : t.K;
+  const uint64_t budget = ((t.cap + 63) >> 6) + n + 2;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    Blob<KS> k;
+    k.load(in_keys + i * K, K);
+    const uint64_t h = k.hash(K, t.mode) & t.mask;
+    home_out[i] = static_cast<uint32_t>(h);
+    uint64_t s;
+    if (!bm_claim(Q, h, t.cap, budget, &s)) atomicOr(&ctr->status, kStatusDryFail);
+  }
+}
+
+// Pass 2: closure id = first barrier (bit still set in Q) at/after home.
+// With no barrier (the batch fills every free slot) or a failed dry pass,
+// every key gets the same id and one lane replays the whole batch in order.
+__global__ void __launch_bounds__(1024)
+k_ins_closure(uint64_t cap, const unsigned long long* __restrict__ Q, const uint32_t* __restrict__ home, uint64_t n,
+              uint32_t* __restrict__ cid, uint32_t* __restrict__ idx, const Counters* ctr) {
+  const bool failed = (ctr->status & kStatusDryFail) != 0;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    cid[i] = failed ? static_cast<uint32_t>(cap) : static_cast<uint32_t>(bm_next_set(Q, home[i], cap));
+    idx[i] = static_cast<uint32_t>(i);
+  }
+}
+
+// Sequential insert of one pair into the table (rayon_cpu.rs:157-262 with
+// the batch applied in order).  Returns 1 if a free slot was claimed.
+// Every iteration advances the walk by one slot (++p, or an eviction that
+// restarts the carried key at the next slot), so with a free slot anywhere
+// the walk ends within cap steps.  Only a table with no free slot at all
+// (reachable through the uncounted device insert) can exceed the budget; the
+// walk then stops and flags kStatusBudget instead of circling the ring.
+template <int KS, int VS>
+__device__ inline uint32_t insert_one(const Table& t, const uint8_t* __restrict__ kin, const uint8_t* __restrict__ vin,
+                                      uint32_t K, uint32_t V, Counters* ctr) {
+  Blob<KS> ck;
+  Blob<VS> cv;
+  ck.load(kin, K);
+  cv.load(vin, V);
+  uint64_t home = ck.hash(K, t.mode) & t.mask;
+  uint64_t p = 0;
+  for (uint64_t step = 0;; ++step) {
+    if (p > t.mask) return 0;  // carried pair dropped (rayon_cpu.rs:167-169)
+    if (step > t.cap + 1) {
+      atomicOr(&ctr->status, kStatusBudget);
+      return 0;
+    }
+    const uint64_t idx = (home + p) & t.mask;
+    const uint32_t f = t.flags[idx];
+    const uint32_t st = f & 0xFu;
+    const uint64_t d = f >> 4;
+    if (st == kOccupied) {
+      // equal keys share a home, hence sit at displacement p == d
+      if (d == p && ck.eq(t.keys + idx * K, K)) {
+        cv.store(t.vals + idx * V, V);  // update in place, not new
+        return 0;
+      }
+      if (p > d) {  // Robin Hood eviction: swap and carry the resident on
+        Blob<KS> ek;
+        Blob<VS> ev;
+        ek.load(t.keys + idx * K, K);
+        ev.load(t.vals + idx * V, V);
+        ck.store(t.keys + idx * K, K);
+        cv.store(t.vals + idx * V, V);
+        t.flags[idx] = make_flag(kOccupied, p);
+        ck = ek;
+        cv = ev;
+        home = ck.hash(K, t.mode) & t.mask;
+        p = d + 1;
+        continue;
+      }
+      ++p;
+      continue;
+    }
+    // EMPTY or TOMBSTONE: claim.
+    ck.store(t.keys + idx * K, K);
+    cv.store(t.vals + idx * V, V);
+    t.flags[idx] = make_flag(kOccupied, p);
+    const unsigned long long bit = 1ull << (idx & 63);
+    atomicAnd(&t.freebits[idx >> 6], ~bit);
+    atomicAnd(&t.emptybits[idx >> 6], ~bit);
+    return 1;
+  }
+}
+
+// Pass 3: one lane per closure replays its keys in batch order.
+template <int KS, int VS>
+__global__ void __launch_bounds__(1024)
+k_ins_replay(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __restrict__ in_vals, uint64_t n,
+             const uint32_t* __restrict__ cid_s, const uint32_t* __restrict__ idx_s, Counters* ctr) {
+  const uint32_t K = KS ? KS : t.K;
+  const uint32_t V = VS ? VS : t.V;
+  for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * blockDim.x;; base += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    if (base >= n) break;
+    const uint64_t s = base + threadIdx.x;
+    uint32_t claimed = 0;
+    if (s < n && (s == 0 || cid_s[s] != cid_s[s - 1])) {
+      const uint32_t c = cid_s[s];
+      for (uint64_t u = s; u < n && cid_s[u] == c; ++u) {
+        const uint64_t i = idx_s[u];
+        claimed += insert_one<KS, VS>(t, in_keys + i * K, in_vals + i * V, K, V, ctr);
+      }
+    }
+    wave_add(&ctr->count, claimed);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// bulk_remove, exact (see header).
+// ---------------------------------------------------------------------------
+
+// Pass 1: first event of each key on the pre-batch table; matches register
+// their index in owner[slot] with atomicMin (the earliest remover wins).
+template <int KS, int G>
+__global__ void __launch_bounds__(1024)
+k_rem_probe(Table t, const uint8_t* __restrict__ qkeys, uint64_t n, unsigned long long* __restrict__ ev) {
+  const uint32_t K = KS ? KS : t.K;
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t gl = lane & (G - 1);
+  const uint32_t gshift = lane & ~(G - 1u);
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x / G;
+  for (uint64_t qi = (static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / G; qi < n; qi += stride) {
+    Blob<KS> q;
+    q.load(qkeys + qi * K, K);
+    const uint64_t home = q.hash(K, t.mode) & t.mask;
+    uint32_t kind;
+    uint64_t slot;
+    group_probe<KS, G>(t, q, K, home, gl, gshift, &kind, &slot);
+    if (gl == 0) {
+      ev[qi] = (static_cast<unsigned long long>(kind) << 32) | slot;
+      if (kind == kEvMatch) atomicMin(&t.owner[slot], static_cast<uint32_t>(qi));
+    }
+  }
+}
+
+// Pass 2: conflict detection.  Key i keeps its pre-batch event unless an
+// earlier key tombstones that slot: a duplicate match (owner != i) or an
+// occupied terminator removed by an earlier key (owner < i).
+__global__ void __launch_bounds__(1024)
+k_rem_check(Table t, const unsigned long long* __restrict__ ev, uint64_t n, uint8_t* __restrict__ action,
+            Counters* ctr) {
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    const uint32_t kind = static_cast<uint32_t>(ev[i] >> 32);
+    const uint64_t slot = ev[i] & 0xFFFFFFFFull;
+    uint8_t a = 0;
+    bool conflict = false;
+    if (kind == kEvMatch) {
+      const uint32_t o = t.owner[slot];
+      a = (o == static_cast<uint32_t>(i)) ? 1 : 0;
+      conflict = !a;
+    } else if (kind == kEvTermOcc) {
+      conflict = t.owner[slot] < static_cast<uint32_t>(i);
+    }
+    action[i] = a;
+    if (conflict) atomicOr(&ctr->status, kStatusConflict);
+  }
+}
+
+// Pass 3: apply (only when the batch had no conflict) and reset owner[].
+__global__ void __launch_bounds__(1024)
+k_rem_apply(Table t, const unsigned long long* __restrict__ ev, const uint8_t* __restrict__ action, uint64_t n,
+            Counters* ctr) {
+  const bool conflict = (ctr->status & kStatusConflict) != 0;
+  for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * blockDim.x;; base += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    if (base >= n) break;
+    const uint64_t i = base + threadIdx.x;
+    uint32_t removed = 0;
+    if (i < n && static_cast<uint32_t>(ev[i] >> 32) == kEvMatch) {
+      const uint64_t slot = ev[i] & 0xFFFFFFFFull;
+      if (!conflict && action[i]) {
+        t.flags[slot] = kTombstone;  // raw 2: distance bits cleared
+        atomicOr(&t.freebits[slot >> 6], 1ull << (slot & 63));
+        removed = 1;
+      }
+      t.owner[slot] = kNoOwner;
+    }
+    wave_add(&ctr->count, removed);
+  }
+}
+
+// Exact replay, step 1: run id = first EMPTY slot at/after home (removes
+// never cross or create EMPTY slots).  No EMPTY slot: one run.
+template <int KS>
+__global__ void __launch_bounds__(1024)
+k_rem_runs(Table t, const uint8_t* __restrict__ qkeys, uint64_t n, uint32_t* __restrict__ cid,
+           uint32_t* __restrict__ idx) {
+  const uint32_t K = KS ? KS : t.K;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    Blob<KS> q;
+    q.load(qkeys + i * K, K);
+    const uint64_t h = q.hash(K, t.mode) & t.mask;
+    cid[i] = static_cast<uint32_t>(bm_next_set(t.emptybits, h, t.cap));
+    idx[i] = static_cast<uint32_t>(i);
+  }
+}
+
+// Sequential remove of one key (rayon_cpu.rs:276-321 applied in order).
+template <int KS>
+__device__ inline uint32_t remove_one(const Table& t, const uint8_t* __restrict__ kin, uint32_t K) {
+  Blob<KS> q;
+  q.load(kin, K);
+  const uint64_t home = q.hash(K, t.mode) & t.mask;
+  for (uint64_t p = 0; p <= t.mask; ++p) {
+    const uint64_t idx = (home + p) & t.mask;
+    const uint32_t f = t.flags[idx];
+    const uint32_t st = f & 0xFu;
+    const uint64_t d = f >> 4;
+    if (st == kEmpty) return 0;
+    if (st == kOccupied) {
+      if (p > d) return 0;
+      if (d == p && q.eq(t.keys + idx * K, K)) {
+        t.flags[idx] = kTombstone;
+        atomicOr(&t.freebits[idx >> 6], 1ull << (idx & 63));
+        return 1;
+      }
+    }
+  }
+  return 0;
+}
+
+// Exact replay, step 2: one lane per run, keys in batch order.
+template <int KS>
+__global__ void __launch_bounds__(1024)
+k_rem_replay(Table t, const uint8_t* __restrict__ qkeys, uint64_t n, const uint32_t* __restrict__ cid_s,
+             const uint32_t* __restrict__ idx_s, Counters* ctr) {
+  const uint32_t K = KS ? KS : t.K;
+  for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * blockDim.x;; base += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    if (base >= n) break;
+    const uint64_t s = base + threadIdx.x;
+    uint32_t removed = 0;
+    if (s < n && (s == 0 || cid_s[s] != cid_s[s - 1])) {
+      const uint32_t c = cid_s[s];
+      for (uint64_t u = s; u < n && cid_s[u] == c; ++u) removed += remove_one<KS>(t, qkeys + idx_s[u] * K, K);
+    }
+    wave_add(&ctr->count, removed);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Table maintenance.
+// ---------------------------------------------------------------------------
+__global__ void k_fill_u32(uint32_t* __restrict__ p, uint64_t n, uint32_t v) {
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+    p[i] = v;
+}
+
+// clear (kernels.cu:319-326): flags EMPTY; both bitmaps all-free.
+__global__ void k_clear(Table t) {
+  const uint64_t words = (t.cap + 63) >> 6;
+  const unsigned long long full = t.cap >= 64 ? ~0ull : ((1ull << t.cap) - 1ull);
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < t.cap;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    t.flags[i] = kEmpty;
+    if (i < words) {
+      t.freebits[i] = full;
+      t.emptybits[i] = full;
+    }
+  }
+}
+
+// Rebuilds both bitmaps from flags (after an import): one wave per 64 slots.
+__global__ void k_rebuild_bits(Table t) {
+  const uint64_t words = (t.cap + 63) >> 6;
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint64_t wave = (static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+  const uint64_t waves = (static_cast<uint64_t>(gridDim.x) * blockDim.x) >> 6;
+  for (uint64_t w = wave; w < words; w += waves) {
+    const uint64_t i = (w << 6) + lane;
+    const uint32_t st = i < t.cap ? (t.flags[i] & 0xFu) : kOccupied;
+    const unsigned long long fb = __ballot(st != kOccupied);
+    const unsigned long long eb = __ballot(st == kEmpty);
+    if (lane == 0) {
+      t.freebits[w] = fb;
+      t.emptybits[w] = eb;
+    }
+  }
+}
+
+// recount (kernels.cu:328-337): OCCUPIED slots, one atomic per wave.
+__global__ void k_count(Table t, Counters* ctr) {
+  for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * blockDim.x;; base += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    if (base >= t.cap) break;
+    const uint64_t i = base + threadIdx.x;
+    const bool occ = i < t.cap && (t.flags[i] & 0xFu) == kOccupied;
+    const unsigned long long b = __ballot(occ);
+    if ((threadIdx.x & 63u) == 0 && b) atomicAdd(&ctr->count, static_cast<unsigned long long>(__popcll(b)));
+  }
+}
+
+}  // namespace dev
+}  // namespace fm

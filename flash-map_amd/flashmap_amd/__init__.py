@@ -1,0 +1,500 @@
+"""Python mirror of the reference FlashMap API over the C ABI.
+
+Mirrors src/lib.rs (FlashMap, FlashMapBuilder), src/hash.rs (HashStrategy),
+src/error.rs (FlashMapError) and src/async_map.rs (AsyncFlashMap).  Keys and
+values are fixed-size byte strings: any numpy array is viewed as raw bytes,
+``n = nbytes // key_size`` (bytemuck::cast_slice, src/gpu.rs:120).
+
+The shared library ``libflashmap_amd.so`` next to this file must be built
+(``make -C flash-map_amd``); importing fails loudly when it is missing.
+"""
+from __future__ import annotations
+
+import asyncio
+import ctypes
+import os
+import threading
+from typing import List, Optional, Sequence, Tuple
+
+import numpy as np
+
+__all__ = [
+    "FlashMap", "FlashMapBuilder", "HashStrategy", "Backend", "FlashMapError",
+    "TableFull", "GpuRequired", "AsyncFlashMap", "lib", "LIB_PATH",
+]
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libflashmap_amd.so")
+
+# fm_status (include/flashmap.h) == FlashMapError variants (src/error.rs:4-39)
+OK, DEVICE_INIT, GPU_ALLOC, KERNEL_LAUNCH, TRANSFER, TABLE_FULL, ZERO_CAPACITY, \
+    NO_BACKEND, GPU_REQUIRED, LOCK_POISONED, ASYNC_JOIN, INVALID_ARGUMENT = range(12)
+
+
+class HashStrategy:
+    """src/hash.rs:3-27 (numbering = HashStrategy::to_mode)."""
+    IDENTITY = 0
+    MURMUR3 = 1
+
+
+class Backend:
+    """enum FlashMapBackend (src/lib.rs:98-103)."""
+    NONE = 0
+    GPU = 1
+    CPU = 2
+
+
+class FlashMapError(Exception):
+    """src/error.rs:4-39; ``status`` is the fm_status code."""
+
+    def __init__(self, status: int, message: str):
+        super().__init__(message)
+        self.status = status
+
+
+class TableFull(FlashMapError):
+    """FlashMapError::TableFull (error.rs:17-24); load_factor in percent."""
+
+    def __init__(self, status, message, occupied, capacity, load_factor):
+        super().__init__(status, message)
+        self.occupied = occupied
+        self.capacity = capacity
+        self.load_factor = load_factor
+
+
+class GpuRequired(FlashMapError):
+    """FlashMapError::GpuRequired (error.rs:32-33)."""
+
+
+class _Builder(ctypes.Structure):
+    _fields_ = [
+        ("capacity", ctypes.c_uint64),
+        ("key_size", ctypes.c_uint32),
+        ("value_size", ctypes.c_uint32),
+        ("hash_strategy", ctypes.c_uint32),
+        ("device_id", ctypes.c_int32),
+        ("force_cpu", ctypes.c_uint32),
+        ("require_gpu", ctypes.c_uint32),
+        ("quiet", ctypes.c_uint32),
+    ]
+
+
+class Tuning(ctypes.Structure):
+    _fields_ = [
+        ("get_group", ctypes.c_uint32),
+        ("get_block", ctypes.c_uint32),
+        ("remove_group", ctypes.c_uint32),
+        ("insert_block", ctypes.c_uint32),
+    ]
+
+
+_LIB = None
+_LIB_LOCK = threading.Lock()
+
+
+def lib() -> ctypes.CDLL:
+    """Loads libflashmap_amd.so (raises if it was not built)."""
+    global _LIB
+    with _LIB_LOCK:
+        if _LIB is not None:
+            return _LIB
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with `make -C flash-map_amd` "
+                              "(or __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, u64, u32, i32 = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int32
+        pu64 = ctypes.POINTER(ctypes.c_uint64)
+        sig = {
+            "fm_builder_init": (None, [ctypes.POINTER(_Builder), u64, u32, u32]),
+            "fm_build": (ctypes.c_int, [ctypes.POINTER(_Builder), ctypes.POINTER(vp)]),
+            "fm_with_capacity": (ctypes.c_int, [u64, u32, u32, ctypes.POINTER(vp)]),
+            "fm_destroy": (None, [vp]),
+            "fm_bulk_get": (ctypes.c_int, [vp, vp, u64, vp, vp]),
+            "fm_bulk_insert": (ctypes.c_int, [vp, vp, vp, u64, pu64]),
+            "fm_bulk_remove": (ctypes.c_int, [vp, vp, u64, pu64]),
+            "fm_len": (u64, [vp]),
+            "fm_capacity": (u64, [vp]),
+            "fm_load_factor": (ctypes.c_double, [vp]),
+            "fm_clear": (ctypes.c_int, [vp]),
+            "fm_backend": (ctypes.c_int, [vp]),
+            "fm_key_size": (u32, [vp]),
+            "fm_value_size": (u32, [vp]),
+            "fm_debug_string": (ctypes.c_int, [vp, ctypes.c_char_p, ctypes.c_size_t]),
+            "fm_last_error_message": (ctypes.c_size_t, [ctypes.c_char_p, ctypes.c_size_t]),
+            "fm_last_table_full": (ctypes.c_int, [pu64, pu64, ctypes.POINTER(ctypes.c_double)]),
+            "fm_device_stream": (vp, [vp]),
+            "fm_device_id": (i32, [vp]),
+            "fm_alloc_device": (ctypes.c_int, [vp, u64, ctypes.POINTER(vp)]),
+            "fm_free_device": (ctypes.c_int, [vp, vp]),
+            "fm_upload": (ctypes.c_int, [vp, vp, u64, ctypes.POINTER(vp)]),
+            "fm_download": (ctypes.c_int, [vp, vp, u64, vp]),
+            "fm_bulk_get_device": (ctypes.c_int, [vp, vp, u64, vp, vp]),
+            "fm_bulk_get_values_device": (ctypes.c_int, [vp, vp, u64, vp]),
+            "fm_bulk_insert_device": (ctypes.c_int, [vp, vp, vp, u64, pu64]),
+            "fm_bulk_insert_device_uncounted": (ctypes.c_int, [vp, vp, vp, u64]),
+            "fm_bulk_remove_device": (ctypes.c_int, [vp, vp, u64, pu64]),
+            "fm_recount": (ctypes.c_int, [vp, pu64]),
+            "fm_synchronize": (ctypes.c_int, [vp]),
+            "fm_export_slots": (ctypes.c_int, [vp, vp, vp, vp]),
+            "fm_import_slots": (ctypes.c_int, [vp, vp, vp, vp, u64]),
+            "fm_get_tuning": (ctypes.c_int, [vp, ctypes.POINTER(Tuning)]),
+            "fm_set_tuning": (ctypes.c_int, [vp, ctypes.POINTER(Tuning)]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+        return L
+
+
+def _last_error(status: int) -> FlashMapError:
+    L = lib()
+    buf = ctypes.create_string_buffer(512)
+    L.fm_last_error_message(buf, 512)
+    msg = buf.value.decode(errors="replace")
+    if status == TABLE_FULL:
+        occ, cap, lf = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_double()
+        L.fm_last_table_full(ctypes.byref(occ), ctypes.byref(cap), ctypes.byref(lf))
+        return TableFull(status, msg, occ.value, cap.value, lf.value)
+    if status == GPU_REQUIRED:
+        return GpuRequired(status, msg)
+    return FlashMapError(status, msg)
+
+
+def _check(status: int):
+    if status != OK:
+        raise _last_error(status)
+
+
+def _as_bytes(a, size: int, what: str) -> Tuple[np.ndarray, int]:
+    """Contiguous uint8 view of `a` and its element count for `size`-byte items."""
+    if isinstance(a, (bytes, bytearray, memoryview)):
+        arr = np.frombuffer(bytes(a), dtype=np.uint8)
+    elif isinstance(a, np.ndarray):
+        arr = np.ascontiguousarray(a).reshape(-1).view(np.uint8)
+    else:  # sequence of byte strings
+        arr = np.frombuffer(b"".join(bytes(x) for x in a), dtype=np.uint8) if len(a) else \
+            np.zeros(0, dtype=np.uint8)
+    if arr.size % size:
+        raise ValueError(f"{what}: {arr.size} bytes is not a multiple of {size}")
+    return arr, arr.size // size
+
+
+def _ptr(arr: np.ndarray) -> ctypes.c_void_p:
+    return ctypes.c_void_p(arr.ctypes.data if arr.size else 0)
+
+
+class DeviceBuffer:
+    """A device allocation owned by a map (CudaSlice<u8> in the reference)."""
+
+    def __init__(self, owner: "FlashMap", ptr: int, nbytes: int):
+        self.owner, self.ptr, self.nbytes = owner, ptr, nbytes
+
+    def free(self):
+        if self.ptr and self.owner._h:
+            lib().fm_free_device(self.owner._h, ctypes.c_void_p(self.ptr))
+        self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class FlashMap:
+    """FlashMap<K,V> with K and V given as byte sizes (src/lib.rs:94-351)."""
+
+    def __init__(self, handle: int, key_size: int, value_size: int):
+        self._h = ctypes.c_void_p(handle)
+        self.key_size = key_size
+        self.value_size = value_size
+
+    # -- construction (lib.rs:110-117) --
+    @staticmethod
+    def with_capacity(capacity: int, key_size: int = 8, value_size: int = 8) -> "FlashMap":
+        return FlashMapBuilder(capacity, key_size, value_size).build()
+
+    @staticmethod
+    def builder(capacity: int, key_size: int = 8, value_size: int = 8) -> "FlashMapBuilder":
+        return FlashMapBuilder(capacity, key_size, value_size)
+
+    def close(self):
+        if self._h:
+            lib().fm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- host-facing API (lib.rs:123-207) --
+    def bulk_get(self, keys) -> Tuple[np.ndarray, np.ndarray]:
+        """Returns (values: (n, V) uint8, found: (n,) bool); misses are zeroed."""
+        k, n = _as_bytes(keys, self.key_size, "keys")
+        vals = np.zeros((n, self.value_size), dtype=np.uint8)
+        found = np.zeros(n, dtype=np.uint8)
+        if n:
+            _check(lib().fm_bulk_get(self._h, _ptr(k), n, _ptr(vals), _ptr(found)))
+        return vals, found.astype(bool)
+
+    def bulk_get_options(self, keys) -> List[Optional[bytes]]:
+        """Vec<Option<V>> form of bulk_get (lib.rs:124)."""
+        vals, found = self.bulk_get(keys)
+        return [bytes(vals[i]) if found[i] else None for i in range(len(found))]
+
+    def bulk_insert(self, keys, values) -> int:
+        """Number of newly occupied slots (lib.rs:143-150)."""
+        k, n = _as_bytes(keys, self.key_size, "keys")
+        v, nv = _as_bytes(values, self.value_size, "values")
+        if n != nv:
+            raise ValueError(f"{n} keys but {nv} values")
+        out = ctypes.c_uint64(0)
+        _check(lib().fm_bulk_insert(self._h, _ptr(k), _ptr(v), n, ctypes.byref(out)))
+        return int(out.value)
+
+    def bulk_insert_pairs(self, pairs: Sequence[Tuple[bytes, bytes]]) -> int:
+        """bulk_insert(&[(K,V)]) with the tuples split as gpu.rs:212-217 does."""
+        return self.bulk_insert([p[0] for p in pairs], [p[1] for p in pairs])
+
+    def bulk_remove(self, keys) -> int:
+        k, n = _as_bytes(keys, self.key_size, "keys")
+        out = ctypes.c_uint64(0)
+        _check(lib().fm_bulk_remove(self._h, _ptr(k), n, ctypes.byref(out)))
+        return int(out.value)
+
+    def __len__(self) -> int:
+        return int(lib().fm_len(self._h))
+
+    def len(self) -> int:
+        return len(self)
+
+    def is_empty(self) -> bool:
+        return len(self) == 0
+
+    def capacity(self) -> int:
+        return int(lib().fm_capacity(self._h))
+
+    def load_factor(self) -> float:
+        return float(lib().fm_load_factor(self._h))
+
+    def clear(self):
+        _check(lib().fm_clear(self._h))
+
+    def backend(self) -> int:
+        return int(lib().fm_backend(self._h))
+
+    def __repr__(self) -> str:
+        buf = ctypes.create_string_buffer(256)
+        _check(lib().fm_debug_string(self._h, buf, 256))
+        return buf.value.decode()
+
+    # -- device-resident API (lib.rs:213-350) --
+    def device_stream(self) -> int:
+        return int(lib().fm_device_stream(self._h) or 0)
+
+    def alloc_device(self, nbytes: int) -> DeviceBuffer:
+        p = ctypes.c_void_p()
+        _check(lib().fm_alloc_device(self._h, nbytes, ctypes.byref(p)))
+        return DeviceBuffer(self, p.value or 0, nbytes)
+
+    def upload(self, data) -> DeviceBuffer:
+        arr, _ = _as_bytes(data, 1, "data")
+        p = ctypes.c_void_p()
+        _check(lib().fm_upload(self._h, _ptr(arr), arr.size, ctypes.byref(p)))
+        return DeviceBuffer(self, p.value or 0, arr.size)
+
+    upload_keys = upload
+    upload_values = upload
+
+    def download(self, buf: DeviceBuffer, nbytes: Optional[int] = None) -> np.ndarray:
+        nbytes = buf.nbytes if nbytes is None else nbytes
+        out = np.zeros(nbytes, dtype=np.uint8)
+        _check(lib().fm_download(self._h, ctypes.c_void_p(buf.ptr), nbytes, _ptr(out)))
+        return out
+
+    def bulk_get_device(self, d_keys: DeviceBuffer, count: int,
+                        d_out: DeviceBuffer, d_found: DeviceBuffer):
+        _check(lib().fm_bulk_get_device(self._h, ctypes.c_void_p(d_keys.ptr), count,
+                                        ctypes.c_void_p(d_out.ptr), ctypes.c_void_p(d_found.ptr)))
+
+    def bulk_get_values_device(self, d_keys: DeviceBuffer, count: int, d_out: DeviceBuffer):
+        _check(lib().fm_bulk_get_values_device(self._h, ctypes.c_void_p(d_keys.ptr), count,
+                                               ctypes.c_void_p(d_out.ptr)))
+
+    def bulk_insert_device(self, d_keys: DeviceBuffer, d_vals: DeviceBuffer, count: int) -> int:
+        out = ctypes.c_uint64(0)
+        _check(lib().fm_bulk_insert_device(self._h, ctypes.c_void_p(d_keys.ptr),
+                                           ctypes.c_void_p(d_vals.ptr), count, ctypes.byref(out)))
+        return int(out.value)
+
+    def bulk_insert_device_uncounted(self, d_keys: DeviceBuffer, d_vals: DeviceBuffer, count: int):
+        _check(lib().fm_bulk_insert_device_uncounted(self._h, ctypes.c_void_p(d_keys.ptr),
+                                                     ctypes.c_void_p(d_vals.ptr), count))
+
+    def bulk_remove_device(self, d_keys: DeviceBuffer, count: int) -> int:
+        out = ctypes.c_uint64(0)
+        _check(lib().fm_bulk_remove_device(self._h, ctypes.c_void_p(d_keys.ptr), count,
+                                           ctypes.byref(out)))
+        return int(out.value)
+
+    def recount(self) -> int:
+        out = ctypes.c_uint64(0)
+        _check(lib().fm_recount(self._h, ctypes.byref(out)))
+        return int(out.value)
+
+    def synchronize(self):
+        _check(lib().fm_synchronize(self._h))
+
+    # -- logical slot array (test support) --
+    def export_slots(self):
+        cap = self.capacity()
+        flags = np.zeros(cap, dtype=np.uint32)
+        keys = np.zeros(cap * self.key_size, dtype=np.uint8)
+        vals = np.zeros(cap * self.value_size, dtype=np.uint8)
+        _check(lib().fm_export_slots(self._h, _ptr(flags), _ptr(keys), _ptr(vals)))
+        return flags, keys, vals
+
+    def import_slots(self, flags, keys, vals, length: int):
+        f = np.ascontiguousarray(flags, dtype=np.uint32)
+        k = np.ascontiguousarray(keys).view(np.uint8).reshape(-1)
+        v = np.ascontiguousarray(vals).view(np.uint8).reshape(-1)
+        _check(lib().fm_import_slots(self._h, _ptr(f), _ptr(k), _ptr(v), length))
+
+    def tuning(self) -> Tuning:
+        t = Tuning()
+        _check(lib().fm_get_tuning(self._h, ctypes.byref(t)))
+        return t
+
+    def set_tuning(self, get_group=None, get_block=None, remove_group=None, insert_block=None):
+        t = self.tuning()
+        for name, val in (("get_group", get_group), ("get_block", get_block),
+                          ("remove_group", remove_group), ("insert_block", insert_block)):
+            if val is not None:
+                setattr(t, name, val)
+        _check(lib().fm_set_tuning(self._h, ctypes.byref(t)))
+
+
+class FlashMapBuilder:
+    """src/lib.rs:377-449."""
+
+    def __init__(self, capacity: int, key_size: int = 8, value_size: int = 8):
+        self._b = _Builder()
+        lib().fm_builder_init(ctypes.byref(self._b), capacity, key_size, value_size)
+
+    def hash_strategy(self, strategy: int) -> "FlashMapBuilder":
+        self._b.hash_strategy = strategy
+        return self
+
+    def device_id(self, dev: int) -> "FlashMapBuilder":
+        self._b.device_id = dev
+        return self
+
+    def force_cpu(self) -> "FlashMapBuilder":
+        self._b.force_cpu = 1
+        return self
+
+    def require_gpu(self) -> "FlashMapBuilder":
+        self._b.require_gpu = 1
+        return self
+
+    def quiet(self) -> "FlashMapBuilder":
+        self._b.quiet = 1
+        return self
+
+    def build(self) -> FlashMap:
+        h = ctypes.c_void_p()
+        _check(lib().fm_build(ctypes.byref(self._b), ctypes.byref(h)))
+        return FlashMap(h.value, self._b.key_size, self._b.value_size)
+
+
+class _RwLock:
+    """Reader-writer lock (std::sync::RwLock in src/async_map.rs:33)."""
+
+    def __init__(self):
+        self._cv = threading.Condition()
+        self._readers = 0
+        self._writer = False
+
+    def acquire_read(self):
+        with self._cv:
+            while self._writer:
+                self._cv.wait()
+            self._readers += 1
+
+    def release_read(self):
+        with self._cv:
+            self._readers -= 1
+            self._cv.notify_all()
+
+    def acquire_write(self):
+        with self._cv:
+            while self._writer or self._readers:
+                self._cv.wait()
+            self._writer = True
+
+    def release_write(self):
+        with self._cv:
+            self._writer = False
+            self._cv.notify_all()
+
+
+class AsyncFlashMap:
+    """src/async_map.rs:32-147: every op runs in a worker thread
+    (spawn_blocking); gets share a read lock, mutations take the write lock."""
+
+    def __init__(self, fmap: FlashMap):
+        self._map = fmap
+        self._lock = _RwLock()
+
+    def _read(self, fn, *a):
+        self._lock.acquire_read()
+        try:
+            return fn(*a)
+        finally:
+            self._lock.release_read()
+
+    def _write(self, fn, *a):
+        self._lock.acquire_write()
+        try:
+            return fn(*a)
+        finally:
+            self._lock.release_write()
+
+    async def bulk_get(self, keys):
+        return await asyncio.get_running_loop().run_in_executor(None, self._read, self._map.bulk_get, keys)
+
+    async def bulk_insert(self, keys, values) -> int:
+        return await asyncio.get_running_loop().run_in_executor(
+            None, self._write, self._map.bulk_insert, keys, values)
+
+    async def bulk_remove(self, keys) -> int:
+        return await asyncio.get_running_loop().run_in_executor(None, self._write, self._map.bulk_remove, keys)
+
+    async def clear(self):
+        return await asyncio.get_running_loop().run_in_executor(None, self._write, self._map.clear)
+
+    def len(self) -> int:
+        return self._read(len, self._map)
+
+    def is_empty(self) -> bool:
+        return self.len() == 0
+
+    def capacity(self) -> int:
+        return self._read(self._map.capacity)
+
+    def load_factor(self) -> float:
+        return self._read(self._map.load_factor)
+
+    def clone(self) -> "AsyncFlashMap":
+        c = AsyncFlashMap.__new__(AsyncFlashMap)
+        c._map, c._lock = self._map, self._lock
+        return c

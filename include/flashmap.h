@@ -1,0 +1,194 @@
+/*
+ * flashmap.h -- C ABI of the MI355X FlashMap backend.
+ *
+ * Drop-in boundary for the reference's FlashMap<K,V> (src/lib.rs).  K and V
+ * are erased to byte sizes; keys and values are raw little-endian bytes
+ * (bytemuck views, src/gpu.rs:120, :212-217).  Each entry point cites the
+ * reference interface it replaces.  All functions are thread-safe per handle
+ * (calls on one handle serialise on an internal mutex; src/async_map.rs:55
+ * runs concurrent gets on one map).
+ *
+ * Results of every bulk op are bit-exact with the reference's CPU path
+ * applied in batch index order (src/rayon_cpu.rs), including duplicate keys
+ * inside a batch and tombstone effects.
+ */
+#ifndef FLASHMAP_H
+#define FLASHMAP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FM_ABI_VERSION 1
+
+typedef struct fm_map fm_map;
+
+/* One status per FlashMapError variant (src/error.rs:4-39), same order. */
+typedef enum fm_status {
+  FM_OK = 0,
+  FM_ERR_DEVICE_INIT = 1,    /* CudaInit        error.rs:5-6   */
+  FM_ERR_GPU_ALLOC = 2,      /* GpuAlloc        error.rs:8-9   */
+  FM_ERR_KERNEL_LAUNCH = 3,  /* KernelLaunch    error.rs:11-12 */
+  FM_ERR_TRANSFER = 4,       /* Transfer        error.rs:14-15 */
+  FM_ERR_TABLE_FULL = 5,     /* TableFull{..}   error.rs:17-24 */
+  FM_ERR_ZERO_CAPACITY = 6,  /* ZeroCapacity    error.rs:26-27 */
+  FM_ERR_NO_BACKEND = 7,     /* NoBackend       error.rs:29-30 */
+  FM_ERR_GPU_REQUIRED = 8,   /* GpuRequired     error.rs:32-33 */
+  FM_ERR_LOCK_POISONED = 9,  /* LockPoisoned    error.rs:35-36 */
+  FM_ERR_ASYNC_JOIN = 10,    /* AsyncJoin       error.rs:38-39 */
+  /* No reference counterpart: null pointer, size mismatch, misaligned
+   * device buffer, key/value size outside the supported range. */
+  FM_ERR_INVALID_ARGUMENT = 11
+} fm_status;
+
+/* HashStrategy (src/hash.rs:3-27); numbering = HashStrategy::to_mode. */
+typedef enum fm_hash_strategy {
+  FM_HASH_IDENTITY = 0,
+  FM_HASH_MURMUR3 = 1
+} fm_hash_strategy;
+
+/* enum FlashMapBackend (src/lib.rs:98-103). */
+typedef enum fm_backend_kind {
+  FM_BACKEND_NONE = 0,
+  FM_BACKEND_GPU = 1, /* FlashMapBackend::Gpu   */
+  FM_BACKEND_CPU = 2  /* FlashMapBackend::Rayon */
+} fm_backend_kind;
+
+/* FlashMapBuilder (src/lib.rs:377-411). */
+typedef struct fm_builder {
+  uint64_t capacity;      /* FlashMapBuilder::new(capacity)       */
+  uint32_t key_size;      /* size_of::<K>()                       */
+  uint32_t value_size;    /* size_of::<V>()                       */
+  uint32_t hash_strategy; /* .hash_strategy() lib.rs:396-399      */
+  int32_t device_id;      /* .device_id()     lib.rs:402-405      */
+  uint32_t force_cpu;     /* .force_cpu()     lib.rs:408-411      */
+  uint32_t require_gpu;   /* no reference counterpart: fail with the GPU
+                             error instead of falling back (tests, benches) */
+  uint32_t quiet;         /* no reference counterpart: suppress the
+                             fallback message of lib.rs:433-435          */
+} fm_builder;
+
+/* FlashMapBuilder::new (lib.rs:386-393): Identity, device 0, no force. */
+void fm_builder_init(fm_builder* b, uint64_t capacity, uint32_t key_size,
+                     uint32_t value_size);
+/* FlashMapBuilder::build (lib.rs:414-449): GPU first, then the CPU backend,
+ * printing "[flash-map] GPU unavailable (...), using CPU backend" on
+ * fallback.  Capacity rounds to max(c,16).next_power_of_two() on every
+ * backend (rayon_cpu.rs:64).  c == 0 gives FM_ERR_ZERO_CAPACITY on the GPU
+ * tier (gpu.rs:50-52), which then falls back like any GPU error. */
+fm_status fm_build(const fm_builder* b, fm_map** out);
+/* FlashMap::with_capacity (lib.rs:110-112). */
+fm_status fm_with_capacity(uint64_t capacity, uint32_t key_size,
+                           uint32_t value_size, fm_map** out);
+/* Drop for FlashMap / GpuFlashMap (frees all device memory). */
+void fm_destroy(fm_map* m);
+
+/* ---- host-facing API (lib.rs:123-207; gpu.rs:111-374) ---------------- */
+
+/* FlashMap::bulk_get (lib.rs:124-131, gpu.rs:111-191).  keys: n*K bytes;
+ * out_values: n*V bytes (misses zeroed); out_found: n bytes (1 hit, 0 miss;
+ * the Option discriminant). */
+fm_status fm_bulk_get(fm_map* m, const void* keys, uint64_t n,
+                      void* out_values, uint8_t* out_found);
+/* FlashMap::bulk_insert (lib.rs:143-150, gpu.rs:193-273).  keys n*K bytes,
+ * values n*V bytes (the (K,V) tuples split into two arrays as gpu.rs:212-217
+ * does).  TableFull when len + n > capacity, before any mutation.
+ * out_new: number of newly occupied slots. */
+fm_status fm_bulk_insert(fm_map* m, const void* keys, const void* values,
+                         uint64_t n, uint64_t* out_new);
+/* FlashMap::bulk_remove (lib.rs:155-162, gpu.rs:275-332). */
+fm_status fm_bulk_remove(fm_map* m, const void* keys, uint64_t n,
+                         uint64_t* out_removed);
+/* len / capacity / load_factor / clear (lib.rs:165-207). */
+uint64_t fm_len(const fm_map* m);
+uint64_t fm_capacity(const fm_map* m);
+double fm_load_factor(const fm_map* m);
+fm_status fm_clear(fm_map* m);
+/* Which FlashMapBackend arm was built (lib.rs:357-362). */
+fm_backend_kind fm_backend(const fm_map* m);
+uint32_t fm_key_size(const fm_map* m);
+uint32_t fm_value_size(const fm_map* m);
+/* impl Debug for FlashMap (lib.rs:353-370):
+ * FlashMap { backend: "GPU", len: N, capacity: C, load_factor: "x.y%" } */
+fm_status fm_debug_string(const fm_map* m, char* buf, size_t buf_len);
+
+/* Error detail of the calling thread's last failed call (Display text of
+ * the FlashMapError, error.rs:4-39).  Returns the untruncated length. */
+size_t fm_last_error_message(char* buf, size_t buf_len);
+/* TableFull fields of the calling thread's last FM_ERR_TABLE_FULL
+ * (error.rs:17-24; load_factor in percent as gpu.rs:204). */
+fm_status fm_last_table_full(uint64_t* occupied, uint64_t* capacity,
+                             double* load_factor_pct);
+
+/* ---- device-resident API (lib.rs:213-350, gpu.rs:376-712) -------------
+ * GPU backend only; the CPU backend returns FM_ERR_GPU_REQUIRED
+ * (lib.rs:235).  Device buffers come from fm_alloc_device / fm_upload (or
+ * any device allocation on the map's device, 16-byte aligned).  Work is
+ * ordered on the map's stream; calls that return a count synchronise. */
+
+/* FlashMap::device (lib.rs:218-224): the hipStream_t of the map. */
+void* fm_device_stream(fm_map* m);
+int32_t fm_device_id(const fm_map* m);
+/* FlashMap::alloc_device (lib.rs:251-257, gpu.rs:401-406): zeroed. */
+fm_status fm_alloc_device(fm_map* m, uint64_t bytes, void** out_dptr);
+/* Drop for CudaSlice. */
+fm_status fm_free_device(fm_map* m, void* dptr);
+/* upload_keys / upload_values (lib.rs:231-247, gpu.rs:386-399). */
+fm_status fm_upload(fm_map* m, const void* host, uint64_t bytes,
+                    void** out_dptr);
+/* download (lib.rs:261-267, gpu.rs:409-413). */
+fm_status fm_download(fm_map* m, const void* dptr, uint64_t bytes,
+                      void* host);
+/* bulk_get_device (lib.rs:274-284, gpu.rs:422-478).  The caller passes the
+ * output buffers (count*V and count bytes) instead of receiving fresh
+ * allocations; asynchronous on the map's stream. */
+fm_status fm_bulk_get_device(fm_map* m, const void* d_keys, uint64_t count,
+                             void* d_out_values, uint8_t* d_out_found);
+/* bulk_get_values_device (lib.rs:291-301, gpu.rs:484-536): misses zeroed. */
+fm_status fm_bulk_get_values_device(fm_map* m, const void* d_keys,
+                                    uint64_t count, void* d_out_values);
+/* bulk_insert_device (lib.rs:307-318, gpu.rs:541-611). */
+fm_status fm_bulk_insert_device(fm_map* m, const void* d_keys,
+                                const void* d_values, uint64_t count,
+                                uint64_t* out_new);
+/* bulk_insert_device_uncounted (lib.rs:325-338, gpu.rs:618-672): no
+ * TableFull check, no readback, len += count. */
+fm_status fm_bulk_insert_device_uncounted(fm_map* m, const void* d_keys,
+                                          const void* d_values,
+                                          uint64_t count);
+/* No reference counterpart (the device family has no remove): device-key
+ * variant of bulk_remove, same semantics. */
+fm_status fm_bulk_remove_device(fm_map* m, const void* d_keys, uint64_t count,
+                                uint64_t* out_removed);
+/* recount (lib.rs:344-350, gpu.rs:677-712): len = number of OCCUPIED slots. */
+fm_status fm_recount(fm_map* m, uint64_t* out_len);
+/* Waits for all work queued on the map's stream. */
+fm_status fm_synchronize(fm_map* m);
+
+/* ---- logical slot array (test support; no reference counterpart) ------
+ * flags: capacity u32 (dist<<4 | state, kernels.cu:13-30); keys/values:
+ * capacity*K / capacity*V bytes, zeroed where the slot is not OCCUPIED. */
+fm_status fm_export_slots(fm_map* m, uint32_t* flags, void* keys,
+                          void* values);
+fm_status fm_import_slots(fm_map* m, const uint32_t* flags, const void* keys,
+                          const void* values, uint64_t len);
+
+/* ---- launch tuning (no reference counterpart) --------------------------
+ * group: lanes that probe one key together (1..64, power of two);
+ * block: threads per workgroup (64..1024, multiple of 64). */
+typedef struct fm_tuning {
+  uint32_t get_group;
+  uint32_t get_block;
+  uint32_t remove_group;
+  uint32_t insert_block;
+} fm_tuning;
+fm_status fm_get_tuning(const fm_map* m, fm_tuning* out);
+fm_status fm_set_tuning(fm_map* m, const fm_tuning* t);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FLASHMAP_H */
