@@ -300,6 +300,24 @@ fm_status fm_import_slots(fm_map* m, const uint32_t* flags, const void* keys, co
   return m->be->import_slots(flags, keys, values, len);
 }
 
+fm_status fm_copy_table(fm_map* dst, fm_map* src) {
+  if (!dst || !src) return bad_arg("fm_copy_table: null map");
+  if (dst == src) return FM_OK;
+  std::scoped_lock guard(dst->mu, src->mu);
+  return dst->be->copy_from(*src->be);
+}
+
+fm_status fm_profile_enable(fm_map* m, int enable) {
+  FM_LOCK(m);
+  return m->be->profile_enable(enable != 0);
+}
+
+fm_status fm_profile_read(fm_map* m, fm_kernel_stat* out, uint32_t max, uint32_t* count) {
+  FM_LOCK(m);
+  if (!out || !count) return bad_arg("fm_profile_read: null argument");
+  return m->be->profile_read(out, max, count);
+}
+
 fm_status fm_get_tuning(const fm_map* m, fm_tuning* out) {
   if (!m || !out) return bad_arg("fm_get_tuning: null argument");
   std::lock_guard<std::mutex> g(m->mu);

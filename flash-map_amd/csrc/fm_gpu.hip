@@ -10,8 +10,10 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "fm_internal.h"
 #include "fm_kernels.h"
@@ -30,8 +32,12 @@ constexpr uint64_t kMaxGpuCapacity = 1ull << 31;  // u32 slot ids in scratch
 // ---------------------------------------------------------------------------
 struct Ops {
   hipError_t (*get)(const Table&, const uint8_t*, uint64_t, uint8_t*, uint8_t*, uint32_t, uint32_t, hipStream_t);
-  hipError_t (*ins_dry)(const Table&, const uint8_t*, uint64_t, unsigned long long*, uint32_t*, Counters*, uint32_t,
-                        hipStream_t);
+  hipError_t (*ins_dry)(const Table&, const uint8_t*, uint64_t, unsigned long long*, uint32_t*, uint32_t*, Counters*,
+                        uint32_t, hipStream_t);
+  hipError_t (*ins_leader)(const Table&, const uint8_t*, const uint8_t*, uint64_t, const unsigned long long*,
+                           const uint32_t*, uint32_t*, Counters*, hipStream_t);
+  hipError_t (*ins_big)(const Table&, const uint8_t*, const uint8_t*, uint64_t, const unsigned long long*,
+                        const uint32_t*, Counters*, hipStream_t);
   hipError_t (*ins_replay)(const Table&, const uint8_t*, const uint8_t*, uint64_t, const uint32_t*, const uint32_t*,
                            Counters*, uint32_t, hipStream_t);
   hipError_t (*rem_probe)(const Table&, const uint8_t*, uint64_t, unsigned long long*, uint32_t, uint32_t, hipStream_t);
@@ -66,8 +72,24 @@ hipError_t launch_get(const Table& t, const uint8_t* q, uint64_t n, uint8_t* ov,
 
 template <int KS>
 hipError_t launch_ins_dry(const Table& t, const uint8_t* k, uint64_t n, unsigned long long* Q, uint32_t* home,
-                          Counters* c, uint32_t block, hipStream_t s) {
-  hipLaunchKernelGGL((dev::k_ins_dry<KS>), dim3(grid_for(n, block)), dim3(block), 0, s, t, k, n, Q, home, c);
+                          uint32_t* dslot, Counters* c, uint32_t block, hipStream_t s) {
+  hipLaunchKernelGGL((dev::k_ins_dry<KS>), dim3(grid_for(n, block)), dim3(block), 0, s, t, k, n, Q, home, dslot, c);
+  return hipGetLastError();
+}
+
+template <int KS, int VS>
+hipError_t launch_ins_leader(const Table& t, const uint8_t* k, const uint8_t* v, uint64_t n,
+                             const unsigned long long* Q, const uint32_t* dslot, uint32_t* big, Counters* c,
+                             hipStream_t s) {
+  hipLaunchKernelGGL((dev::k_ins_leader<KS, VS>), dim3(grid_for(n, 256)), dim3(256), 0, s, t, k, v, n, Q, dslot, big,
+                     c);
+  return hipGetLastError();
+}
+
+template <int KS, int VS>
+hipError_t launch_ins_big(const Table& t, const uint8_t* k, const uint8_t* v, uint64_t n, const unsigned long long* Q,
+                          const uint32_t* big, Counters* c, hipStream_t s) {
+  hipLaunchKernelGGL((dev::k_ins_big<KS, VS>), dim3(256), dim3(1024), 0, s, t, k, v, n, Q, big, c);
   return hipGetLastError();
 }
 
@@ -117,6 +139,8 @@ Ops make_ops() {
   Ops o;
   o.get = &launch_get<KS, VS>;
   o.ins_dry = &launch_ins_dry<KS>;
+  o.ins_leader = &launch_ins_leader<KS, VS>;
+  o.ins_big = &launch_ins_big<KS, VS>;
   o.ins_replay = &launch_ins_replay<KS, VS>;
   o.rem_probe = &launch_rem_probe<KS>;
   o.rem_runs = &launch_rem_runs<KS>;
@@ -183,6 +207,9 @@ class GpuBackend final : public Backend {
   fm_status bulk_remove_device(const void* d_keys, uint64_t n, uint64_t* out_removed) override;
   fm_status recount(uint64_t* out) override;
   fm_status synchronize() override;
+  fm_status copy_from(Backend& src) override;
+  fm_status profile_enable(bool on) override;
+  fm_status profile_read(fm_kernel_stat* out, uint32_t max, uint32_t* count) override;
   fm_status get_tuning(fm_tuning* out) const override {
     *out = tune_;
     return FM_OK;
@@ -209,6 +236,14 @@ class GpuBackend final : public Backend {
   fm_status check_insert_status(const Counters& c) {
     if (c.status & dev::kStatusBudget)
       return fail(FM_ERR_KERNEL_LAUNCH, "insert walk found no free slot in a full table (uncounted inserts overfilled it)");
+    if ((c.status & dev::kStatusDryFail) && last_insert_led_) {
+      // Unreachable while len never undercounts occupied slots; the leader
+      // kernels skipped the batch, so only owner[] needs restoring.
+      hipLaunchKernelGGL(dev::k_fill_u32, dim3(grid_for(t_.cap, 256)), dim3(256), 0, stream_, t_.owner, t_.cap,
+                         dev::kNoOwner);
+      (void)hipStreamSynchronize(stream_);
+      return fail(FM_ERR_KERNEL_LAUNCH, "insert: free-slot accounting disagrees with the table; batch not applied");
+    }
     return FM_OK;
   }
 
@@ -225,10 +260,14 @@ class GpuBackend final : public Backend {
   uint64_t len_ = 0;
   Ops ops_{};
   fm_tuning tune_{4, 256, 4, 256};
+  bool tune_sort_only_ = false;  // FM_INSERT_SORT_PATH=1 forces the sort-grouped insert
+  bool last_insert_led_ = false;
 
   // scratch (grown geometrically, reused)
   uint64_t scratch_n_ = 0;
   uint32_t* home_ = nullptr;
+  uint32_t* dslot_ = nullptr;
+  uint32_t* big_ = nullptr;
   uint32_t* cid_ = nullptr;
   uint32_t* idx_ = nullptr;
   uint32_t* cid_s_ = nullptr;
@@ -240,6 +279,37 @@ class GpuBackend final : public Backend {
   unsigned long long* Q_ = nullptr;
   Counters* ctr_ = nullptr;
   Counters* h_ctr_ = nullptr;  // pinned
+
+  // per-kernel event timing
+  enum {
+    kKidGet, kKidInsCopy, kKidInsDry, kKidInsClosure, kKidInsSort, kKidInsReplay, kKidRemProbe, kKidRemCheck,
+    kKidRemApply, kKidRemRuns, kKidRemSort, kKidRemReplay, kKidCopyTable, kKidInsLeader, kKidInsBig, kKidInsBits,
+    kNumKid
+  };
+  struct PendingTiming {
+    int kid;
+    hipEvent_t a, b;
+  };
+  struct Prof {  // brackets the launches of one scope when profiling is on
+    GpuBackend* g;
+    int kid;
+    Prof(GpuBackend* g_, int k) : g(g_), kid(k) {
+      if (g->prof_) g->prof_mark(kid, true);
+    }
+    ~Prof() {
+      if (g->prof_) g->prof_mark(kid, false);
+    }
+  };
+  static const char* kid_name(int k);
+  hipEvent_t take_event();
+  void prof_mark(int kid, bool begin);
+  void harvest();
+  bool prof_ = false;
+  hipEvent_t prof_open_ = nullptr;
+  std::vector<PendingTiming> pending_;
+  std::vector<hipEvent_t> ev_pool_;
+  double prof_ms_[kNumKid] = {};
+  uint64_t prof_n_[kNumKid] = {};
   int sort_end_bit_ = 1;
 
   // host-API staging
@@ -270,6 +340,8 @@ void GpuBackend::release() {
   dfree(t_.emptybits);
   dfree(t_.owner);
   dfree(home_);
+  dfree(dslot_);
+  dfree(big_);
   dfree(cid_);
   dfree(idx_);
   dfree(cid_s_);
@@ -286,6 +358,11 @@ void GpuBackend::release() {
   dfree(st_vals_);
   dfree(st_out_);
   dfree(st_found_);
+  pending_.clear();
+  for (hipEvent_t e : ev_pool_) (void)hipEventDestroy(e);
+  ev_pool_.clear();
+  if (prof_open_) (void)hipEventDestroy(prof_open_);
+  prof_open_ = nullptr;
   if (stream_) (void)hipStreamDestroy(stream_);
   stream_ = nullptr;
   dev_ = -1;
@@ -345,6 +422,7 @@ fm_status GpuBackend::init(const fm_builder& b, std::string* err) {
     return FM_ERR_GPU_ALLOC;
   }
   ops_ = select_ops(t_.K, t_.V);
+  if (const char* env = std::getenv("FM_INSERT_SORT_PATH")) tune_sort_only_ = env[0] == '1';
   sort_end_bit_ = 1;
   while ((1ull << sort_end_bit_) <= cap) ++sort_end_bit_;  // ids in [0, cap]
   hipLaunchKernelGGL(dev::k_clear, dim3(grid_for(cap, 256)), dim3(256), 0, stream_, t_);
@@ -373,6 +451,8 @@ fm_status GpuBackend::ensure_scratch(uint64_t n) {
   if (n > 0x7FFFFFFFull) return fail(FM_ERR_INVALID_ARGUMENT, "batch above 2^31-1 keys");
   (void)hipStreamSynchronize(stream_);
   dfree(home_);
+  dfree(dslot_);
+  dfree(big_);
   dfree(cid_);
   dfree(idx_);
   dfree(cid_s_);
@@ -384,6 +464,7 @@ fm_status GpuBackend::ensure_scratch(uint64_t n) {
   scratch_n_ = 0;
   hipError_t e;
   if ((e = hipMalloc(&home_, want * 4)) != hipSuccess || (e = hipMalloc(&cid_, want * 4)) != hipSuccess ||
+      (e = hipMalloc(&dslot_, want * 4)) != hipSuccess || (e = hipMalloc(&big_, want * 4)) != hipSuccess ||
       (e = hipMalloc(&idx_, want * 4)) != hipSuccess || (e = hipMalloc(&cid_s_, want * 4)) != hipSuccess ||
       (e = hipMalloc(&idx_s_, want * 4)) != hipSuccess || (e = hipMalloc(&ev_, want * 8)) != hipSuccess ||
       (e = hipMalloc(&action_, want)) != hipSuccess)
@@ -443,6 +524,7 @@ fm_status GpuBackend::check_device_ptr(const void* p, uint32_t align, const char
 }
 
 fm_status GpuBackend::run_get(const uint8_t* d_keys, uint64_t n, uint8_t* d_out, uint8_t* d_found) {
+  Prof p(this, kKidGet);
   return check_launch(ops_.get(t_, d_keys, n, d_out, d_found, tune_.get_group, tune_.get_block, stream_), "k_get");
 }
 
@@ -451,17 +533,58 @@ fm_status GpuBackend::run_insert(const uint8_t* d_keys, const uint8_t* d_vals, u
   if ((st = ensure_scratch(n)) != FM_OK) return st;
   if ((st = zero_counters()) != FM_OK) return st;
   const uint64_t words = (t_.cap + 63) / 64;
-  hipError_t e = hipMemcpyAsync(Q_, t_.freebits, words * 8, hipMemcpyDeviceToDevice, stream_);
-  if (e != hipSuccess) return fail(FM_ERR_TRANSFER, hip_msg("bitmap copy", e));
   const uint32_t blk = tune_.insert_block;
-  if ((st = check_launch(ops_.ins_dry(t_, d_keys, n, Q_, home_, ctr_, blk, stream_), "k_ins_dry")) != FM_OK) return st;
-  hipLaunchKernelGGL(dev::k_ins_closure, dim3(grid_for(n, blk)), dim3(blk), 0, stream_, t_.cap, Q_, home_, n, cid_,
-                     idx_, ctr_);
-  if ((st = check_launch(hipGetLastError(), "k_ins_closure")) != FM_OK) return st;
-  size_t tmp = sort_tmp_bytes_;
-  e = hipcub::DeviceRadixSort::SortPairs(sort_tmp_, tmp, cid_, cid_s_, idx_, idx_s_, static_cast<int>(n), 0,
-                                         sort_end_bit_, stream_);
-  if (e != hipSuccess) return fail(FM_ERR_KERNEL_LAUNCH, hip_msg("radix sort", e));
+  {
+    Prof p(this, kKidInsCopy);
+    hipError_t e = hipMemcpyAsync(Q_, t_.freebits, words * 8, hipMemcpyDeviceToDevice, stream_);
+    if (e != hipSuccess) return fail(FM_ERR_TRANSFER, hip_msg("bitmap copy", e));
+  }
+  // With len + n < cap the dry pass provably leaves a free slot (len never
+  // undercounts occupied slots), so closures are bounded by barriers and are
+  // led without sorting.  Otherwise (the batch may fill the table) group by
+  // closure id with a stable radix sort, which also covers "no barrier".
+  last_insert_led_ = len_ + n < t_.cap && !tune_sort_only_;
+  if (last_insert_led_) {
+    {
+      Prof p(this, kKidInsDry);
+      if ((st = check_launch(ops_.ins_dry(t_, d_keys, n, Q_, nullptr, dslot_, ctr_, blk, stream_), "k_ins_dry")) !=
+          FM_OK)
+        return st;
+    }
+    {
+      Prof p(this, kKidInsLeader);
+      if ((st = check_launch(ops_.ins_leader(t_, d_keys, d_vals, n, Q_, dslot_, big_, ctr_, stream_),
+                             "k_ins_leader")) != FM_OK)
+        return st;
+    }
+    {
+      Prof p(this, kKidInsBig);
+      if ((st = check_launch(ops_.ins_big(t_, d_keys, d_vals, n, Q_, big_, ctr_, stream_), "k_ins_big")) != FM_OK)
+        return st;
+    }
+    Prof p(this, kKidInsBits);
+    hipLaunchKernelGGL(dev::k_ins_bits, dim3(grid_for(n, blk)), dim3(blk), 0, stream_, t_, dslot_, n, ctr_);
+    return check_launch(hipGetLastError(), "k_ins_bits");
+  }
+  {
+    Prof p(this, kKidInsDry);
+    if ((st = check_launch(ops_.ins_dry(t_, d_keys, n, Q_, home_, nullptr, ctr_, blk, stream_), "k_ins_dry")) != FM_OK)
+      return st;
+  }
+  {
+    Prof p(this, kKidInsClosure);
+    hipLaunchKernelGGL(dev::k_ins_closure, dim3(grid_for(n, blk)), dim3(blk), 0, stream_, t_.cap, Q_, home_, n, cid_,
+                       idx_, ctr_);
+    if ((st = check_launch(hipGetLastError(), "k_ins_closure")) != FM_OK) return st;
+  }
+  {
+    Prof p(this, kKidInsSort);
+    size_t tmp = sort_tmp_bytes_;
+    hipError_t e = hipcub::DeviceRadixSort::SortPairs(sort_tmp_, tmp, cid_, cid_s_, idx_, idx_s_, static_cast<int>(n),
+                                                      0, sort_end_bit_, stream_);
+    if (e != hipSuccess) return fail(FM_ERR_KERNEL_LAUNCH, hip_msg("radix sort", e));
+  }
+  Prof p(this, kKidInsReplay);
   return check_launch(ops_.ins_replay(t_, d_keys, d_vals, n, cid_s_, idx_s_, ctr_, blk, stream_), "k_ins_replay");
 }
 
@@ -470,30 +593,128 @@ fm_status GpuBackend::run_remove(const uint8_t* d_keys, uint64_t n, uint64_t* re
   if ((st = ensure_scratch(n)) != FM_OK) return st;
   if ((st = zero_counters()) != FM_OK) return st;
   const uint32_t blk = tune_.insert_block;
-  if ((st = check_launch(ops_.rem_probe(t_, d_keys, n, ev_, tune_.remove_group, tune_.get_block, stream_),
-                         "k_rem_probe")) != FM_OK)
-    return st;
-  hipLaunchKernelGGL(dev::k_rem_check, dim3(grid_for(n, blk)), dim3(blk), 0, stream_, t_, ev_, n, action_, ctr_);
-  if ((st = check_launch(hipGetLastError(), "k_rem_check")) != FM_OK) return st;
-  hipLaunchKernelGGL(dev::k_rem_apply, dim3(grid_for(n, blk)), dim3(blk), 0, stream_, t_, ev_, action_, n, ctr_);
-  if ((st = check_launch(hipGetLastError(), "k_rem_apply")) != FM_OK) return st;
+  {
+    Prof p(this, kKidRemProbe);
+    if ((st = check_launch(ops_.rem_probe(t_, d_keys, n, ev_, tune_.remove_group, tune_.get_block, stream_),
+                           "k_rem_probe")) != FM_OK)
+      return st;
+  }
+  {
+    Prof p(this, kKidRemCheck);
+    hipLaunchKernelGGL(dev::k_rem_check, dim3(grid_for(n, blk)), dim3(blk), 0, stream_, t_, ev_, n, action_, ctr_);
+    if ((st = check_launch(hipGetLastError(), "k_rem_check")) != FM_OK) return st;
+  }
+  {
+    Prof p(this, kKidRemApply);
+    hipLaunchKernelGGL(dev::k_rem_apply, dim3(grid_for(n, blk)), dim3(blk), 0, stream_, t_, ev_, action_, n, ctr_);
+    if ((st = check_launch(hipGetLastError(), "k_rem_apply")) != FM_OK) return st;
+  }
   Counters c;
   if ((st = read_counters(&c)) != FM_OK) return st;
   if (c.status & dev::kStatusConflict) {
     // Exact replay grouped by EMPTY-delimited runs, from the untouched
     // pre-batch state.
     if ((st = zero_counters()) != FM_OK) return st;
-    if ((st = check_launch(ops_.rem_runs(t_, d_keys, n, cid_, idx_, blk, stream_), "k_rem_runs")) != FM_OK) return st;
-    size_t tmp = sort_tmp_bytes_;
-    hipError_t e = hipcub::DeviceRadixSort::SortPairs(sort_tmp_, tmp, cid_, cid_s_, idx_, idx_s_, static_cast<int>(n),
-                                                      0, sort_end_bit_, stream_);
-    if (e != hipSuccess) return fail(FM_ERR_KERNEL_LAUNCH, hip_msg("radix sort", e));
-    if ((st = check_launch(ops_.rem_replay(t_, d_keys, n, cid_s_, idx_s_, ctr_, blk, stream_), "k_rem_replay")) !=
-        FM_OK)
-      return st;
+    {
+      Prof p(this, kKidRemRuns);
+      if ((st = check_launch(ops_.rem_runs(t_, d_keys, n, cid_, idx_, blk, stream_), "k_rem_runs")) != FM_OK)
+        return st;
+    }
+    {
+      Prof p(this, kKidRemSort);
+      size_t tmp = sort_tmp_bytes_;
+      hipError_t e = hipcub::DeviceRadixSort::SortPairs(sort_tmp_, tmp, cid_, cid_s_, idx_, idx_s_,
+                                                        static_cast<int>(n), 0, sort_end_bit_, stream_);
+      if (e != hipSuccess) return fail(FM_ERR_KERNEL_LAUNCH, hip_msg("radix sort", e));
+    }
+    {
+      Prof p(this, kKidRemReplay);
+      if ((st = check_launch(ops_.rem_replay(t_, d_keys, n, cid_s_, idx_s_, ctr_, blk, stream_), "k_rem_replay")) !=
+          FM_OK)
+        return st;
+    }
     if ((st = read_counters(&c)) != FM_OK) return st;
   }
   *removed = c.count;
+  return FM_OK;
+}
+
+// ---- per-kernel timing -----------------------------------------------------
+
+const char* GpuBackend::kid_name(int k) {
+  static const char* names[kNumKid] = {"k_get",        "ins_bitmap_copy", "k_ins_dry",    "k_ins_closure",
+                                       "ins_radix_sort", "k_ins_replay",  "k_rem_probe",  "k_rem_check",
+                                       "k_rem_apply",  "k_rem_runs",      "rem_radix_sort", "k_rem_replay",
+                                       "copy_table",   "k_ins_leader",    "k_ins_big",    "k_ins_bits"};
+  return names[k];
+}
+
+hipEvent_t GpuBackend::take_event() {
+  if (!ev_pool_.empty()) {
+    hipEvent_t e = ev_pool_.back();
+    ev_pool_.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+void GpuBackend::prof_mark(int kid, bool begin) {
+  if (begin) {
+    prof_open_ = take_event();
+    if (prof_open_) (void)hipEventRecord(prof_open_, stream_);
+    return;
+  }
+  hipEvent_t b = take_event();
+  if (!prof_open_ || !b) return;
+  (void)hipEventRecord(b, stream_);
+  pending_.push_back(PendingTiming{kid, prof_open_, b});
+  prof_open_ = nullptr;
+  if (pending_.size() > 4096) harvest();
+}
+
+void GpuBackend::harvest() {
+  if (pending_.empty()) return;
+  (void)hipStreamSynchronize(stream_);
+  for (const PendingTiming& r : pending_) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+      prof_ms_[r.kid] += ms;
+      prof_n_[r.kid] += 1;
+    }
+    ev_pool_.push_back(r.a);
+    ev_pool_.push_back(r.b);
+  }
+  pending_.clear();
+}
+
+fm_status GpuBackend::profile_enable(bool on) {
+  fm_status st;
+  if ((st = use_device()) != FM_OK) return st;
+  harvest();
+  prof_ = on;
+  for (int k = 0; k < kNumKid; ++k) {
+    prof_ms_[k] = 0.0;
+    prof_n_[k] = 0;
+  }
+  return FM_OK;
+}
+
+fm_status GpuBackend::profile_read(fm_kernel_stat* out, uint32_t max, uint32_t* count) {
+  fm_status st;
+  if ((st = use_device()) != FM_OK) return st;
+  harvest();
+  uint32_t c = 0;
+  for (int k = 0; k < kNumKid && c < max; ++k) {
+    if (!prof_n_[k]) continue;
+    std::memset(out[c].name, 0, sizeof out[c].name);
+    std::strncpy(out[c].name, kid_name(k), sizeof out[c].name - 1);
+    out[c].launches = prof_n_[k];
+    out[c].total_ms = prof_ms_[k];
+    ++c;
+  }
+  *count = c;
   return FM_OK;
 }
 
@@ -715,6 +936,27 @@ fm_status GpuBackend::synchronize() {
   if ((st = use_device()) != FM_OK) return st;
   hipError_t e = hipStreamSynchronize(stream_);
   return e == hipSuccess ? FM_OK : fail(FM_ERR_KERNEL_LAUNCH, hip_msg("synchronize", e));
+}
+
+fm_status GpuBackend::copy_from(Backend& other) {
+  GpuBackend* src = dynamic_cast<GpuBackend*>(&other);
+  if (!src || src->dev_ != dev_ || src->t_.cap != t_.cap || src->t_.K != t_.K || src->t_.V != t_.V ||
+      src->t_.mode != t_.mode)
+    return fail(FM_ERR_INVALID_ARGUMENT, "fm_copy_table: maps differ in device, capacity, sizes or hash");
+  if (src == this) return FM_OK;
+  fm_status st;
+  if ((st = use_device()) != FM_OK) return st;
+  hipError_t e = hipStreamSynchronize(src->stream_);
+  const uint64_t words = (t_.cap + 63) / 64;
+  Prof p(this, kKidCopyTable);
+  if (e == hipSuccess) e = hipMemcpyAsync(t_.flags, src->t_.flags, t_.cap * 4, hipMemcpyDeviceToDevice, stream_);
+  if (e == hipSuccess) e = hipMemcpyAsync(t_.keys, src->t_.keys, t_.cap * t_.K, hipMemcpyDeviceToDevice, stream_);
+  if (e == hipSuccess) e = hipMemcpyAsync(t_.vals, src->t_.vals, t_.cap * t_.V, hipMemcpyDeviceToDevice, stream_);
+  if (e == hipSuccess) e = hipMemcpyAsync(t_.freebits, src->t_.freebits, words * 8, hipMemcpyDeviceToDevice, stream_);
+  if (e == hipSuccess) e = hipMemcpyAsync(t_.emptybits, src->t_.emptybits, words * 8, hipMemcpyDeviceToDevice, stream_);
+  if (e != hipSuccess) return fail(FM_ERR_TRANSFER, hip_msg("fm_copy_table", e));
+  len_ = src->len_;
+  return FM_OK;
 }
 
 }  // namespace

@@ -49,6 +49,9 @@ class Backend {
   virtual fm_status bulk_remove_device(const void*, uint64_t, uint64_t*) { return gpu_required(); }
   virtual fm_status recount(uint64_t*) { return gpu_required(); }
   virtual fm_status synchronize() { return FM_OK; }
+  virtual fm_status copy_from(Backend&) { return gpu_required(); }
+  virtual fm_status profile_enable(bool) { return gpu_required(); }
+  virtual fm_status profile_read(fm_kernel_stat*, uint32_t, uint32_t*) { return gpu_required(); }
   virtual fm_status get_tuning(fm_tuning*) const { return gpu_required(); }
   virtual fm_status set_tuning(const fm_tuning*) { return gpu_required(); }
 };

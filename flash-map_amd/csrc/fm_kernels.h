@@ -75,7 +75,7 @@ struct Table {
 
 struct Counters {
   uint32_t status;
-  uint32_t pad;
+  uint32_t nbig;  // closures queued for k_ins_big
   unsigned long long count;
 };
 
@@ -311,11 +311,12 @@ k_get(Table t, const uint8_t* __restrict__ qkeys, uint64_t n, uint8_t* __restric
 // ---------------------------------------------------------------------------
 
 // Pass 1: hash every key and dry-claim one free bit of Q (a copy of
-// freebits) at/after its home.
+// freebits) at/after its home.  With `dslot` given, the claimed slot is
+// recorded per key and the key's index is written to owner[slot].
 template <int KS>
 __global__ void __launch_bounds__(1024)
 k_ins_dry(Table t, const uint8_t* __restrict__ in_keys, uint64_t n, unsigned long long* __restrict__ Q,
-          uint32_t* __restrict__ home_out, Counters* ctr) {
+          uint32_t* __restrict__ home_out, uint32_t* __restrict__ dslot, Counters* ctr) {
   const uint32_t K = KS ? KS : t.K;
   const uint64_t budget = ((t.cap + 63) >> 6) + n + 2;
   for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
@@ -323,9 +324,17 @@ k_ins_dry(Table t, const uint8_t* __restrict__ in_keys, uint64_t n, unsigned lon
     Blob<KS> k;
     k.load(in_keys + i * K, K);
     const uint64_t h = k.hash(K, t.mode) & t.mask;
-    home_out[i] = static_cast<uint32_t>(h);
+    if (home_out) home_out[i] = static_cast<uint32_t>(h);
     uint64_t s;
-    if (!bm_claim(Q, h, t.cap, budget, &s)) atomicOr(&ctr->status, kStatusDryFail);
+    if (!bm_claim(Q, h, t.cap, budget, &s)) {
+      atomicOr(&ctr->status, kStatusDryFail);
+      if (dslot) dslot[i] = 0;  // never read: the leader kernels stop on kStatusDryFail
+      continue;
+    }
+    if (dslot) {
+      dslot[i] = static_cast<uint32_t>(s);
+      t.owner[s] = static_cast<uint32_t>(i);
+    }
   }
 }
 
@@ -422,6 +431,194 @@ k_ins_replay(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __rest
       }
     }
     wave_add(&ctr->count, claimed);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Sort-free closure replay, used when a barrier provably exists (len + n <
+// cap, so the dry pass leaves a free slot).  P = t.freebits (pre-batch free
+// slots; untouched until k_ins_bits), Q = free slots after the dry pass,
+// N = P & ~Q = dry-claimed slots.  Every closure is an arc between two
+// barriers; its N slots are exactly its keys' dry claims and owner[] maps
+// each N slot to its key.  The key whose dry slot is the first N slot of the
+// arc (the nearest P slot below it is a barrier) leads the closure.
+// ---------------------------------------------------------------------------
+constexpr int kLeaderMax = 32;  // members one leader lane orders and replays
+constexpr int kBigMax = 4096;   // members a workgroup sorts in LDS at a time
+
+__device__ __forceinline__ bool bit_at(const unsigned long long* bm, uint64_t i) {
+  return (bm[i >> 6] >> (i & 63)) & 1ull;
+}
+
+// Highest set bit strictly below `start`, cyclically (`start` itself when it
+// is the only set bit).
+__device__ inline uint64_t bm_prev_set(const unsigned long long* bm, uint64_t start, uint64_t cap) {
+  const uint64_t words = (cap + 63) >> 6;
+  uint64_t w = start >> 6;
+  unsigned long long word = bm[w] & ((1ull << (start & 63)) - 1ull);
+  for (uint64_t it = 0; it <= words; ++it) {
+    if (word) return (w << 6) + static_cast<uint64_t>(63 - __clzll(word));
+    w = (w == 0) ? words - 1 : w - 1;
+    word = bm[w];
+  }
+  return start;
+}
+
+// Leaders order and replay closures of up to kLeaderMax keys; larger
+// closures are queued for k_ins_big.
+template <int KS, int VS>
+__global__ void __launch_bounds__(256)
+k_ins_leader(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __restrict__ in_vals, uint64_t n,
+             const unsigned long long* __restrict__ Q, const uint32_t* __restrict__ dslot, uint32_t* __restrict__ big,
+             Counters* ctr) {
+  __shared__ uint32_t lst[kLeaderMax * 256];  // column per lane: lst[j * 256 + lane]
+  const uint32_t K = KS ? KS : t.K;
+  const uint32_t V = VS ? VS : t.V;
+  const unsigned long long* P = t.freebits;
+  uint32_t* my = lst + threadIdx.x;
+  if (ctr->status & kStatusDryFail) return;  // block-uniform
+  for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * blockDim.x;; base += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    if (base >= n) break;
+    const uint64_t i = base + threadIdx.x;
+    uint32_t claimed = 0;
+    if (i < n) {
+      const uint64_t s = dslot[i];
+      const uint64_t b = bm_prev_set(P, s, t.cap);
+      if (b != s && bit_at(Q, b)) {
+        uint32_t m = 0;
+        uint64_t pos = s;
+        for (;;) {
+          const uint64_t x = bm_next_set(P, pos, t.cap);
+          if (bit_at(Q, x)) break;  // the closure's barrier
+          if (m == kLeaderMax) {
+            m = kLeaderMax + 1;
+            break;
+          }
+          my[m * 256] = t.owner[x];
+          ++m;
+          pos = (x + 1 == t.cap) ? 0 : x + 1;
+        }
+        if (m > kLeaderMax) {
+          big[atomicAdd(&ctr->nbig, 1u)] = static_cast<uint32_t>(s);
+        } else {
+          for (uint32_t a = 1; a < m; ++a) {  // insertion sort by batch index
+            const uint32_t v = my[a * 256];
+            uint32_t c = a;
+            while (c > 0 && my[(c - 1) * 256] > v) {
+              my[c * 256] = my[(c - 1) * 256];
+              --c;
+            }
+            my[c * 256] = v;
+          }
+          for (uint32_t j = 0; j < m; ++j) {
+            const uint64_t k = my[j * 256];
+            claimed += insert_one<KS, VS>(t, in_keys + k * K, in_vals + k * V, K, V, ctr);
+          }
+        }
+      }
+    }
+    wave_add(&ctr->count, claimed);
+  }
+}
+
+// One workgroup per large closure: gather the members' indices from the arc
+// into LDS (in windows of kBigMax batch indices when there are more), sort
+// them (bitonic), and replay them in order from lane 0.
+template <int KS, int VS>
+__global__ void __launch_bounds__(1024)
+k_ins_big(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __restrict__ in_vals, uint64_t n,
+          const unsigned long long* __restrict__ Q, const uint32_t* __restrict__ big, Counters* ctr) {
+  __shared__ uint32_t lst[kBigMax];
+  __shared__ uint32_t cnt;
+  __shared__ unsigned long long barrier_at;
+  const uint32_t K = KS ? KS : t.K;
+  const uint32_t V = VS ? VS : t.V;
+  const unsigned long long* P = t.freebits;
+  const uint64_t words = (t.cap + 63) >> 6;
+  const uint32_t nbig = __hip_atomic_load(&ctr->nbig, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  for (uint32_t c = blockIdx.x; c < nbig; c += gridDim.x) {
+    const uint64_t s = big[c];
+    if (threadIdx.x == 0) barrier_at = bm_next_set(Q, s, t.cap);
+    __syncthreads();
+    const uint64_t arc = (barrier_at + t.cap - s) & t.mask;  // slots s .. barrier-1
+    const uint64_t w0 = s >> 6;
+    const uint64_t nwords = min(((s & 63) + arc + 63) >> 6, words);
+    uint64_t lo = 0, hi = ~0ull;
+    bool windowed = false;
+    while (lo < n) {
+      if (threadIdx.x == 0) cnt = 0;
+      __syncthreads();
+      for (uint64_t wi = threadIdx.x; wi < nwords; wi += blockDim.x) {
+        const uint64_t w = (w0 + wi) % words;
+        unsigned long long bits = P[w] & ~Q[w];
+        while (bits) {
+          const int b = __ffsll(bits) - 1;
+          bits &= bits - 1;
+          const uint64_t p = (w << 6) + b;
+          if (((p + t.cap - s) & t.mask) >= arc) continue;
+          const uint32_t idx = t.owner[p];
+          if (idx < lo || idx >= hi) continue;
+          const uint32_t k = atomicAdd(&cnt, 1u);
+          if (k < kBigMax) lst[k] = idx;
+        }
+      }
+      __syncthreads();
+      const uint32_t m = cnt;
+      if (m > kBigMax) {  // too many for one pass: windows of kBigMax indices
+        windowed = true;
+        hi = lo + kBigMax;
+        __syncthreads();
+        continue;
+      }
+      uint32_t p2 = 1;
+      while (p2 < m) p2 <<= 1;
+      for (uint32_t k = m + threadIdx.x; k < p2; k += blockDim.x) lst[k] = 0xFFFFFFFFu;
+      __syncthreads();
+      for (uint32_t size = 2; size <= p2; size <<= 1) {
+        for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+          for (uint32_t k = threadIdx.x; k < p2; k += blockDim.x) {
+            const uint32_t j = k ^ stride;
+            if (j > k) {
+              const uint32_t a = lst[k], b2 = lst[j];
+              if ((a > b2) == ((k & size) == 0)) {
+                lst[k] = b2;
+                lst[j] = a;
+              }
+            }
+          }
+          __syncthreads();
+        }
+      }
+      if (threadIdx.x == 0) {
+        uint32_t claimed = 0;
+        for (uint32_t j = 0; j < m; ++j) {
+          const uint64_t k = lst[j];
+          claimed += insert_one<KS, VS>(t, in_keys + k * K, in_vals + k * V, K, V, ctr);
+        }
+        if (claimed) atomicAdd(&ctr->count, static_cast<unsigned long long>(claimed));
+      }
+      __syncthreads();
+      if (!windowed) break;
+      lo = hi;
+      hi = lo + kBigMax;
+    }
+  }
+}
+
+// Publishes the batch's real claims (a subset of its dry claims) in both
+// bitmaps and resets owner[] for the next op.
+__global__ void __launch_bounds__(1024)
+k_ins_bits(Table t, const uint32_t* __restrict__ dslot, uint64_t n, const Counters* ctr) {
+  if (ctr->status & kStatusDryFail) return;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    const uint64_t s = dslot[i];
+    if ((t.flags[s] & 0xFu) == kOccupied) {
+      const unsigned long long bit = 1ull << (s & 63);
+      atomicAnd(&t.freebits[s >> 6], ~bit);
+      atomicAnd(&t.emptybits[s >> 6], ~bit);
+    }
+    t.owner[s] = kNoOwner;
   }
 }
 

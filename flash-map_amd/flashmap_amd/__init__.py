@@ -65,6 +65,10 @@ class GpuRequired(FlashMapError):
     """FlashMapError::GpuRequired (error.rs:32-33)."""
 
 
+class KernelStat(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 32), ("launches", ctypes.c_uint64), ("total_ms", ctypes.c_double)]
+
+
 class _Builder(ctypes.Structure):
     _fields_ = [
         ("capacity", ctypes.c_uint64),
@@ -136,6 +140,10 @@ def lib() -> ctypes.CDLL:
             "fm_synchronize": (ctypes.c_int, [vp]),
             "fm_export_slots": (ctypes.c_int, [vp, vp, vp, vp]),
             "fm_import_slots": (ctypes.c_int, [vp, vp, vp, vp, u64]),
+            "fm_copy_table": (ctypes.c_int, [vp, vp]),
+            "fm_profile_enable": (ctypes.c_int, [vp, ctypes.c_int]),
+            "fm_profile_read": (ctypes.c_int, [vp, ctypes.POINTER(KernelStat), u32,
+                                               ctypes.POINTER(ctypes.c_uint32)]),
             "fm_get_tuning": (ctypes.c_int, [vp, ctypes.POINTER(Tuning)]),
             "fm_set_tuning": (ctypes.c_int, [vp, ctypes.POINTER(Tuning)]),
         }
@@ -368,6 +376,21 @@ class FlashMap:
         k = np.ascontiguousarray(keys).view(np.uint8).reshape(-1)
         v = np.ascontiguousarray(vals).view(np.uint8).reshape(-1)
         _check(lib().fm_import_slots(self._h, _ptr(f), _ptr(k), _ptr(v), length))
+
+    def copy_from(self, src: "FlashMap"):
+        """Asynchronous device copy of src's table into this map."""
+        _check(lib().fm_copy_table(self._h, src._h))
+
+    def profile(self, enable: bool = True):
+        """Turns per-kernel event timing on/off (resets the totals)."""
+        _check(lib().fm_profile_enable(self._h, 1 if enable else 0))
+
+    def profile_read(self) -> dict:
+        """{kernel name: (launches, total device ms)} since profile()."""
+        arr = (KernelStat * 64)()
+        n = ctypes.c_uint32(0)
+        _check(lib().fm_profile_read(self._h, arr, 64, ctypes.byref(n)))
+        return {arr[i].name.decode(): (int(arr[i].launches), float(arr[i].total_ms)) for i in range(n.value)}
 
     def tuning(self) -> Tuning:
         t = Tuning()

@@ -176,6 +176,26 @@ fm_status fm_export_slots(fm_map* m, uint32_t* flags, void* keys,
 fm_status fm_import_slots(fm_map* m, const uint32_t* flags, const void* keys,
                           const void* values, uint64_t len);
 
+/* No reference counterpart: copies src's whole table (flags, keys, values,
+ * len) into dst, which must have the same device, capacity, K, V and hash
+ * strategy.  Asynchronous on dst's stream (src is quiesced first); used to
+ * restore a preloaded state between benchmark steps. */
+fm_status fm_copy_table(fm_map* dst, fm_map* src);
+
+/* ---- per-kernel timing (no reference counterpart) ----------------------
+ * When enabled, every kernel the map launches is bracketed by hipEvents on
+ * the map's stream; fm_profile_read waits for the stream and reports, per
+ * kernel name, the launch count and summed device milliseconds since the
+ * last fm_profile_enable. */
+typedef struct fm_kernel_stat {
+  char name[32];
+  uint64_t launches;
+  double total_ms;
+} fm_kernel_stat;
+fm_status fm_profile_enable(fm_map* m, int enable);
+fm_status fm_profile_read(fm_map* m, fm_kernel_stat* out, uint32_t max,
+                          uint32_t* count);
+
 /* ---- launch tuning (no reference counterpart) --------------------------
  * group: lanes that probe one key together (1..64, power of two);
  * block: threads per workgroup (64..1024, multiple of 64). */

@@ -58,6 +58,21 @@ def test_gpu_vs_oracle_random_histories(seed):
     apply_and_compare(gpu_make(cap, K, V, mode, group=group), OracleAdapter(cap, K, V, mode), ops)
 
 
+@pytest.mark.parametrize("seed", range(6))
+def test_gpu_sort_grouped_insert_path(seed, monkeypatch):
+    """The radix-sort grouping (used when a batch may fill the table) on
+    ordinary histories, forced with FM_INSERT_SORT_PATH=1 at map creation."""
+    monkeypatch.setenv("FM_INSERT_SORT_PATH", "1")
+    rng = np.random.default_rng(2000 + seed)
+    cap = [64, 256, 1024][seed % 3]
+    K, V = [(8, 8), (32, 128), (3, 5)][seed % 3]
+    say(f"sort path seed={seed} cap={cap} K={K} V={V}")
+    ops = random_history(rng, cap, K, V, 20, max(1, cap // 2), cap * 2)
+    m = gpu_make(cap, K, V, seed % 2)
+    monkeypatch.delenv("FM_INSERT_SORT_PATH")
+    apply_and_compare(m, OracleAdapter(cap, K, V, seed % 2), ops)
+
+
 @pytest.mark.parametrize("seed", range(4))
 def test_gpu_fill_to_full_no_barrier(seed):
     """Batches that consume every free slot (single closure, no barrier)."""
