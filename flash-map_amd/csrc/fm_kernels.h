@@ -359,7 +359,10 @@ k_ins_closure(uint64_t cap, const unsigned long long* __restrict__ Q, const uint
 // the walk ends within cap steps.  Only a table with no free slot at all
 // (reachable through the uncounted device insert) can exceed the budget; the
 // walk then stops and flags kStatusBudget instead of circling the ring.
-template <int KS, int VS>
+// kPublishBits: clear the claimed slot's bitmap bits here.  The leader path
+// passes false: there t.freebits is the pre-batch set other lanes are still
+// scanning, and k_ins_bits publishes the claims after the replay.
+template <int KS, int VS, bool kPublishBits = true>
 __device__ inline uint32_t insert_one(const Table& t, const uint8_t* __restrict__ kin, const uint8_t* __restrict__ vin,
                                       uint32_t K, uint32_t V, Counters* ctr) {
   Blob<KS> ck;
@@ -405,9 +408,11 @@ __device__ inline uint32_t insert_one(const Table& t, const uint8_t* __restrict_
     ck.store(t.keys + idx * K, K);
     cv.store(t.vals + idx * V, V);
     t.flags[idx] = make_flag(kOccupied, p);
-    const unsigned long long bit = 1ull << (idx & 63);
-    atomicAnd(&t.freebits[idx >> 6], ~bit);
-    atomicAnd(&t.emptybits[idx >> 6], ~bit);
+    if constexpr (kPublishBits) {
+      const unsigned long long bit = 1ull << (idx & 63);
+      atomicAnd(&t.freebits[idx >> 6], ~bit);
+      atomicAnd(&t.emptybits[idx >> 6], ~bit);
+    }
     return 1;
   }
 }
@@ -512,7 +517,7 @@ k_ins_leader(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __rest
           }
           for (uint32_t j = 0; j < m; ++j) {
             const uint64_t k = my[j * 256];
-            claimed += insert_one<KS, VS>(t, in_keys + k * K, in_vals + k * V, K, V, ctr);
+            claimed += insert_one<KS, VS, false>(t, in_keys + k * K, in_vals + k * V, K, V, ctr);
           }
         }
       }
@@ -593,7 +598,7 @@ k_ins_big(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __restric
         uint32_t claimed = 0;
         for (uint32_t j = 0; j < m; ++j) {
           const uint64_t k = lst[j];
-          claimed += insert_one<KS, VS>(t, in_keys + k * K, in_vals + k * V, K, V, ctr);
+          claimed += insert_one<KS, VS, false>(t, in_keys + k * K, in_vals + k * V, K, V, ctr);
         }
         if (claimed) atomicAdd(&ctr->count, static_cast<unsigned long long>(claimed));
       }

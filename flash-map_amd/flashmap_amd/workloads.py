@@ -186,3 +186,134 @@ def config2(n_preload=8_000_000, n_batches=20, batch=100_000, n_get=1_000_000, h
     cfg.get_keys = gk[st.permutation(n_get)]
     cfg.remove_keys = pk[st.sample(n_preload, n_remove)]
     return cfg
+
+
+# ---------------------------------------------------------------------------
+# Config 3: FlashMap<[u8;32],u64> with_capacity(50,000,000), seed 3.
+# Load keys with value = row id; probes: 90% Zipf(1.0) over the inserted keys
+# (rank -> key through a random permutation), 10% fresh misses, shuffled.
+# ---------------------------------------------------------------------------
+@dataclass
+class Config3:
+    capacity: int
+    keys: np.ndarray     # (n, 32) u8
+    row_ids: np.ndarray  # (n,) u64
+    probes: np.ndarray   # (m, 32) u8
+    probe_hit: np.ndarray  # (m,) bool -- True where the probe is an inserted key
+
+
+def config3(n_load=40_000_000, n_probe=100_000_000, zipf_s=1.0, hit=0.9, capacity=50_000_000, seed=3,
+            K=32) -> Config3:
+    st = Stream(seed)
+    keys = st.bytes(n_load, K)
+    rows = np.arange(n_load, dtype=np.uint64)
+    n_hit = int(round(n_probe * hit))
+    perm = st.permutation(n_load)
+    ranks = st.zipf(n_load, zipf_s, n_hit) - 1
+    hits = keys[perm[ranks]]
+    miss = st.bytes(n_probe - n_hit, K)
+    order = st.permutation(n_probe)
+    probes = np.concatenate([hits, miss])[order]
+    is_hit = np.concatenate([np.ones(n_hit, bool), np.zeros(n_probe - n_hit, bool)])[order]
+    return Config3(capacity, keys, rows, probes, is_hit)
+
+
+# ---------------------------------------------------------------------------
+# Config 4: 512 independent FlashMap<u64,[u8;32]> tables, seed 4.
+# ---------------------------------------------------------------------------
+@dataclass
+class TableWork:
+    capacity: int           # rounded slot count
+    preload_keys: np.ndarray  # u64
+    preload_vals: np.ndarray  # (n, 32) u8
+    ins_keys: np.ndarray
+    ins_vals: np.ndarray
+    get_keys: np.ndarray
+    rem_keys: np.ndarray
+
+
+def _round_cap(c):
+    c = max(16, int(c))
+    return 1 << (c - 1).bit_length()
+
+
+def config4_table(t: int, seed=4, cap_lo=1 << 10, cap_hi=1 << 22, ins_hi=1 << 16, get_hi=1 << 16,
+                  rem_hi=1 << 12, V=32) -> TableWork:
+    """Table t of config 4 (each table has its own sub-stream seed*1000003+t)."""
+    st = Stream(seed * 1000003 + t)
+    cap = _round_cap(int(st.log_uniform(cap_lo, cap_hi, 1)[0]))
+    lf = 0.5 + 0.35 * float(st.unit(1)[0])
+    n_pre = int(lf * cap)
+    raw = st.u64(n_pre + 64)
+    _, first = np.unique(raw, return_index=True)
+    pre = raw[np.sort(first)][:n_pre]
+    pre_v = st.bytes(n_pre, V)
+    n_ins = int(st.log_uniform(1, ins_hi, 1)[0])
+    n_over = min(int(round(0.3 * n_ins)), n_pre)
+    over = pre[st.sample(n_pre, n_over)] if n_over else np.zeros(0, np.uint64)
+    fresh = st.u64(n_ins - n_over)
+    ik = np.concatenate([over, fresh])
+    ik = ik[st.permutation(ik.size)]
+    iv = st.bytes(ik.size, V)
+    n_get = int(st.log_uniform(1, get_hi, 1)[0])
+    n_hit = int(round(0.7 * n_get))
+    gk = np.concatenate([pre[st.below(max(n_pre, 1), n_hit).astype(np.int64)] if n_pre else
+                         np.zeros(0, np.uint64), st.u64(n_get - n_hit)])
+    gk = gk[st.permutation(gk.size)]
+    n_rem = min(int(st.log_uniform(1, rem_hi, 1)[0]), n_pre)
+    rk = pre[st.sample(n_pre, n_rem)] if n_rem else np.zeros(0, np.uint64)
+    return TableWork(cap, pre, pre_v, ik, iv, gk, rk)
+
+
+# ---------------------------------------------------------------------------
+# Config 5: serving log over one FlashMap<[u8;32],u64> with_capacity(24M),
+# seed 5.  Requests arrive as a Poisson stream; 70% get / 20% insert /
+# 10% remove; sizes log-uniform in [64, 262144]; get keys 80% hits under
+# Zipf(0.9) over the preloaded keys (random rank permutation), 20% misses;
+# inserts 50% overwrites (Zipf) + 50% new keys; removes 80% uniform hits,
+# 20% misses.
+# ---------------------------------------------------------------------------
+@dataclass
+class Request:
+    op: str              # "get" | "insert" | "remove"
+    arrival_s: float
+    keys: np.ndarray     # (n, 32) u8
+    vals: np.ndarray = None  # (n,) u64 for inserts
+
+
+def config5_preload(n_preload=20_000_000, capacity=24_000_000, seed=5, K=32):
+    st = Stream(seed)
+    keys = st.bytes(n_preload, K)
+    vals = st.u64(n_preload)
+    return capacity, keys, vals
+
+
+def config5_requests(preload_keys: np.ndarray, rate: float, n_requests: int, seed=5,
+                     size_lo=64, size_hi=262_144, zipf_s=0.9) -> List[Request]:
+    st = Stream(seed * 7919 + int(rate))
+    n_pre = preload_keys.shape[0]
+    perm = st.permutation(n_pre)
+    gaps = -np.log1p(-st.unit(n_requests)) / rate
+    arrivals = np.cumsum(gaps)
+    kinds = st.unit(n_requests)
+    sizes = st.log_uniform(size_lo, size_hi, n_requests)
+    out = []
+    for r in range(n_requests):
+        n = int(sizes[r])
+        u = kinds[r]
+        if u < 0.7:
+            n_hit = int(round(0.8 * n))
+            hk = preload_keys[perm[st.zipf(n_pre, zipf_s, n_hit) - 1]]
+            ks = np.concatenate([hk, st.bytes(n - n_hit, preload_keys.shape[1])])
+            out.append(Request("get", float(arrivals[r]), ks[st.permutation(n)]))
+        elif u < 0.9:
+            n_over = n // 2
+            ok = preload_keys[perm[st.zipf(n_pre, zipf_s, n_over) - 1]]
+            ks = np.concatenate([ok, st.bytes(n - n_over, preload_keys.shape[1])])
+            out.append(Request("insert", float(arrivals[r]), ks[st.permutation(n)], st.u64(n)))
+        else:
+            n_hit = int(round(0.8 * n))
+            hk = preload_keys[st.below(n_pre, n_hit).astype(np.int64)]
+            ks = np.concatenate([hk, st.bytes(n - n_hit, preload_keys.shape[1])])
+            out.append(Request("remove", float(arrivals[r]), ks[st.permutation(n)]))
+    return out
