@@ -341,7 +341,7 @@ k_ins_dry(Table t, const uint8_t* __restrict__ in_keys, uint64_t n, unsigned lon
 // Pass 2: closure id = first barrier (bit still set in Q) at/after home.
 // With no barrier (the batch fills every free slot) or a failed dry pass,
 // every key gets the same id and one lane replays the whole batch in order.
-__global__ void __launch_bounds__(1024)
+static __global__ void __launch_bounds__(1024)
 k_ins_closure(uint64_t cap, const unsigned long long* __restrict__ Q, const uint32_t* __restrict__ home, uint64_t n,
               uint32_t* __restrict__ cid, uint32_t* __restrict__ idx, const Counters* ctr) {
   const bool failed = (ctr->status & kStatusDryFail) != 0;
@@ -612,7 +612,7 @@ k_ins_big(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __restric
 
 // Publishes the batch's real claims (a subset of its dry claims) in both
 // bitmaps and resets owner[] for the next op.
-__global__ void __launch_bounds__(1024)
+static __global__ void __launch_bounds__(1024)
 k_ins_bits(Table t, const uint32_t* __restrict__ dslot, uint64_t n, const Counters* ctr) {
   if (ctr->status & kStatusDryFail) return;
   for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
@@ -658,7 +658,7 @@ k_rem_probe(Table t, const uint8_t* __restrict__ qkeys, uint64_t n, unsigned lon
 // Pass 2: conflict detection.  Key i keeps its pre-batch event unless an
 // earlier key tombstones that slot: a duplicate match (owner != i) or an
 // occupied terminator removed by an earlier key (owner < i).
-__global__ void __launch_bounds__(1024)
+static __global__ void __launch_bounds__(1024)
 k_rem_check(Table t, const unsigned long long* __restrict__ ev, uint64_t n, uint8_t* __restrict__ action,
             Counters* ctr) {
   for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
@@ -680,7 +680,7 @@ k_rem_check(Table t, const unsigned long long* __restrict__ ev, uint64_t n, uint
 }
 
 // Pass 3: apply (only when the batch had no conflict) and reset owner[].
-__global__ void __launch_bounds__(1024)
+static __global__ void __launch_bounds__(1024)
 k_rem_apply(Table t, const unsigned long long* __restrict__ ev, const uint8_t* __restrict__ action, uint64_t n,
             Counters* ctr) {
   const bool conflict = (ctr->status & kStatusConflict) != 0;
@@ -763,14 +763,14 @@ k_rem_replay(Table t, const uint8_t* __restrict__ qkeys, uint64_t n, const uint3
 // ---------------------------------------------------------------------------
 // Table maintenance.
 // ---------------------------------------------------------------------------
-__global__ void k_fill_u32(uint32_t* __restrict__ p, uint64_t n, uint32_t v) {
+static __global__ void k_fill_u32(uint32_t* __restrict__ p, uint64_t n, uint32_t v) {
   for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
        i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
     p[i] = v;
 }
 
 // clear (kernels.cu:319-326): flags EMPTY; both bitmaps all-free.
-__global__ void k_clear(Table t) {
+static __global__ void k_clear(Table t) {
   const uint64_t words = (t.cap + 63) >> 6;
   const unsigned long long full = t.cap >= 64 ? ~0ull : ((1ull << t.cap) - 1ull);
   for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < t.cap;
@@ -784,7 +784,7 @@ __global__ void k_clear(Table t) {
 }
 
 // Rebuilds both bitmaps from flags (after an import): one wave per 64 slots.
-__global__ void k_rebuild_bits(Table t) {
+static __global__ void k_rebuild_bits(Table t) {
   const uint64_t words = (t.cap + 63) >> 6;
   const uint32_t lane = threadIdx.x & 63u;
   const uint64_t wave = (static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
@@ -802,7 +802,7 @@ __global__ void k_rebuild_bits(Table t) {
 }
 
 // recount (kernels.cu:328-337): OCCUPIED slots, one atomic per wave.
-__global__ void k_count(Table t, Counters* ctr) {
+static __global__ void k_count(Table t, Counters* ctr) {
   for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * blockDim.x;; base += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
     if (base >= t.cap) break;
     const uint64_t i = base + threadIdx.x;

@@ -208,6 +208,70 @@ typedef struct fm_tuning {
 fm_status fm_get_tuning(const fm_map* m, fm_tuning* out);
 fm_status fm_set_tuning(fm_map* m, const fm_tuning* t);
 
+/* ---- table groups: many independent maps, fused launches ---------------
+ * No reference counterpart (the reference launches once per call per map,
+ * gpu.rs:141-167).  A group holds n independent tables with one key/value
+ * size and hash strategy on one device; each op runs over all tables in a
+ * few persistent launches, and fm_group_step replays one captured hipGraph
+ * for insert -> get -> remove on every table.  Per table the results equal n
+ * separate FlashMaps applying the same batches in the same order, TableFull
+ * included (that table's insert batch is skipped, status FM_ERR_TABLE_FULL).
+ *
+ * Batches are concatenated in table order in device buffers; counts[t] is
+ * the number of keys for table t (0 allowed).  Outputs per key follow the
+ * same concatenation; per-table outputs are host arrays of n entries. */
+typedef struct fm_group fm_group;
+
+typedef struct fm_group_tuning {
+  uint32_t group;   /* lanes per key in get/remove probes: 1,2,4,8,16     */
+  uint32_t block;   /* threads per workgroup = keys per chunk: 64..512    */
+  uint32_t grid;    /* persistent workgroups per launch: 64..8192         */
+} fm_group_tuning;
+
+fm_status fm_group_create(uint32_t n_tables, const uint64_t* capacities,
+                          uint32_t key_size, uint32_t value_size,
+                          uint32_t hash_strategy, int32_t device_id,
+                          fm_group** out);
+void fm_group_destroy(fm_group* g);
+uint32_t fm_group_tables(const fm_group* g);
+uint64_t fm_group_len(const fm_group* g, uint32_t table);
+uint64_t fm_group_capacity(const fm_group* g, uint32_t table);
+void* fm_group_stream(fm_group* g);
+fm_status fm_group_alloc_device(fm_group* g, uint64_t bytes, void** out);
+fm_status fm_group_free_device(fm_group* g, void* p);
+fm_status fm_group_upload(fm_group* g, const void* host, uint64_t bytes,
+                          void** out);
+fm_status fm_group_download(fm_group* g, const void* dptr, uint64_t bytes,
+                            void* host);
+/* bulk_insert on every table (out_new, out_status: n entries each). */
+fm_status fm_group_bulk_insert_device(fm_group* g, const void* d_keys,
+                                      const void* d_values,
+                                      const uint64_t* counts,
+                                      uint64_t* out_new, int32_t* out_status);
+/* bulk_get on every table (outputs per key, misses zeroed). */
+fm_status fm_group_bulk_get_device(fm_group* g, const void* d_keys,
+                                   const uint64_t* counts,
+                                   void* d_out_values, uint8_t* d_out_found);
+/* bulk_remove on every table (out_removed: n entries). */
+fm_status fm_group_bulk_remove_device(fm_group* g, const void* d_keys,
+                                      const uint64_t* counts,
+                                      uint64_t* out_removed);
+/* insert -> get -> remove on every table as one hipGraph launch (captured
+ * on first use and re-captured when buffers or tuning change). */
+fm_status fm_group_step_device(fm_group* g, const void* d_ins_keys,
+                               const void* d_ins_values,
+                               const uint64_t* ins_counts,
+                               const void* d_get_keys,
+                               const uint64_t* get_counts, void* d_out_values,
+                               uint8_t* d_out_found, const void* d_rem_keys,
+                               const uint64_t* rem_counts, uint64_t* out_new,
+                               int32_t* out_status, uint64_t* out_removed);
+fm_status fm_group_export_slots(fm_group* g, uint32_t table, uint32_t* flags,
+                                void* keys, void* values);
+fm_status fm_group_copy(fm_group* dst, fm_group* src);
+fm_status fm_group_get_tuning(const fm_group* g, fm_group_tuning* out);
+fm_status fm_group_set_tuning(fm_group* g, const fm_group_tuning* t);
+
 #ifdef __cplusplus
 }
 #endif
