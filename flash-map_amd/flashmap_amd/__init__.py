@@ -65,6 +65,10 @@ class GpuRequired(FlashMapError):
     """FlashMapError::GpuRequired (error.rs:32-33)."""
 
 
+class GroupTuning(ctypes.Structure):
+    _fields_ = [("group", ctypes.c_uint32), ("block", ctypes.c_uint32), ("grid", ctypes.c_uint32)]
+
+
 class KernelStat(ctypes.Structure):
     _fields_ = [("name", ctypes.c_char * 32), ("launches", ctypes.c_uint64), ("total_ms", ctypes.c_double)]
 
@@ -141,6 +145,25 @@ def lib() -> ctypes.CDLL:
             "fm_export_slots": (ctypes.c_int, [vp, vp, vp, vp]),
             "fm_import_slots": (ctypes.c_int, [vp, vp, vp, vp, u64]),
             "fm_copy_table": (ctypes.c_int, [vp, vp]),
+            "fm_group_create": (ctypes.c_int, [u32, pu64, u32, u32, u32, i32, ctypes.POINTER(vp)]),
+            "fm_group_destroy": (None, [vp]),
+            "fm_group_tables": (u32, [vp]),
+            "fm_group_len": (u64, [vp, u32]),
+            "fm_group_capacity": (u64, [vp, u32]),
+            "fm_group_stream": (vp, [vp]),
+            "fm_group_alloc_device": (ctypes.c_int, [vp, u64, ctypes.POINTER(vp)]),
+            "fm_group_free_device": (ctypes.c_int, [vp, vp]),
+            "fm_group_upload": (ctypes.c_int, [vp, vp, u64, ctypes.POINTER(vp)]),
+            "fm_group_download": (ctypes.c_int, [vp, vp, u64, vp]),
+            "fm_group_bulk_insert_device": (ctypes.c_int, [vp, vp, vp, pu64, pu64, ctypes.POINTER(i32)]),
+            "fm_group_bulk_get_device": (ctypes.c_int, [vp, vp, pu64, vp, vp]),
+            "fm_group_bulk_remove_device": (ctypes.c_int, [vp, vp, pu64, pu64]),
+            "fm_group_step_device": (ctypes.c_int, [vp, vp, vp, pu64, vp, pu64, vp, vp, vp, pu64, pu64,
+                                                    ctypes.POINTER(i32), pu64]),
+            "fm_group_export_slots": (ctypes.c_int, [vp, u32, vp, vp, vp]),
+            "fm_group_copy": (ctypes.c_int, [vp, vp]),
+            "fm_group_get_tuning": (ctypes.c_int, [vp, ctypes.POINTER(GroupTuning)]),
+            "fm_group_set_tuning": (ctypes.c_int, [vp, ctypes.POINTER(GroupTuning)]),
             "fm_profile_enable": (ctypes.c_int, [vp, ctypes.c_int]),
             "fm_profile_read": (ctypes.c_int, [vp, ctypes.POINTER(KernelStat), u32,
                                                ctypes.POINTER(ctypes.c_uint32)]),
@@ -404,6 +427,141 @@ class FlashMap:
             if val is not None:
                 setattr(t, name, val)
         _check(lib().fm_set_tuning(self._h, ctypes.byref(t)))
+
+
+class GroupBuffer:
+    """A device allocation owned by a FlashMapGroup."""
+
+    def __init__(self, owner: "FlashMapGroup", ptr: int, nbytes: int):
+        self.owner, self.ptr, self.nbytes = owner, ptr, nbytes
+
+    def free(self):
+        if self.ptr and self.owner._h:
+            lib().fm_group_free_device(self.owner._h, ctypes.c_void_p(self.ptr))
+        self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def _u64s(xs) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(xs, dtype=np.uint64))
+
+
+class FlashMapGroup:
+    """n independent FlashMaps on one GPU driven by fused launches
+    (fm_group_* in include/flashmap.h).  Batches are concatenated in table
+    order; ``counts[t]`` keys belong to table t."""
+
+    def __init__(self, capacities, key_size: int, value_size: int, hash_strategy: int = 0, device_id: int = 0):
+        caps = _u64s(capacities)
+        h = ctypes.c_void_p()
+        self._h = None
+        _check(lib().fm_group_create(caps.size, caps.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), key_size,
+                                     value_size, hash_strategy, device_id, ctypes.byref(h)))
+        self._h = h
+        self.n = caps.size
+        self.key_size, self.value_size = key_size, value_size
+
+    def close(self):
+        if self._h:
+            lib().fm_group_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def len(self, t: int) -> int:
+        return int(lib().fm_group_len(self._h, t))
+
+    def capacity(self, t: int) -> int:
+        return int(lib().fm_group_capacity(self._h, t))
+
+    def stream(self) -> int:
+        return int(lib().fm_group_stream(self._h) or 0)
+
+    def alloc_device(self, nbytes: int) -> GroupBuffer:
+        p = ctypes.c_void_p()
+        _check(lib().fm_group_alloc_device(self._h, nbytes, ctypes.byref(p)))
+        return GroupBuffer(self, p.value or 0, nbytes)
+
+    def upload(self, data) -> GroupBuffer:
+        arr, _ = _as_bytes(data, 1, "data")
+        p = ctypes.c_void_p()
+        _check(lib().fm_group_upload(self._h, _ptr(arr), arr.size, ctypes.byref(p)))
+        return GroupBuffer(self, p.value or 0, arr.size)
+
+    def download(self, buf: GroupBuffer, nbytes: Optional[int] = None) -> np.ndarray:
+        nbytes = buf.nbytes if nbytes is None else nbytes
+        out = np.zeros(nbytes, dtype=np.uint8)
+        _check(lib().fm_group_download(self._h, ctypes.c_void_p(buf.ptr), nbytes, _ptr(out)))
+        return out
+
+    @staticmethod
+    def _p64(a):
+        return a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))
+
+    def bulk_insert_device(self, d_keys: GroupBuffer, d_vals: GroupBuffer, counts):
+        c = _u64s(counts)
+        new = np.zeros(self.n, dtype=np.uint64)
+        status = np.zeros(self.n, dtype=np.int32)
+        _check(lib().fm_group_bulk_insert_device(self._h, ctypes.c_void_p(d_keys.ptr), ctypes.c_void_p(d_vals.ptr),
+                                                 self._p64(c), self._p64(new),
+                                                 status.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
+        return new, status
+
+    def bulk_get_device(self, d_keys: GroupBuffer, counts, d_out: GroupBuffer, d_found: GroupBuffer):
+        c = _u64s(counts)
+        _check(lib().fm_group_bulk_get_device(self._h, ctypes.c_void_p(d_keys.ptr), self._p64(c),
+                                              ctypes.c_void_p(d_out.ptr), ctypes.c_void_p(d_found.ptr)))
+
+    def bulk_remove_device(self, d_keys: GroupBuffer, counts):
+        c = _u64s(counts)
+        rem = np.zeros(self.n, dtype=np.uint64)
+        _check(lib().fm_group_bulk_remove_device(self._h, ctypes.c_void_p(d_keys.ptr), self._p64(c), self._p64(rem)))
+        return rem
+
+    def step_device(self, ins_keys, ins_vals, ins_counts, get_keys, get_counts, d_out, d_found, rem_keys,
+                    rem_counts):
+        ic, gc, rc = _u64s(ins_counts), _u64s(get_counts), _u64s(rem_counts)
+        new = np.zeros(self.n, dtype=np.uint64)
+        status = np.zeros(self.n, dtype=np.int32)
+        rem = np.zeros(self.n, dtype=np.uint64)
+        vp = ctypes.c_void_p
+        _check(lib().fm_group_step_device(self._h, vp(ins_keys.ptr), vp(ins_vals.ptr), self._p64(ic),
+                                          vp(get_keys.ptr), self._p64(gc), vp(d_out.ptr), vp(d_found.ptr),
+                                          vp(rem_keys.ptr), self._p64(rc), self._p64(new),
+                                          status.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), self._p64(rem)))
+        return new, status, rem
+
+    def export_slots(self, t: int):
+        cap = self.capacity(t)
+        flags = np.zeros(cap, dtype=np.uint32)
+        keys = np.zeros(cap * self.key_size, dtype=np.uint8)
+        vals = np.zeros(cap * self.value_size, dtype=np.uint8)
+        _check(lib().fm_group_export_slots(self._h, t, _ptr(flags), _ptr(keys), _ptr(vals)))
+        return flags, keys, vals
+
+    def copy_from(self, src: "FlashMapGroup"):
+        _check(lib().fm_group_copy(self._h, src._h))
+
+    def tuning(self) -> GroupTuning:
+        t = GroupTuning()
+        _check(lib().fm_group_get_tuning(self._h, ctypes.byref(t)))
+        return t
+
+    def set_tuning(self, group=None, block=None, grid=None):
+        t = self.tuning()
+        for name, val in (("group", group), ("block", block), ("grid", grid)):
+            if val is not None:
+                setattr(t, name, val)
+        _check(lib().fm_group_set_tuning(self._h, ctypes.byref(t)))
 
 
 class FlashMapBuilder:

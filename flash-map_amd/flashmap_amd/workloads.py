@@ -33,11 +33,18 @@ _M2 = np.uint64(0x94D049BB133111EB)
 def splitmix64(seed: int, start: int, count: int) -> np.ndarray:
     """Outputs start..start+count-1 of the SplitMix64 stream seeded `seed`."""
     with np.errstate(over="ignore"):
-        i = np.arange(start + 1, start + count + 1, dtype=np.uint64)
-        z = np.uint64(seed) + i * GOLDEN
-        z = (z ^ (z >> np.uint64(30))) * _M1
-        z = (z ^ (z >> np.uint64(27))) * _M2
-        return z ^ (z >> np.uint64(31))
+        z = np.arange(start + 1, start + count + 1, dtype=np.uint64)
+        z *= GOLDEN
+        z += np.uint64(seed)
+        t = z >> np.uint64(30)
+        z ^= t
+        z *= _M1
+        np.right_shift(z, np.uint64(27), out=t)
+        z ^= t
+        z *= _M2
+        np.right_shift(z, np.uint64(31), out=t)
+        z ^= t
+        return z
 
 
 class Stream:
@@ -69,7 +76,15 @@ class Stream:
         return np.argsort(self.u64(m), kind="stable")
 
     def sample(self, m: int, k: int) -> np.ndarray:
-        """k distinct indices of range(m), in random order."""
+        """k distinct indices of range(m), in random order.  For k << m:
+        2k+64 bounded draws with repeats dropped (first occurrences kept);
+        otherwise (or if too few survive) the first k of a permutation."""
+        if 8 * k < m < (1 << 32):
+            draws = self.below(m, 2 * k + 64).astype(np.int64)
+            _, first = np.unique(draws, return_index=True)
+            idx = draws[np.sort(first)]
+            if idx.size >= k:
+                return idx[:k]
         return self.permutation(m)[:k]
 
     def log_uniform(self, a: int, b: int, n: int) -> np.ndarray:
@@ -244,9 +259,9 @@ def config4_table(t: int, seed=4, cap_lo=1 << 10, cap_hi=1 << 22, ins_hi=1 << 16
     cap = _round_cap(int(st.log_uniform(cap_lo, cap_hi, 1)[0]))
     lf = 0.5 + 0.35 * float(st.unit(1)[0])
     n_pre = int(lf * cap)
-    raw = st.u64(n_pre + 64)
-    _, first = np.unique(raw, return_index=True)
-    pre = raw[np.sort(first)][:n_pre]
+    # SplitMix64's output is a bijection of its counter: draws from one
+    # stream never repeat, so preload and fresh keys are distinct by design.
+    pre = st.u64(n_pre)
     pre_v = st.bytes(n_pre, V)
     n_ins = int(st.log_uniform(1, ins_hi, 1)[0])
     n_over = min(int(round(0.3 * n_ins)), n_pre)
