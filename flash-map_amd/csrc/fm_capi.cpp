@@ -12,11 +12,6 @@
 
 #include "fm_internal.h"
 
-struct fm_map {
-  std::unique_ptr<fm::Backend> be;
-  mutable std::mutex mu;
-};
-
 namespace fm {
 namespace {
 thread_local fm_status t_code = FM_OK;

@@ -208,6 +208,28 @@ class GpuBackend final : public Backend {
   fm_status recount(uint64_t* out) override;
   fm_status synchronize() override;
   fm_status copy_from(Backend& src) override;
+  fm_status serve_get(const void* d_keys, uint64_t n, void* d_out, uint8_t* d_found) override {
+    fm_status st;
+    if ((st = use_device()) != FM_OK) return st;
+    return run_get(static_cast<const uint8_t*>(d_keys), n, static_cast<uint8_t*>(d_out), d_found);
+  }
+  fm_status serve_insert(const void* d_keys, const void* d_vals, uint64_t n, uint8_t* d_claimed) override {
+    fm_status st;
+    if ((st = use_device()) != FM_OK) return st;
+    if ((st = run_insert(static_cast<const uint8_t*>(d_keys), static_cast<const uint8_t*>(d_vals), n, d_claimed)) !=
+        FM_OK)
+      return st;
+    len_ += n;
+    return FM_OK;
+  }
+  void serve_settle_insert(uint64_t n, uint64_t claimed) override { len_ -= std::min(len_, n - claimed); }
+  fm_status serve_remove(const void* d_keys, uint64_t n, uint8_t* d_removed, uint64_t* removed) override {
+    fm_status st;
+    if ((st = use_device()) != FM_OK) return st;
+    if ((st = run_remove(static_cast<const uint8_t*>(d_keys), n, removed, d_removed)) != FM_OK) return st;
+    len_ -= std::min(len_, *removed);
+    return FM_OK;
+  }
   fm_status profile_enable(bool on) override;
   fm_status profile_read(fm_kernel_stat* out, uint32_t max, uint32_t* count) override;
   fm_status get_tuning(fm_tuning* out) const override {
@@ -249,8 +271,9 @@ class GpuBackend final : public Backend {
 
   // Device pipelines (inputs on the device, ordered on stream_).
   fm_status run_get(const uint8_t* d_keys, uint64_t n, uint8_t* d_out, uint8_t* d_found);
-  fm_status run_insert(const uint8_t* d_keys, const uint8_t* d_vals, uint64_t n);  // leaves count in ctr
-  fm_status run_remove(const uint8_t* d_keys, uint64_t n, uint64_t* removed);
+  fm_status run_insert(const uint8_t* d_keys, const uint8_t* d_vals, uint64_t n,
+                       uint8_t* outcome = nullptr);  // leaves count in ctr
+  fm_status run_remove(const uint8_t* d_keys, uint64_t n, uint64_t* removed, uint8_t* outcome = nullptr);
 
   void release();
 
@@ -528,8 +551,10 @@ fm_status GpuBackend::run_get(const uint8_t* d_keys, uint64_t n, uint8_t* d_out,
   return check_launch(ops_.get(t_, d_keys, n, d_out, d_found, tune_.get_group, tune_.get_block, stream_), "k_get");
 }
 
-fm_status GpuBackend::run_insert(const uint8_t* d_keys, const uint8_t* d_vals, uint64_t n) {
+fm_status GpuBackend::run_insert(const uint8_t* d_keys, const uint8_t* d_vals, uint64_t n, uint8_t* outcome) {
   fm_status st;
+  Table t = t_;
+  t.outcome = outcome;
   if ((st = ensure_scratch(n)) != FM_OK) return st;
   if ((st = zero_counters()) != FM_OK) return st;
   const uint64_t words = (t_.cap + 63) / 64;
@@ -553,13 +578,13 @@ fm_status GpuBackend::run_insert(const uint8_t* d_keys, const uint8_t* d_vals, u
     }
     {
       Prof p(this, kKidInsLeader);
-      if ((st = check_launch(ops_.ins_leader(t_, d_keys, d_vals, n, Q_, dslot_, big_, ctr_, stream_),
+      if ((st = check_launch(ops_.ins_leader(t, d_keys, d_vals, n, Q_, dslot_, big_, ctr_, stream_),
                              "k_ins_leader")) != FM_OK)
         return st;
     }
     {
       Prof p(this, kKidInsBig);
-      if ((st = check_launch(ops_.ins_big(t_, d_keys, d_vals, n, Q_, big_, ctr_, stream_), "k_ins_big")) != FM_OK)
+      if ((st = check_launch(ops_.ins_big(t, d_keys, d_vals, n, Q_, big_, ctr_, stream_), "k_ins_big")) != FM_OK)
         return st;
     }
     Prof p(this, kKidInsBits);
@@ -585,11 +610,13 @@ fm_status GpuBackend::run_insert(const uint8_t* d_keys, const uint8_t* d_vals, u
     if (e != hipSuccess) return fail(FM_ERR_KERNEL_LAUNCH, hip_msg("radix sort", e));
   }
   Prof p(this, kKidInsReplay);
-  return check_launch(ops_.ins_replay(t_, d_keys, d_vals, n, cid_s_, idx_s_, ctr_, blk, stream_), "k_ins_replay");
+  return check_launch(ops_.ins_replay(t, d_keys, d_vals, n, cid_s_, idx_s_, ctr_, blk, stream_), "k_ins_replay");
 }
 
-fm_status GpuBackend::run_remove(const uint8_t* d_keys, uint64_t n, uint64_t* removed) {
+fm_status GpuBackend::run_remove(const uint8_t* d_keys, uint64_t n, uint64_t* removed, uint8_t* outcome) {
   fm_status st;
+  Table t = t_;
+  t.outcome = outcome;
   if ((st = ensure_scratch(n)) != FM_OK) return st;
   if ((st = zero_counters()) != FM_OK) return st;
   const uint32_t blk = tune_.insert_block;
@@ -606,7 +633,7 @@ fm_status GpuBackend::run_remove(const uint8_t* d_keys, uint64_t n, uint64_t* re
   }
   {
     Prof p(this, kKidRemApply);
-    hipLaunchKernelGGL(dev::k_rem_apply, dim3(grid_for(n, blk)), dim3(blk), 0, stream_, t_, ev_, action_, n, ctr_);
+    hipLaunchKernelGGL(dev::k_rem_apply, dim3(grid_for(n, blk)), dim3(blk), 0, stream_, t, ev_, action_, n, ctr_);
     if ((st = check_launch(hipGetLastError(), "k_rem_apply")) != FM_OK) return st;
   }
   Counters c;
@@ -629,7 +656,7 @@ fm_status GpuBackend::run_remove(const uint8_t* d_keys, uint64_t n, uint64_t* re
     }
     {
       Prof p(this, kKidRemReplay);
-      if ((st = check_launch(ops_.rem_replay(t_, d_keys, n, cid_s_, idx_s_, ctr_, blk, stream_), "k_rem_replay")) !=
+      if ((st = check_launch(ops_.rem_replay(t, d_keys, n, cid_s_, idx_s_, ctr_, blk, stream_), "k_rem_replay")) !=
           FM_OK)
         return st;
     }

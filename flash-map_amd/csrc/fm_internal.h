@@ -8,6 +8,7 @@
 #include <stdint.h>
 
 #include <memory>
+#include <mutex>
 #include <string>
 
 #include "../../include/flashmap.h"
@@ -50,6 +51,15 @@ class Backend {
   virtual fm_status recount(uint64_t*) { return gpu_required(); }
   virtual fm_status synchronize() { return FM_OK; }
   virtual fm_status copy_from(Backend&) { return gpu_required(); }
+
+  // Serving-loop hooks (GPU only).  Device buffers; work is ordered on the
+  // map's stream.  serve_insert raises len by n (an upper bound, so len never
+  // undercounts occupied slots) and serve_settle_insert lowers it by the
+  // keys that did not claim a slot once their outcome flags are known.
+  virtual fm_status serve_get(const void*, uint64_t, void*, uint8_t*) { return gpu_required(); }
+  virtual fm_status serve_insert(const void*, const void*, uint64_t, uint8_t*) { return gpu_required(); }
+  virtual void serve_settle_insert(uint64_t, uint64_t) {}
+  virtual fm_status serve_remove(const void*, uint64_t, uint8_t*, uint64_t*) { return gpu_required(); }
   virtual fm_status profile_enable(bool) { return gpu_required(); }
   virtual fm_status profile_read(fm_kernel_stat*, uint32_t, uint32_t*) { return gpu_required(); }
   virtual fm_status get_tuning(fm_tuning*) const { return gpu_required(); }
@@ -65,3 +75,9 @@ std::unique_ptr<Backend> make_gpu_backend(const fm_builder& b, fm_status* st, st
 std::unique_ptr<Backend> make_cpu_backend(const fm_builder& b, fm_status* st, std::string* err);
 
 }  // namespace fm
+
+// FlashMap<K,V> (lib.rs:94-96): one backend arm behind a per-handle lock.
+struct fm_map {
+  std::unique_ptr<fm::Backend> be;
+  mutable std::mutex mu;
+};

@@ -71,6 +71,9 @@ struct Table {
   uint32_t K;
   uint32_t V;
   uint32_t mode;
+  // Per-op, not table state: when set, inserts write 1/0 per batch key for
+  // "claimed a free slot" and removes 1/0 for "tombstoned a slot".
+  uint8_t* outcome;
 };
 
 struct Counters {
@@ -432,7 +435,9 @@ k_ins_replay(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __rest
       const uint32_t c = cid_s[s];
       for (uint64_t u = s; u < n && cid_s[u] == c; ++u) {
         const uint64_t i = idx_s[u];
-        claimed += insert_one<KS, VS>(t, in_keys + i * K, in_vals + i * V, K, V, ctr);
+        const uint32_t c1 = insert_one<KS, VS>(t, in_keys + i * K, in_vals + i * V, K, V, ctr);
+        claimed += c1;
+        if (t.outcome) t.outcome[i] = static_cast<uint8_t>(c1);
       }
     }
     wave_add(&ctr->count, claimed);
@@ -517,7 +522,9 @@ k_ins_leader(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __rest
           }
           for (uint32_t j = 0; j < m; ++j) {
             const uint64_t k = my[j * 256];
-            claimed += insert_one<KS, VS, false>(t, in_keys + k * K, in_vals + k * V, K, V, ctr);
+            const uint32_t c1 = insert_one<KS, VS, false>(t, in_keys + k * K, in_vals + k * V, K, V, ctr);
+            claimed += c1;
+            if (t.outcome) t.outcome[k] = static_cast<uint8_t>(c1);
           }
         }
       }
@@ -598,7 +605,9 @@ k_ins_big(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __restric
         uint32_t claimed = 0;
         for (uint32_t j = 0; j < m; ++j) {
           const uint64_t k = lst[j];
-          claimed += insert_one<KS, VS, false>(t, in_keys + k * K, in_vals + k * V, K, V, ctr);
+          const uint32_t c1 = insert_one<KS, VS, false>(t, in_keys + k * K, in_vals + k * V, K, V, ctr);
+          claimed += c1;
+          if (t.outcome) t.outcome[k] = static_cast<uint8_t>(c1);
         }
         if (claimed) atomicAdd(&ctr->count, static_cast<unsigned long long>(claimed));
       }
@@ -697,6 +706,7 @@ k_rem_apply(Table t, const unsigned long long* __restrict__ ev, const uint8_t* _
       }
       t.owner[slot] = kNoOwner;
     }
+    if (t.outcome && i < n && !conflict) t.outcome[i] = static_cast<uint8_t>(removed);
     wave_add(&ctr->count, removed);
   }
 }
@@ -754,7 +764,11 @@ k_rem_replay(Table t, const uint8_t* __restrict__ qkeys, uint64_t n, const uint3
     uint32_t removed = 0;
     if (s < n && (s == 0 || cid_s[s] != cid_s[s - 1])) {
       const uint32_t c = cid_s[s];
-      for (uint64_t u = s; u < n && cid_s[u] == c; ++u) removed += remove_one<KS>(t, qkeys + idx_s[u] * K, K);
+      for (uint64_t u = s; u < n && cid_s[u] == c; ++u) {
+        const uint32_t r = remove_one<KS>(t, qkeys + idx_s[u] * K, K);
+        removed += r;
+        if (t.outcome) t.outcome[idx_s[u]] = static_cast<uint8_t>(r);
+      }
     }
     wave_add(&ctr->count, removed);
   }

@@ -208,6 +208,56 @@ typedef struct fm_tuning {
 fm_status fm_get_tuning(const fm_map* m, fm_tuning* out);
 fm_status fm_set_tuning(fm_map* m, const fm_tuning* t);
 
+/* ---- serving loop (no reference counterpart) ---------------------------
+ * Closest reference analogue: AsyncFlashMap (src/async_map.rs:32-106), which
+ * runs each call on a blocking thread under an RwLock with no ordering
+ * guarantee.  fm_serve_log replays a log of requests against one GPU map as
+ * a continuous-batching server: requests are admitted in arrival order as
+ * device staging memory frees up, adjacent requests of the same kind are
+ * coalesced into one device batch, and the host->device copy, the table op
+ * and the device->host copy of consecutive batches overlap on three streams.
+ * Results equal applying the requests one by one in arrival order (TableFull
+ * included: an insert that would overflow the table fails alone).  The map
+ * is used exclusively for the duration of the call. */
+typedef enum fm_op { FM_OP_GET = 0, FM_OP_INSERT = 1, FM_OP_REMOVE = 2 } fm_op;
+
+typedef struct fm_serve_request {
+  uint32_t op;          /* fm_op */
+  uint32_t reserved;
+  uint64_t n;           /* keys in the request */
+  const void* keys;     /* host, n*K */
+  const void* values;   /* host, n*V (insert) */
+  void* out_values;     /* host, n*V (get; misses zeroed) */
+  uint8_t* out_found;   /* host, n (get), nullable */
+  uint8_t* out_flags;   /* host, n, nullable: insert 1 = claimed a free
+                           slot, remove 1 = removed */
+  double arrival_s;     /* offered arrival time, seconds from the start */
+  /* results */
+  uint64_t count;       /* get: hits; insert: new slots; remove: removed */
+  int32_t status;       /* FM_OK or FM_ERR_TABLE_FULL */
+  int32_t batch;        /* device batch the request ran in */
+  double admit_s;       /* when its staging was granted */
+  double done_s;        /* when its results were in host memory */
+} fm_serve_request;
+
+typedef struct fm_serve_config {
+  uint64_t staging_bytes;  /* device + pinned staging ring (each) */
+  uint64_t max_batch_keys; /* coalescing limit per device batch */
+  uint32_t realtime;       /* 1: release requests at arrival_s; 0: all at t=0 */
+  uint32_t reserved;
+} fm_serve_config;
+
+typedef struct fm_serve_stats {
+  uint64_t requests, batches, keys;
+  double wall_s;           /* first arrival to last completion */
+  double p50_ms, p99_ms, mean_ms, max_ms; /* done_s - arrival_s */
+  uint64_t max_inflight_batches;
+  uint64_t staging_waits;  /* admissions that waited for staging */
+} fm_serve_stats;
+
+fm_status fm_serve_log(fm_map* m, fm_serve_request* reqs, uint64_t n_reqs,
+                       const fm_serve_config* cfg, fm_serve_stats* stats);
+
 /* ---- table groups: many independent maps, fused launches ---------------
  * No reference counterpart (the reference launches once per call per map,
  * gpu.rs:141-167).  A group holds n independent tables with one key/value

@@ -80,11 +80,13 @@ def test_gpu_fill_to_full_no_barrier(seed):
     cap = 256
     say(f"fill to full seed={seed}")
     m, o = gpu_make(cap, 8, 8, seed % 2), OracleAdapter(cap, 8, 8, seed % 2)
-    ks = rng.choice(100_000, size=cap, replace=False).astype(np.uint64)
-    ops = [("ins", ks[:100].view(np.uint8).reshape(-1, 8), ks[:100].view(np.uint8).reshape(-1, 8)),
-           ("rem", ks[:30].view(np.uint8).reshape(-1, 8)),
-           ("ins", ks[100:].view(np.uint8).reshape(-1, 8)[:cap - 70], ks[100:].view(np.uint8).reshape(-1, 8)[:cap - 70]),
-           ("get", ks.view(np.uint8).reshape(-1, 8))]
+    ks = rng.choice(100_000, size=cap + 100, replace=False).astype(np.uint64)
+    rows = ks.view(np.uint8).reshape(-1, 8)
+    # 100 in, 30 out (tombstones), then exactly the 186 free slots' worth
+    ops = [("ins", rows[:100], rows[:100]),
+           ("rem", rows[:30]),
+           ("ins", rows[100:100 + cap - 70], rows[100:100 + cap - 70]),
+           ("get", rows)]
     apply_and_compare(m, o, ops)
     assert len(m) == cap
 
