@@ -123,11 +123,21 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--scale", type=float, default=1.0, help="shrink the workload (testing only)")
+    ap.add_argument("--config", choices=["c2", "c4", "c5"], default="c2",
+                    help="c2: account-state mix (default, the metric line); c4: 512-table fused vs plain; "
+                         "c5: serving loop at several offered loads")
+    ap.add_argument("--tables", type=int, default=512, help="c4: number of tables")
+    ap.add_argument("--requests", type=int, default=400, help="c5: requests per offered load")
+    ap.add_argument("--rates", default="2000,10000,50000", help="c5: offered loads, requests/s")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the table would shard by "
               "slot range; see the sharding design). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
+    if args.config == "c4":
+        return run_c4(args)
+    if args.config == "c5":
+        return run_c5(args)
 
     s = args.scale
     log("generating workload")
@@ -264,6 +274,198 @@ def main():
         "check": check,
     }
     print(json.dumps(out), flush=True)
+
+
+# ---------------------------------------------------------------------------
+# Config 4: 512 independent FlashMap<u64,[u8;32]> tables, one step =
+# bulk_insert -> bulk_get -> bulk_remove on every table.  The fused path (one
+# hipGraph launch for the whole step over all tables) against the plain path
+# (one bulk call per table-op on 512 separate maps), both from the same
+# preloaded state, results compared table by table.
+# ---------------------------------------------------------------------------
+def run_c4(args):
+    from flashmap_amd.workloads import config4_table
+
+    n = args.tables
+    log(f"c4: generating {n} tables")
+    tabs = [config4_table(t, cap_hi=max(1 << 10, int((1 << 22) * args.scale))) for t in range(n)]
+    caps = [t.capacity for t in tabs]
+    V = 32
+
+    def rows64(a):
+        return np.ascontiguousarray(a, dtype=np.uint64).view(np.uint8).reshape(-1, 8)
+
+    def cat(parts, w):
+        return np.concatenate([np.ascontiguousarray(p).reshape(-1, w) for p in parts]) if parts else \
+            np.zeros((0, w), np.uint8)
+
+    snap = fm.FlashMapGroup(caps, 8, V)
+    work = fm.FlashMapGroup(caps, 8, V)
+    log(f"c4: preloading {sum(len(t.preload_keys) for t in tabs)} keys into {sum(caps)} slots (fused)")
+    pk, pv = snap.upload(cat([rows64(t.preload_keys) for t in tabs], 8)), snap.upload(cat([t.preload_vals for t in tabs], V))
+    new, status = snap.bulk_insert_device(pk, pv, [len(t.preload_keys) for t in tabs])
+    assert all(int(new[i]) == len(tabs[i].preload_keys) for i in range(n))
+    pk.free()
+    pv.free()
+    ic, gc, rc = [len(t.ins_keys) for t in tabs], [len(t.get_keys) for t in tabs], [len(t.rem_keys) for t in tabs]
+    ik, iv = work.upload(cat([rows64(t.ins_keys) for t in tabs], 8)), work.upload(cat([t.ins_vals for t in tabs], V))
+    gk, rk = work.upload(cat([rows64(t.get_keys) for t in tabs], 8)), work.upload(cat([rows64(t.rem_keys) for t in tabs], 8))
+    out, found = work.alloc_device(max(1, sum(gc)) * V), work.alloc_device(max(1, sum(gc)))
+    keys_per_step = sum(ic) + sum(gc) + sum(rc)
+
+    # fused: warm up (captures the graph), then time restored repetitions
+    reps = max(1, args.steps)
+    fused_ms = []
+    for r in range(reps + 1):
+        work.copy_from(snap)
+        work.download(out, 1)  # waits for the restore on the group's stream
+        t0 = time.perf_counter()
+        f_new, f_status, f_rem = work.step_device(ik, iv, ic, gk, gc, out, found, rk, rc)
+        dt = time.perf_counter() - t0
+        if r:
+            fused_ms.append(dt * 1e3)
+    f_vals = work.download(out, sum(gc) * V).reshape(-1, V)
+    f_found = work.download(found, sum(gc)).astype(bool)
+    f_slots = [hashlib.sha256(b"".join(a.tobytes() for a in work.export_slots(t))).hexdigest() for t in range(n)]
+
+    # plain: 512 separate maps, one call per table-op
+    log("c4: building plain per-table maps")
+    maps = []
+    for t, w in enumerate(tabs):
+        m = fm.FlashMap.builder(w.capacity, 8, V).require_gpu().build()
+        dk, dv = m.upload(rows64(w.preload_keys)), m.upload(w.preload_vals)
+        if len(w.preload_keys):
+            m.bulk_insert_device(dk, dv, len(w.preload_keys))
+        dk.free()
+        dv.free()
+        bufs = dict(ik=m.upload(rows64(w.ins_keys)), iv=m.upload(w.ins_vals), gk=m.upload(rows64(w.get_keys)),
+                    rk=m.upload(rows64(w.rem_keys)), out=m.alloc_device(max(1, len(w.get_keys)) * V),
+                    found=m.alloc_device(max(1, len(w.get_keys))))
+        maps.append((m, bufs))
+    m0, _ = maps[0]
+    p_new, p_status, p_rem = [0] * n, [0] * n, [0] * n
+    t0 = time.perf_counter()
+    for t, (m, b) in enumerate(maps):
+        try:
+            p_new[t] = m.bulk_insert_device(b["ik"], b["iv"], ic[t]) if ic[t] else 0
+        except fm.TableFull:
+            p_status[t] = fm.TABLE_FULL
+        if gc[t]:
+            m.bulk_get_device(b["gk"], gc[t], b["out"], b["found"])
+        p_rem[t] = m.bulk_remove_device(b["rk"], rc[t]) if rc[t] else 0
+    for m, _ in maps:
+        m.synchronize()
+    plain_ms = (time.perf_counter() - t0) * 1e3
+
+    # table-by-table comparison of the two paths
+    exact = True
+    off = 0
+    for t, (m, b) in enumerate(maps):
+        v = m.download(b["out"], gc[t] * V).reshape(-1, V) if gc[t] else np.zeros((0, V), np.uint8)
+        f = m.download(b["found"], gc[t]).astype(bool) if gc[t] else np.zeros(0, bool)
+        same = (int(f_new[t]) == p_new[t] and int(f_status[t]) == p_status[t] and int(f_rem[t]) == p_rem[t]
+                and np.array_equal(v, f_vals[off:off + gc[t]]) and np.array_equal(f, f_found[off:off + gc[t]])
+                and hashlib.sha256(b"".join(a.tobytes() for a in m.export_slots())).hexdigest() == f_slots[t])
+        off += gc[t]
+        if not same:
+            exact = False
+            log(f"c4: table {t} differs between fused and plain paths")
+    # oracle on a sample of tables (the largest one, TableFull ones, a spread)
+    from oracle.binding import OracleMap
+    sample = sorted(set([int(np.argmax(caps))] + [t for t in range(n) if p_status[t]][:4] + list(range(0, n, max(1, n // 12)))))
+    oracle_ok = True
+    for t in sample:
+        w = tabs[t]
+        o = OracleMap(w.capacity, 8, V, 0)
+        o.bulk_insert(rows64(w.preload_keys), w.preload_vals)
+        try:
+            en, es = o.bulk_insert(rows64(w.ins_keys), w.ins_vals), 0
+        except Exception:
+            en, es = 0, fm.TABLE_FULL
+        ov, of = o.bulk_get(rows64(w.get_keys))
+        er = o.bulk_remove(rows64(w.rem_keys))
+        ok = (en, es, er) == (int(f_new[t]), int(f_status[t]), int(f_rem[t])) and \
+            hashlib.sha256(b"".join(a.tobytes() for a in o.export())).hexdigest() == f_slots[t]
+        oracle_ok &= ok
+        if not ok:
+            log(f"c4: table {t} differs from the oracle")
+        o.close()
+    fused = float(np.median(fused_ms))
+    out_line = {
+        "metric": "batched multi-table step Mkeys/s (fused hipGraph vs plain per-table launches)",
+        "value": round(keys_per_step / fused / 1e3, 3), "unit": "Mkeys/s", "n_gpus": 1,
+        "plain_mkeys_s": round(keys_per_step / plain_ms / 1e3, 3),
+        "speedup_vs_plain": round(plain_ms / fused, 2),
+        "fused_ms": round(fused, 3), "plain_ms": round(plain_ms, 3), "reps": reps,
+        "config": {"workload": f"{n} x FlashMap<u64,[u8;32]>, capacities log-uniform [2^10,2^22], load 0.5-0.85; "
+                               "step = insert (log-uniform [1,2^16], 30% overwrite) + get (log-uniform [1,2^16], "
+                               "70% hits) + remove (log-uniform [1,2^12]) per table",
+                   "keys_per_step": keys_per_step, "slots": int(sum(caps)),
+                   "table_full_batches": int(sum(1 for s in f_status if s))},
+        "tuning": {k: getattr(work.tuning(), k) for k in ("group", "block", "grid")},
+        "check": {"fused_equals_plain": exact, "oracle_sample_tables": len(sample), "oracle_exact": bool(oracle_ok)},
+    }
+    print(json.dumps(out_line), flush=True)
+    if not (exact and oracle_ok):
+        sys.exit(1)
+
+
+# ---------------------------------------------------------------------------
+# Config 5: serving loop over one FlashMap<[u8;32],u64> (2^25 slots,
+# 20M preloaded), Poisson arrivals at several offered loads.
+# ---------------------------------------------------------------------------
+def run_c5(args):
+    from flashmap_amd.workloads import config5_preload, config5_requests
+
+    s = args.scale
+    cap, pk, pv = config5_preload(n_preload=int(20_000_000 * s), capacity=int(24_000_000 * s))
+    snap = fm.FlashMap.builder(cap, 32, 8).require_gpu().build()
+    m = fm.FlashMap.builder(cap, 32, 8).require_gpu().build()
+    log(f"c5: preloading {len(pk)} keys")
+    dk, dv = snap.upload(pk), snap.upload(pv)
+    snap.bulk_insert_device(dk, dv, len(pk))
+    dk.free()
+    dv.free()
+    lines = []
+    for ri, rate in enumerate(float(x) for x in args.rates.split(",")):
+        reqs = config5_requests(pk, rate, args.requests)
+        m.copy_from(snap)
+        m.synchronize()
+        log(f"c5: serving {len(reqs)} requests at {rate:.0f}/s")
+        results, st = fm.serve_log(m, reqs, staging_bytes=1 << 30, max_batch_keys=1 << 21, realtime=True)
+        check = None
+        if ri == 0 and not args.no_check:
+            from oracle.binding import OracleMap, TableFull as OTF
+            o = OracleMap(cap, 32, 8, 0)
+            o.bulk_insert(pk, pv)
+            bad = 0
+            for r, res in zip(reqs, results):
+                if r.op == "get":
+                    v, f = o.bulk_get(r.keys)
+                    bad += int(not (np.array_equal(v.reshape(-1, 8), res.values) and
+                                    np.array_equal(f.astype(bool), res.found.astype(bool))))
+                elif r.op == "insert":
+                    try:
+                        bad += int(o.bulk_insert(r.keys, r.vals) != res.count or res.status != 0)
+                    except OTF:
+                        bad += int(res.status != fm.TABLE_FULL)
+                else:
+                    bad += int(o.bulk_remove(r.keys) != res.count)
+            check = {"requests_checked": len(reqs), "mismatches": bad}
+            o.close()
+        lines.append({"offered_req_s": rate, "requests": len(reqs), "keys": int(st.keys),
+                      "throughput_mkeys_s": round(st.keys / st.wall_s / 1e6, 2),
+                      "achieved_req_s": round(len(reqs) / st.wall_s, 1), "p50_ms": round(st.p50_ms, 3),
+                      "p99_ms": round(st.p99_ms, 3), "mean_ms": round(st.mean_ms, 3), "batches": int(st.batches),
+                      "max_inflight_batches": int(st.max_inflight_batches), "staging_waits": int(st.staging_waits),
+                      "check": check})
+    print(json.dumps({"metric": "serving p50/p99 latency and throughput at offered loads", "n_gpus": 1,
+                      "config": {"workload": "FlashMap<[u8;32],u64> with_capacity(24M)=2^25, preload 20M; requests "
+                                             "70% get / 20% insert / 10% remove, log-uniform [64,262144] keys, gets "
+                                             "80% hits Zipf(0.9)", "staging_bytes": 1 << 30},
+                      "loads": lines}), flush=True)
+    if any(l["check"] and l["check"]["mismatches"] for l in lines):
+        sys.exit(1)
 
 
 if __name__ == "__main__":

@@ -65,6 +65,29 @@ class GpuRequired(FlashMapError):
     """FlashMapError::GpuRequired (error.rs:32-33)."""
 
 
+OP_GET, OP_INSERT, OP_REMOVE = 0, 1, 2
+
+
+class ServeRequest(ctypes.Structure):
+    _fields_ = [("op", ctypes.c_uint32), ("reserved", ctypes.c_uint32), ("n", ctypes.c_uint64),
+                ("keys", ctypes.c_void_p), ("values", ctypes.c_void_p), ("out_values", ctypes.c_void_p),
+                ("out_found", ctypes.c_void_p), ("out_flags", ctypes.c_void_p), ("arrival_s", ctypes.c_double),
+                ("count", ctypes.c_uint64), ("status", ctypes.c_int32), ("batch", ctypes.c_int32),
+                ("admit_s", ctypes.c_double), ("done_s", ctypes.c_double)]
+
+
+class ServeConfig(ctypes.Structure):
+    _fields_ = [("staging_bytes", ctypes.c_uint64), ("max_batch_keys", ctypes.c_uint64),
+                ("realtime", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+
+class ServeStats(ctypes.Structure):
+    _fields_ = [("requests", ctypes.c_uint64), ("batches", ctypes.c_uint64), ("keys", ctypes.c_uint64),
+                ("wall_s", ctypes.c_double), ("p50_ms", ctypes.c_double), ("p99_ms", ctypes.c_double),
+                ("mean_ms", ctypes.c_double), ("max_ms", ctypes.c_double),
+                ("max_inflight_batches", ctypes.c_uint64), ("staging_waits", ctypes.c_uint64)]
+
+
 class GroupTuning(ctypes.Structure):
     _fields_ = [("group", ctypes.c_uint32), ("block", ctypes.c_uint32), ("grid", ctypes.c_uint32)]
 
@@ -145,6 +168,8 @@ def lib() -> ctypes.CDLL:
             "fm_export_slots": (ctypes.c_int, [vp, vp, vp, vp]),
             "fm_import_slots": (ctypes.c_int, [vp, vp, vp, vp, u64]),
             "fm_copy_table": (ctypes.c_int, [vp, vp]),
+            "fm_serve_log": (ctypes.c_int, [vp, ctypes.POINTER(ServeRequest), u64, ctypes.POINTER(ServeConfig),
+                                            ctypes.POINTER(ServeStats)]),
             "fm_group_create": (ctypes.c_int, [u32, pu64, u32, u32, u32, i32, ctypes.POINTER(vp)]),
             "fm_group_destroy": (None, [vp]),
             "fm_group_tables": (u32, [vp]),
@@ -427,6 +452,64 @@ class FlashMap:
             if val is not None:
                 setattr(t, name, val)
         _check(lib().fm_set_tuning(self._h, ctypes.byref(t)))
+
+
+class ServeResult:
+    """Outputs of one served request."""
+
+    def __init__(self, op, n, value_size):
+        self.op = op
+        self.values = np.zeros((n, value_size), dtype=np.uint8) if op == OP_GET else None
+        self.found = np.zeros(n, dtype=np.uint8) if op == OP_GET else None
+        self.flags = np.zeros(n, dtype=np.uint8) if op != OP_GET else None
+        self.count = 0
+        self.status = OK
+        self.batch = -1
+        self.arrival_s = self.admit_s = self.done_s = 0.0
+
+    @property
+    def latency_ms(self):
+        return (self.done_s - self.arrival_s) * 1e3
+
+
+def serve_log(fmap: "FlashMap", requests, staging_bytes: int = 256 << 20, max_batch_keys: int = 1 << 20,
+              realtime: bool = True):
+    """Runs a request log through the continuous-batching server
+    (fm_serve_log).  `requests`: objects with .op ("get"|"insert"|"remove"),
+    .keys ((n, K) bytes), .vals ((n,) or (n, V); inserts) and .arrival_s.
+    Returns ([ServeResult], ServeStats)."""
+    ops = {"get": OP_GET, "insert": OP_INSERT, "remove": OP_REMOVE}
+    n = len(requests)
+    arr = (ServeRequest * max(n, 1))()
+    keep, results = [], []
+    for i, r in enumerate(requests):
+        op = ops[r.op]
+        k, nk = _as_bytes(r.keys, fmap.key_size, "keys")
+        res = ServeResult(op, nk, fmap.value_size)
+        res.arrival_s = float(r.arrival_s) if realtime else 0.0
+        a = arr[i]
+        a.op, a.n, a.keys, a.arrival_s = op, nk, k.ctypes.data if nk else None, float(r.arrival_s)
+        keep.append(k)
+        if op == OP_INSERT:
+            v, nv = _as_bytes(r.vals, fmap.value_size, "values")
+            assert nv == nk
+            a.values = v.ctypes.data if nv else None
+            keep.append(v)
+        if op == OP_GET:
+            a.out_values = res.values.ctypes.data if nk else None
+            a.out_found = res.found.ctypes.data if nk else None
+        else:
+            a.out_flags = res.flags.ctypes.data if nk else None
+        results.append(res)
+    cfg = ServeConfig(staging_bytes, max_batch_keys, 1 if realtime else 0, 0)
+    stats = ServeStats()
+    _check(lib().fm_serve_log(fmap._h, arr, n, ctypes.byref(cfg), ctypes.byref(stats)))
+    for i, res in enumerate(results):
+        a = arr[i]
+        res.count, res.status, res.batch = int(a.count), int(a.status), int(a.batch)
+        res.admit_s, res.done_s = float(a.admit_s), float(a.done_s)
+    del keep
+    return results, stats
 
 
 class GroupBuffer:
