@@ -295,11 +295,12 @@ fm_status fm_import_slots(fm_map* m, const uint32_t* flags, const void* keys, co
   return m->be->import_slots(flags, keys, values, len);
 }
 
-fm_status fm_copy_table(fm_map* dst, fm_map* src) {
+fm_status fm_copy_table(fm_map* dst, fm_map* src, uint32_t parts) {
   if (!dst || !src) return bad_arg("fm_copy_table: null map");
+  if (parts > FM_COPY_STRUCTURE) return bad_arg("fm_copy_table: unknown parts");
   if (dst == src) return FM_OK;
   std::scoped_lock guard(dst->mu, src->mu);
-  return dst->be->copy_from(*src->be);
+  return dst->be->copy_from(*src->be, parts);
 }
 
 fm_status fm_profile_enable(fm_map* m, int enable) {

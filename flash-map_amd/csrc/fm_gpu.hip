@@ -35,7 +35,7 @@ struct Ops {
   hipError_t (*ins_dry)(const Table&, const uint8_t*, uint64_t, unsigned long long*, uint32_t*, uint32_t*, Counters*,
                         uint32_t, hipStream_t);
   hipError_t (*ins_leader)(const Table&, const uint8_t*, const uint8_t*, uint64_t, const unsigned long long*,
-                           const uint32_t*, uint32_t*, Counters*, hipStream_t);
+                           const uint32_t*, uint32_t*, Counters*, uint32_t, hipStream_t);
   hipError_t (*ins_big)(const Table&, const uint8_t*, const uint8_t*, uint64_t, const unsigned long long*,
                         const uint32_t*, Counters*, hipStream_t);
   hipError_t (*ins_replay)(const Table&, const uint8_t*, const uint8_t*, uint64_t, const uint32_t*, const uint32_t*,
@@ -80,16 +80,17 @@ hipError_t launch_ins_dry(const Table& t, const uint8_t* k, uint64_t n, unsigned
 template <int KS, int VS>
 hipError_t launch_ins_leader(const Table& t, const uint8_t* k, const uint8_t* v, uint64_t n,
                              const unsigned long long* Q, const uint32_t* dslot, uint32_t* big, Counters* c,
-                             hipStream_t s) {
+                             uint32_t coop_arc, hipStream_t s) {
   hipLaunchKernelGGL((dev::k_ins_leader<KS, VS>), dim3(grid_for(n, 256)), dim3(256), 0, s, t, k, v, n, Q, dslot, big,
-                     c);
+                     c, coop_arc);
   return hipGetLastError();
 }
 
+// One wave per queued closure; enough waves to cover ~4 per SIMD.
 template <int KS, int VS>
 hipError_t launch_ins_big(const Table& t, const uint8_t* k, const uint8_t* v, uint64_t n, const unsigned long long* Q,
                           const uint32_t* big, Counters* c, hipStream_t s) {
-  hipLaunchKernelGGL((dev::k_ins_big<KS, VS>), dim3(256), dim3(1024), 0, s, t, k, v, n, Q, big, c);
+  hipLaunchKernelGGL((dev::k_ins_big<KS, VS>), dim3(4096), dim3(64), 0, s, t, k, v, n, Q, big, c);
   return hipGetLastError();
 }
 
@@ -207,7 +208,7 @@ class GpuBackend final : public Backend {
   fm_status bulk_remove_device(const void* d_keys, uint64_t n, uint64_t* out_removed) override;
   fm_status recount(uint64_t* out) override;
   fm_status synchronize() override;
-  fm_status copy_from(Backend& src) override;
+  fm_status copy_from(Backend& src, uint32_t parts) override;
   fm_status serve_get(const void* d_keys, uint64_t n, void* d_out, uint8_t* d_found) override {
     fm_status st;
     if ((st = use_device()) != FM_OK) return st;
@@ -282,7 +283,7 @@ class GpuBackend final : public Backend {
   Table t_{};
   uint64_t len_ = 0;
   Ops ops_{};
-  fm_tuning tune_{4, 256, 4, 256};
+  fm_tuning tune_{4, 256, 4, 256, 16};
   bool tune_sort_only_ = false;  // FM_INSERT_SORT_PATH=1 forces the sort-grouped insert
   bool last_insert_led_ = false;
 
@@ -578,7 +579,8 @@ fm_status GpuBackend::run_insert(const uint8_t* d_keys, const uint8_t* d_vals, u
     }
     {
       Prof p(this, kKidInsLeader);
-      if ((st = check_launch(ops_.ins_leader(t, d_keys, d_vals, n, Q_, dslot_, big_, ctr_, stream_),
+      if ((st = check_launch(ops_.ins_leader(t, d_keys, d_vals, n, Q_, dslot_, big_, ctr_, tune_.insert_coop_arc,
+                                             stream_),
                              "k_ins_leader")) != FM_OK)
         return st;
     }
@@ -965,7 +967,7 @@ fm_status GpuBackend::synchronize() {
   return e == hipSuccess ? FM_OK : fail(FM_ERR_KERNEL_LAUNCH, hip_msg("synchronize", e));
 }
 
-fm_status GpuBackend::copy_from(Backend& other) {
+fm_status GpuBackend::copy_from(Backend& other, uint32_t parts) {
   GpuBackend* src = dynamic_cast<GpuBackend*>(&other);
   if (!src || src->dev_ != dev_ || src->t_.cap != t_.cap || src->t_.K != t_.K || src->t_.V != t_.V ||
       src->t_.mode != t_.mode)
@@ -978,7 +980,8 @@ fm_status GpuBackend::copy_from(Backend& other) {
   Prof p(this, kKidCopyTable);
   if (e == hipSuccess) e = hipMemcpyAsync(t_.flags, src->t_.flags, t_.cap * 4, hipMemcpyDeviceToDevice, stream_);
   if (e == hipSuccess) e = hipMemcpyAsync(t_.keys, src->t_.keys, t_.cap * t_.K, hipMemcpyDeviceToDevice, stream_);
-  if (e == hipSuccess) e = hipMemcpyAsync(t_.vals, src->t_.vals, t_.cap * t_.V, hipMemcpyDeviceToDevice, stream_);
+  if (e == hipSuccess && parts != FM_COPY_STRUCTURE)
+    e = hipMemcpyAsync(t_.vals, src->t_.vals, t_.cap * t_.V, hipMemcpyDeviceToDevice, stream_);
   if (e == hipSuccess) e = hipMemcpyAsync(t_.freebits, src->t_.freebits, words * 8, hipMemcpyDeviceToDevice, stream_);
   if (e == hipSuccess) e = hipMemcpyAsync(t_.emptybits, src->t_.emptybits, words * 8, hipMemcpyDeviceToDevice, stream_);
   if (e != hipSuccess) return fail(FM_ERR_TRANSFER, hip_msg("fm_copy_table", e));

@@ -50,7 +50,7 @@ class Backend {
   virtual fm_status bulk_remove_device(const void*, uint64_t, uint64_t*) { return gpu_required(); }
   virtual fm_status recount(uint64_t*) { return gpu_required(); }
   virtual fm_status synchronize() { return FM_OK; }
-  virtual fm_status copy_from(Backend&) { return gpu_required(); }
+  virtual fm_status copy_from(Backend&, uint32_t /*parts*/) { return gpu_required(); }
 
   // Serving-loop hooks (GPU only).  Device buffers; work is ordered on the
   // map's stream.  serve_insert raises len by n (an upper bound, so len never

@@ -474,13 +474,152 @@ __device__ inline uint64_t bm_prev_set(const unsigned long long* bm, uint64_t st
   return start;
 }
 
-// Leaders order and replay closures of up to kLeaderMax keys; larger
-// closures are queued for k_ins_big.
+// ---------------------------------------------------------------------------
+// The same sequential insert as insert_one, walked by a whole wave: the 64
+// lanes load 64 consecutive slots (flag + key) in one round trip, the walk's
+// decisions for those slots are taken in order on wave-uniform state
+// (readlane of lane j = slot j), and all moves of the window are applied
+// afterwards (every slot is written at most once per walk; the walk only
+// reads slots ahead of the ones it has written).  Evicted residents' values
+// are loaded once per window; the pair carried across windows lives in
+// `carry` (LDS).  Only for the leader path (a free slot exists, so the walk
+// ends within cap steps and never revisits a slot) and sizes that are
+// multiples of 4.  Called by all 64 lanes of a single-wave workgroup.
+// ---------------------------------------------------------------------------
+template <int KS, int VS>
+__device__ uint32_t insert_one_wave(const Table& t, const uint8_t* __restrict__ kin, const uint8_t* __restrict__ vin,
+                                    uint32_t* carry, Counters* ctr) {
+  static_assert(KS % 4 == 0 && VS % 4 == 0 && KS > 0 && VS > 0, "static word sizes");
+  constexpr int KW = KS / 4, VW = VS / 4;
+  const int lane = threadIdx.x & 63;
+  for (int w = lane; w < KW; w += 64) carry[w] = reinterpret_cast<const uint32_t*>(kin)[w];
+  for (int w = lane; w < VW; w += 64) carry[KW + w] = reinterpret_cast<const uint32_t*>(vin)[w];
+  __syncthreads();
+  Blob<KS> ck;  // carried key, identical in every lane
+#pragma unroll
+  for (int w = 0; w < KW; ++w) ck.w[w] = carry[w];
+  uint64_t home = ck.hash(KS, t.mode) & t.mask;
+  uint64_t p = 0, pos = home, steps = 0;
+  for (;;) {
+    const uint64_t s = (pos + lane) & t.mask;
+    const uint32_t f = t.flags[s];
+    const bool occ = (f & 0xFu) == kOccupied;
+    Blob<KS> rk;
+#pragma unroll
+    for (int w = 0; w < KW; ++w) rk.w[w] = 0;
+    if (occ) rk.load(t.keys + s * KS, KS);
+    auto eq_ballot = [&]() {
+      uint32_t diff = occ ? 0u : 1u;
+#pragma unroll
+      for (int w = 0; w < KW; ++w) diff |= rk.w[w] ^ ck.w[w];
+      return static_cast<unsigned long long>(__ballot(diff == 0));
+    };
+    unsigned long long eqm = eq_ballot();
+    int my_src = -2;       // new content of my slot: -2 none, -1 carry buffer, k: resident of lane k
+    uint32_t my_flag = 0;  // new flag when my_src != -2 and not an update
+    bool my_update = false;
+    int c_src = -1;
+    int end = 0;  // 0 window exhausted, 1 done (update/claim/drop)
+    uint32_t claimed = 0;
+    unsigned long long evm = 0;
+    for (int j = 0; j < 64; ++j) {
+      if (p > t.mask) {  // carried pair dropped (rayon_cpu.rs:167-169)
+        end = 1;
+        break;
+      }
+      if (++steps > t.cap + 1) {
+        if (lane == 0) atomicOr(&ctr->status, kStatusBudget);
+        end = 1;
+        break;
+      }
+      const uint32_t fj = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(f), j));
+      const uint32_t stj = fj & 0xFu;
+      const uint64_t dj = fj >> 4;
+      if (stj == kOccupied) {
+        if (dj == p && ((eqm >> j) & 1ull)) {  // equal key: update its value
+          if (lane == j) {
+            my_src = c_src;
+            my_update = true;
+          }
+          end = 1;
+          break;
+        }
+        if (p > dj) {  // Robin Hood eviction: carried pair in, resident carried on
+          if (lane == j) {
+            my_src = c_src;
+            my_flag = make_flag(kOccupied, p);
+          }
+          evm |= 1ull << j;
+          c_src = j;
+#pragma unroll
+          for (int w = 0; w < KW; ++w)
+            ck.w[w] = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(rk.w[w]), j));
+          eqm = eq_ballot();
+          home = ck.hash(KS, t.mode) & t.mask;
+          p = dj + 1;
+          continue;
+        }
+        ++p;
+        continue;
+      }
+      // EMPTY or TOMBSTONE: claim.
+      if (lane == j) {
+        my_src = c_src;
+        my_flag = make_flag(kOccupied, p);
+      }
+      claimed = 1;
+      end = 1;
+      break;
+    }
+    // Values of residents that were carried on (read before any write).
+    Blob<VS> rv;
+#pragma unroll
+    for (int w = 0; w < VW; ++w) rv.w[w] = 0;
+    if ((evm >> lane) & 1ull) rv.load(t.vals + s * VS, VS);
+    // Fetch every written slot's new key/value from its source.  All lanes
+    // take part in the shuffles.
+    const int src = my_src < 0 ? lane : my_src;
+    Blob<KS> nk;
+    Blob<VS> nv;
+#pragma unroll
+    for (int w = 0; w < KW; ++w) nk.w[w] = static_cast<uint32_t>(__shfl(static_cast<int>(rk.w[w]), src, 64));
+#pragma unroll
+    for (int w = 0; w < VW; ++w) nv.w[w] = static_cast<uint32_t>(__shfl(static_cast<int>(rv.w[w]), src, 64));
+    if (my_src == -1) {
+#pragma unroll
+      for (int w = 0; w < KW; ++w) nk.w[w] = carry[w];
+#pragma unroll
+      for (int w = 0; w < VW; ++w) nv.w[w] = carry[KW + w];
+    }
+    if (my_src != -2) {
+      nv.store(t.vals + s * VS, VS);
+      if (!my_update) {
+        nk.store(t.keys + s * KS, KS);
+        t.flags[s] = my_flag;
+      }
+    }
+    if (end) return claimed;
+    // Window exhausted: the pair carried into the next window goes to LDS.
+    __syncthreads();
+    if (c_src >= 0 && lane == c_src) {
+#pragma unroll
+      for (int w = 0; w < KW; ++w) carry[w] = rk.w[w];
+#pragma unroll
+      for (int w = 0; w < VW; ++w) carry[KW + w] = rv.w[w];
+    }
+    __syncthreads();
+    pos = (pos + 64) & t.mask;
+  }
+}
+
+// Leaders order and replay closures of up to kLeaderMax keys whose arc
+// (barrier to barrier) is at most `coop_arc` slots; other closures are
+// queued for k_ins_big.
 template <int KS, int VS>
 __global__ void __launch_bounds__(256)
 k_ins_leader(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __restrict__ in_vals, uint64_t n,
              const unsigned long long* __restrict__ Q, const uint32_t* __restrict__ dslot, uint32_t* __restrict__ big,
-             Counters* ctr) {
+             Counters* ctr, uint32_t coop_arc) {
   __shared__ uint32_t lst[kLeaderMax * 256];  // column per lane: lst[j * 256 + lane]
   const uint32_t K = KS ? KS : t.K;
   const uint32_t V = VS ? VS : t.V;
@@ -497,9 +636,13 @@ k_ins_leader(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __rest
       if (b != s && bit_at(Q, b)) {
         uint32_t m = 0;
         uint64_t pos = s;
+        bool long_arc = false;
         for (;;) {
           const uint64_t x = bm_next_set(P, pos, t.cap);
-          if (bit_at(Q, x)) break;  // the closure's barrier
+          if (bit_at(Q, x)) {  // the closure's barrier
+            long_arc = ((x - b - 1) & t.mask) > coop_arc;
+            break;
+          }
           if (m == kLeaderMax) {
             m = kLeaderMax + 1;
             break;
@@ -508,7 +651,7 @@ k_ins_leader(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __rest
           ++m;
           pos = (x + 1 == t.cap) ? 0 : x + 1;
         }
-        if (m > kLeaderMax) {
+        if (m > kLeaderMax || long_arc) {
           big[atomicAdd(&ctr->nbig, 1u)] = static_cast<uint32_t>(s);
         } else {
           for (uint32_t a = 1; a < m; ++a) {  // insertion sort by batch index
@@ -535,12 +678,14 @@ k_ins_leader(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __rest
 
 // One workgroup per large closure: gather the members' indices from the arc
 // into LDS (in windows of kBigMax batch indices when there are more), sort
-// them (bitonic), and replay them in order from lane 0.
+// them (bitonic), and replay them in order, each walked by the whole wave
+// (static sizes) or by lane 0.  Launched with 64-thread workgroups.
 template <int KS, int VS>
-__global__ void __launch_bounds__(1024)
+__global__ void __launch_bounds__(64)
 k_ins_big(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __restrict__ in_vals, uint64_t n,
           const unsigned long long* __restrict__ Q, const uint32_t* __restrict__ big, Counters* ctr) {
   __shared__ uint32_t lst[kBigMax];
+  __shared__ uint32_t carry[2 * kMaxGenericSize / 4];
   __shared__ uint32_t cnt;
   __shared__ unsigned long long barrier_at;
   const uint32_t K = KS ? KS : t.K;
@@ -601,15 +746,27 @@ k_ins_big(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __restric
           __syncthreads();
         }
       }
-      if (threadIdx.x == 0) {
+      if constexpr (KS > 0 && VS > 0 && KS % 4 == 0 && VS % 4 == 0) {
+        // Launched as one wave per workgroup: the wave walks each member.
         uint32_t claimed = 0;
         for (uint32_t j = 0; j < m; ++j) {
           const uint64_t k = lst[j];
-          const uint32_t c1 = insert_one<KS, VS, false>(t, in_keys + k * K, in_vals + k * V, K, V, ctr);
+          const uint32_t c1 = insert_one_wave<KS, VS>(t, in_keys + k * K, in_vals + k * V, carry, ctr);
           claimed += c1;
-          if (t.outcome) t.outcome[k] = static_cast<uint8_t>(c1);
+          if (threadIdx.x == 0 && t.outcome) t.outcome[k] = static_cast<uint8_t>(c1);
         }
-        if (claimed) atomicAdd(&ctr->count, static_cast<unsigned long long>(claimed));
+        if (threadIdx.x == 0 && claimed) atomicAdd(&ctr->count, static_cast<unsigned long long>(claimed));
+      } else {
+        if (threadIdx.x == 0) {
+          uint32_t claimed = 0;
+          for (uint32_t j = 0; j < m; ++j) {
+            const uint64_t k = lst[j];
+            const uint32_t c1 = insert_one<KS, VS, false>(t, in_keys + k * K, in_vals + k * V, K, V, ctr);
+            claimed += c1;
+            if (t.outcome) t.outcome[k] = static_cast<uint8_t>(c1);
+          }
+          if (claimed) atomicAdd(&ctr->count, static_cast<unsigned long long>(claimed));
+        }
       }
       __syncthreads();
       if (!windowed) break;

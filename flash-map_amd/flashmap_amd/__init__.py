@@ -115,6 +115,7 @@ class Tuning(ctypes.Structure):
         ("get_block", ctypes.c_uint32),
         ("remove_group", ctypes.c_uint32),
         ("insert_block", ctypes.c_uint32),
+        ("insert_coop_arc", ctypes.c_uint32),
     ]
 
 
@@ -167,7 +168,7 @@ def lib() -> ctypes.CDLL:
             "fm_synchronize": (ctypes.c_int, [vp]),
             "fm_export_slots": (ctypes.c_int, [vp, vp, vp, vp]),
             "fm_import_slots": (ctypes.c_int, [vp, vp, vp, vp, u64]),
-            "fm_copy_table": (ctypes.c_int, [vp, vp]),
+            "fm_copy_table": (ctypes.c_int, [vp, vp, u32]),
             "fm_serve_log": (ctypes.c_int, [vp, ctypes.POINTER(ServeRequest), u64, ctypes.POINTER(ServeConfig),
                                             ctypes.POINTER(ServeStats)]),
             "fm_group_create": (ctypes.c_int, [u32, pu64, u32, u32, u32, i32, ctypes.POINTER(vp)]),
@@ -425,9 +426,10 @@ class FlashMap:
         v = np.ascontiguousarray(vals).view(np.uint8).reshape(-1)
         _check(lib().fm_import_slots(self._h, _ptr(f), _ptr(k), _ptr(v), length))
 
-    def copy_from(self, src: "FlashMap"):
-        """Asynchronous device copy of src's table into this map."""
-        _check(lib().fm_copy_table(self._h, src._h))
+    def copy_from(self, src: "FlashMap", structure_only: bool = False):
+        """Asynchronous device copy of src's table into this map
+        (structure_only: flags, keys and bitmaps; value bytes untouched)."""
+        _check(lib().fm_copy_table(self._h, src._h, 1 if structure_only else 0))
 
     def profile(self, enable: bool = True):
         """Turns per-kernel event timing on/off (resets the totals)."""
@@ -445,10 +447,12 @@ class FlashMap:
         _check(lib().fm_get_tuning(self._h, ctypes.byref(t)))
         return t
 
-    def set_tuning(self, get_group=None, get_block=None, remove_group=None, insert_block=None):
+    def set_tuning(self, get_group=None, get_block=None, remove_group=None, insert_block=None,
+                   insert_coop_arc=None):
         t = self.tuning()
         for name, val in (("get_group", get_group), ("get_block", get_block),
-                          ("remove_group", remove_group), ("insert_block", insert_block)):
+                          ("remove_group", remove_group), ("insert_block", insert_block),
+                          ("insert_coop_arc", insert_coop_arc)):
             if val is not None:
                 setattr(t, name, val)
         _check(lib().fm_set_tuning(self._h, ctypes.byref(t)))

@@ -176,11 +176,15 @@ fm_status fm_export_slots(fm_map* m, uint32_t* flags, void* keys,
 fm_status fm_import_slots(fm_map* m, const uint32_t* flags, const void* keys,
                           const void* values, uint64_t len);
 
-/* No reference counterpart: copies src's whole table (flags, keys, values,
- * len) into dst, which must have the same device, capacity, K, V and hash
- * strategy.  Asynchronous on dst's stream (src is quiesced first); used to
- * restore a preloaded state between benchmark steps. */
-fm_status fm_copy_table(fm_map* dst, fm_map* src);
+/* No reference counterpart: copies src's table into dst, which must have the
+ * same device, capacity, K, V and hash strategy.  Asynchronous on dst's
+ * stream (src is quiesced first).  FM_COPY_ALL copies flags, keys, values,
+ * bitmaps and len; FM_COPY_STRUCTURE leaves dst's value bytes as they are
+ * (values never steer control flow or addresses, so a structure copy resets
+ * the work a later op will do; used between benchmark steps). */
+#define FM_COPY_ALL 0u
+#define FM_COPY_STRUCTURE 1u
+fm_status fm_copy_table(fm_map* dst, fm_map* src, uint32_t parts);
 
 /* ---- per-kernel timing (no reference counterpart) ----------------------
  * When enabled, every kernel the map launches is bracketed by hipEvents on
@@ -204,6 +208,8 @@ typedef struct fm_tuning {
   uint32_t get_block;
   uint32_t remove_group;
   uint32_t insert_block;
+  uint32_t insert_coop_arc; /* closures whose barrier-to-barrier arc is longer
+                               are walked by a whole wave (0: all of them) */
 } fm_tuning;
 fm_status fm_get_tuning(const fm_map* m, fm_tuning* out);
 fm_status fm_set_tuning(fm_map* m, const fm_tuning* t);

@@ -73,6 +73,39 @@ def test_gpu_sort_grouped_insert_path(seed, monkeypatch):
     apply_and_compare(m, OracleAdapter(cap, K, V, seed % 2), ops)
 
 
+@pytest.mark.parametrize("seed", range(10))
+def test_gpu_wave_walk_all_closures(seed):
+    """insert_coop_arc = 0 sends every closure through the wave-cooperative
+    walk (windows of 64 slots, LDS carry across windows)."""
+    rng = np.random.default_rng(3000 + seed)
+    cap = [64, 256, 1024, 4096, 1 << 14][seed % 5]
+    K, V = [(8, 8), (32, 128), (32, 8), (8, 32), (16, 16)][seed % 5]
+    say(f"wave walk seed={seed} cap={cap} K={K} V={V}")
+    ops = random_history(rng, cap, K, V, 20, max(1, cap // 2), cap * 2)
+    m = gpu_make(cap, K, V, seed % 2)
+    m.set_tuning(insert_coop_arc=0)
+    apply_and_compare(m, OracleAdapter(cap, K, V, seed % 2), ops)
+
+
+def test_gpu_wave_walk_crowded_and_config1():
+    from flashmap_amd.workloads import config1
+
+    say("wave walk crowded + config1")
+    rng = np.random.default_rng(78)
+    cap = 1 << 13
+    base = rng.integers(0, 1 << 40, size=3200, dtype=np.int64).astype(np.uint64)
+    rows = np.unique((base << np.uint64(13)) | (base & np.uint64(0x3FF)))[:3000].view(np.uint8).reshape(-1, 8)
+    m = gpu_make(cap, 8, 8)
+    m.set_tuning(insert_coop_arc=0)
+    apply_and_compare(m, OracleAdapter(cap, 8, 8), [("ins", rows[:1500], rows[:1500]), ("rem", rows[:500]),
+                                                    ("ins", rows[1500:], rows[:1500]), ("get", rows)])
+    c = config1(n_insert=200_000, n_query=100_000, n_remove=20_000, capacity=1 << 18)
+    m = gpu_make(c.capacity, 8, 8)
+    m.set_tuning(insert_coop_arc=0)
+    apply_and_compare(m, OracleAdapter(c.capacity, 8, 8), [("ins", c.insert_keys, c.insert_vals), ("get", c.queries),
+                                                           ("rem", c.remove_keys), ("ins", c.remove_keys, c.remove_keys)])
+
+
 @pytest.mark.parametrize("seed", range(4))
 def test_gpu_fill_to_full_no_barrier(seed):
     """Batches that consume every free slot (single closure, no barrier)."""
