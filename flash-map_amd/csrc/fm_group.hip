@@ -96,7 +96,7 @@ __global__ void __launch_bounds__(512)
 kg_ins_leader(const Table* __restrict__ tables, OpDesc op, const uint8_t* __restrict__ keys,
               const uint8_t* __restrict__ vals, const unsigned long long* __restrict__ Q,
               const unsigned long long* __restrict__ qoff, const uint32_t* __restrict__ dslot,
-              unsigned long long* __restrict__ big, uint32_t* nbig, Counters* ctr) {
+              unsigned long long* __restrict__ big, uint32_t* nbig, Counters* ctr, uint32_t coop_arc) {
   extern __shared__ uint32_t lst[];  // kLeaderMax columns of blockDim lanes
   uint32_t* my = lst + threadIdx.x;
   const uint32_t nch = *op.n_chunks;
@@ -115,9 +115,13 @@ kg_ins_leader(const Table* __restrict__ tables, OpDesc op, const uint8_t* __rest
       if (b != s && bit_at(Qt, b)) {
         uint32_t m = 0;
         uint64_t pos = s;
+        bool long_arc = false;
         for (;;) {
           const uint64_t x = bm_next_set(t.freebits, pos, t.cap);
-          if (bit_at(Qt, x)) break;
+          if (bit_at(Qt, x)) {
+            long_arc = ((x - b - 1) & t.mask) > coop_arc;
+            break;
+          }
           if (m == kLeaderMax) {
             m = kLeaderMax + 1;
             break;
@@ -126,7 +130,7 @@ kg_ins_leader(const Table* __restrict__ tables, OpDesc op, const uint8_t* __rest
           ++m;
           pos = (x + 1 == t.cap) ? 0 : x + 1;
         }
-        if (m > kLeaderMax) {
+        if (m > kLeaderMax || long_arc) {
           big[atomicAdd(nbig, 1u)] = (static_cast<unsigned long long>(ch.table) << 32) | s;
         } else {
           for (uint32_t a = 1; a < m; ++a) {
@@ -150,12 +154,13 @@ kg_ins_leader(const Table* __restrict__ tables, OpDesc op, const uint8_t* __rest
 }
 
 template <int KS, int VS>
-__global__ void __launch_bounds__(1024)
+__global__ void __launch_bounds__(64)
 kg_ins_big(const Table* __restrict__ tables, OpDesc op, const uint8_t* __restrict__ keys,
            const uint8_t* __restrict__ vals, const unsigned long long* __restrict__ Q,
            const unsigned long long* __restrict__ qoff, const unsigned long long* __restrict__ big,
            const uint32_t* nbig_p, Counters* ctr) {
   __shared__ uint32_t lst[kBigMax];
+  __shared__ uint32_t carry[2 * kMaxGenericSize / 4];
   __shared__ uint32_t cnt;
   __shared__ unsigned long long barrier_at;
   const uint32_t nbig = *nbig_p;
@@ -220,7 +225,14 @@ kg_ins_big(const Table* __restrict__ tables, OpDesc op, const uint8_t* __restric
           __syncthreads();
         }
       }
-      if (threadIdx.x == 0) {
+      if constexpr (KS > 0 && VS > 0 && KS % 4 == 0 && VS % 4 == 0) {
+        uint32_t claimed = 0;  // one wave per workgroup walks each member
+        for (uint32_t j = 0; j < m; ++j) {
+          const uint64_t gk = to.begin + lst[j];
+          claimed += insert_one_wave<KS, VS>(t, keys + gk * K, vals + gk * V, carry, &ctr[table]);
+        }
+        if (threadIdx.x == 0 && claimed) atomicAdd(&ctr[table].count, static_cast<unsigned long long>(claimed));
+      } else if (threadIdx.x == 0) {
         uint32_t claimed = 0;
         for (uint32_t j = 0; j < m; ++j) {
           const uint64_t gk = to.begin + lst[j];
@@ -419,7 +431,7 @@ inline uint32_t gridn(uint64_t threads, uint32_t block) {
 struct GroupOps {
   void (*insert)(const Table*, OpDesc, const TableOp*, uint32_t, const uint8_t*, const uint8_t*,
                  unsigned long long*, const unsigned long long*, const unsigned long long*, uint64_t, uint32_t*,
-                 unsigned long long*, uint32_t*, Counters*, uint32_t, uint32_t, hipStream_t);
+                 unsigned long long*, uint32_t*, Counters*, uint32_t, uint32_t, uint32_t, hipStream_t);
   void (*get)(const Table*, OpDesc, const uint8_t*, uint8_t*, uint8_t*, uint32_t, uint32_t, uint32_t, hipStream_t);
   void (*remove)(const Table*, OpDesc, const TableOp*, uint32_t, const uint8_t*, unsigned long long*, uint8_t*,
                  Counters*, uint32_t, uint32_t, uint32_t, hipStream_t);
@@ -429,12 +441,12 @@ template <int KS, int VS>
 void g_insert(const Table* tables, OpDesc op, const TableOp* tops, uint32_t n_tables, const uint8_t* keys,
               const uint8_t* vals, unsigned long long* Q, const unsigned long long* freebits_all,
               const unsigned long long* qoff, uint64_t total_words, uint32_t* dslot, unsigned long long* big,
-              uint32_t* nbig, Counters* ctr, uint32_t block, uint32_t grid, hipStream_t s) {
+              uint32_t* nbig, Counters* ctr, uint32_t block, uint32_t grid, uint32_t coop_arc, hipStream_t s) {
   hipLaunchKernelGGL(dev::kg_copy_words, dim3(gridn(total_words, 256)), dim3(256), 0, s, Q, freebits_all, total_words);
   hipLaunchKernelGGL((dev::kg_ins_dry<KS>), dim3(grid), dim3(block), 0, s, tables, op, keys, Q, qoff, dslot, ctr);
   hipLaunchKernelGGL((dev::kg_ins_leader<KS, VS>), dim3(grid), dim3(block), dev::kLeaderMax * block * 4, s, tables, op,
-                     keys, vals, Q, qoff, dslot, big, nbig, ctr);
-  hipLaunchKernelGGL((dev::kg_ins_big<KS, VS>), dim3(128), dim3(1024), 0, s, tables, op, keys, vals, Q, qoff, big,
+                     keys, vals, Q, qoff, dslot, big, nbig, ctr, coop_arc);
+  hipLaunchKernelGGL((dev::kg_ins_big<KS, VS>), dim3(4096), dim3(64), 0, s, tables, op, keys, vals, Q, qoff, big,
                      nbig, ctr);
   hipLaunchKernelGGL((dev::kg_ins_serial<KS, VS>), dim3(gridn(n_tables, 64)), dim3(64), 0, s, tables, tops, n_tables,
                      keys, vals, ctr);
@@ -532,7 +544,7 @@ class GroupImpl {
   hipStream_t stream_ = nullptr;
   std::vector<Table> h_tables_;
   std::vector<uint64_t> len_;
-  fm_group_tuning tune_{4, 256, 1024};
+  fm_group_tuning tune_{4, 256, 1024, 16};
 
  private:
   fm_status fail(fm_status c, const std::string& m) {
@@ -798,7 +810,7 @@ fm_status GroupImpl::stage(int which, const HostOp& op, uint64_t total) {
 fm_status GroupImpl::enqueue_insert(const uint8_t* keys, const uint8_t* vals) {
   OpDesc op{d_tops_[0], d_chunks_[0], d_nch_ + 0};
   ops_.insert(d_tables_, op, d_tops_[0], n_, keys, vals, Q_, free_all_, d_qoff_, total_words_, dslot_, big_, nbig_,
-              ins_ctr_, tune_.block, tune_.grid, stream_);
+              ins_ctr_, tune_.block, tune_.grid, tune_.coop_arc, stream_);
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? FM_OK : fail(FM_ERR_KERNEL_LAUNCH, hmsg("group insert", e));
 }

@@ -89,7 +89,8 @@ class ServeStats(ctypes.Structure):
 
 
 class GroupTuning(ctypes.Structure):
-    _fields_ = [("group", ctypes.c_uint32), ("block", ctypes.c_uint32), ("grid", ctypes.c_uint32)]
+    _fields_ = [("group", ctypes.c_uint32), ("block", ctypes.c_uint32), ("grid", ctypes.c_uint32),
+                ("coop_arc", ctypes.c_uint32)]
 
 
 class KernelStat(ctypes.Structure):
@@ -643,9 +644,9 @@ class FlashMapGroup:
         _check(lib().fm_group_get_tuning(self._h, ctypes.byref(t)))
         return t
 
-    def set_tuning(self, group=None, block=None, grid=None):
+    def set_tuning(self, group=None, block=None, grid=None, coop_arc=None):
         t = self.tuning()
-        for name, val in (("group", group), ("block", block), ("grid", grid)):
+        for name, val in (("group", group), ("block", block), ("grid", grid), ("coop_arc", coop_arc)):
             if val is not None:
                 setattr(t, name, val)
         _check(lib().fm_group_set_tuning(self._h, ctypes.byref(t)))

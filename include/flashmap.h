@@ -282,6 +282,8 @@ typedef struct fm_group_tuning {
   uint32_t group;   /* lanes per key in get/remove probes: 1,2,4,8,16     */
   uint32_t block;   /* threads per workgroup = keys per chunk: 64..512    */
   uint32_t grid;    /* persistent workgroups per launch: 64..8192         */
+  uint32_t coop_arc; /* closures with a longer barrier-to-barrier arc are
+                        walked by a whole wave (0: all of them)          */
 } fm_group_tuning;
 
 fm_status fm_group_create(uint32_t n_tables, const uint64_t* capacities,

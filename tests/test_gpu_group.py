@@ -113,7 +113,7 @@ def test_group_graph_replays_many_steps():
     _run_and_check(_tables(16, 12, 1 << 11, 1 << 9, 1 << 9, 1 << 7), True, steps=4)
 
 
-@pytest.mark.parametrize("tuning", [dict(group=1, block=64, grid=64), dict(group=8, block=512, grid=2048),
+@pytest.mark.parametrize("tuning", [dict(group=1, block=64, grid=64, coop_arc=0), dict(group=8, block=512, grid=2048),
                                     dict(group=16, block=128, grid=300)], ids=["g1b64", "g8b512", "g16b128"])
 def test_group_tunings_agree(tuning):
     say(f"group tuning {tuning}")
