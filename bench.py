@@ -223,7 +223,7 @@ def main():
     avg_s = dms / dn / 1e3
     achieved = per_launch[dom] / avg_s / 1e9
     traffic = None
-    pmc_path = os.path.join(ROOT, "profiles", f"{dom}_c2_pmc.json")
+    pmc_path = os.path.join(ROOT, "profiles", f"{dom}_account_state_10M_pmc.json")
     if os.path.exists(pmc_path):
         with open(pmc_path) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")

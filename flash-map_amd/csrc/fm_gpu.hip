@@ -90,7 +90,7 @@ hipError_t launch_ins_leader(const Table& t, const uint8_t* k, const uint8_t* v,
 template <int KS, int VS>
 hipError_t launch_ins_big(const Table& t, const uint8_t* k, const uint8_t* v, uint64_t n, const unsigned long long* Q,
                           const uint32_t* big, Counters* c, hipStream_t s) {
-  hipLaunchKernelGGL((dev::k_ins_big<KS, VS>), dim3(4096), dim3(64), 0, s, t, k, v, n, Q, big, c);
+  hipLaunchKernelGGL((dev::k_ins_big<KS, VS>), dim3(1024), dim3(64), 0, s, t, k, v, n, Q, big, c);
   return hipGetLastError();
 }
 
@@ -283,7 +283,7 @@ class GpuBackend final : public Backend {
   Table t_{};
   uint64_t len_ = 0;
   Ops ops_{};
-  fm_tuning tune_{4, 256, 4, 256, 16};
+  fm_tuning tune_{4, 256, 4, 256, 64};
   bool tune_sort_only_ = false;  // FM_INSERT_SORT_PATH=1 forces the sort-grouped insert
   bool last_insert_led_ = false;
 

@@ -446,7 +446,7 @@ void g_insert(const Table* tables, OpDesc op, const TableOp* tops, uint32_t n_ta
   hipLaunchKernelGGL((dev::kg_ins_dry<KS>), dim3(grid), dim3(block), 0, s, tables, op, keys, Q, qoff, dslot, ctr);
   hipLaunchKernelGGL((dev::kg_ins_leader<KS, VS>), dim3(grid), dim3(block), dev::kLeaderMax * block * 4, s, tables, op,
                      keys, vals, Q, qoff, dslot, big, nbig, ctr, coop_arc);
-  hipLaunchKernelGGL((dev::kg_ins_big<KS, VS>), dim3(4096), dim3(64), 0, s, tables, op, keys, vals, Q, qoff, big,
+  hipLaunchKernelGGL((dev::kg_ins_big<KS, VS>), dim3(1024), dim3(64), 0, s, tables, op, keys, vals, Q, qoff, big,
                      nbig, ctr);
   hipLaunchKernelGGL((dev::kg_ins_serial<KS, VS>), dim3(gridn(n_tables, 64)), dim3(64), 0, s, tables, tops, n_tables,
                      keys, vals, ctr);
@@ -544,7 +544,7 @@ class GroupImpl {
   hipStream_t stream_ = nullptr;
   std::vector<Table> h_tables_;
   std::vector<uint64_t> len_;
-  fm_group_tuning tune_{4, 256, 1024, 16};
+  fm_group_tuning tune_{4, 256, 1024, 64};
 
  private:
   fm_status fail(fm_status c, const std::string& m) {
