@@ -44,6 +44,8 @@ struct Ops {
   hipError_t (*rem_runs)(const Table&, const uint8_t*, uint64_t, uint32_t*, uint32_t*, uint32_t, hipStream_t);
   hipError_t (*rem_replay)(const Table&, const uint8_t*, uint64_t, const uint32_t*, const uint32_t*, Counters*,
                            uint32_t, hipStream_t);
+  hipError_t (*ins_fused)(const Table&, const dev::FusedInsert&, uint32_t, hipStream_t);
+  hipError_t (*ins_fused_blocks)(int, uint32_t*);
 };
 
 inline uint32_t grid_for(uint64_t threads, uint32_t block) {
@@ -135,6 +137,35 @@ hipError_t launch_rem_replay(const Table& t, const uint8_t* q, uint64_t n, const
   return hipGetLastError();
 }
 
+// Cooperative launch: every workgroup is resident, so the kernel's grid-wide
+// barriers are safe; `max_blocks` is the co-resident limit.
+template <int KS, int VS>
+hipError_t launch_ins_fused(const Table& t, const dev::FusedInsert& a, uint32_t max_blocks, hipStream_t s) {
+  const uint64_t want = (a.n + dev::kFusedBlock - 1) / dev::kFusedBlock;
+  const uint32_t grid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(want, max_blocks)));
+  Table tt = t;
+  dev::FusedInsert aa = a;
+  void* args[] = {&tt, &aa};
+  return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&dev::k_ins_fused<KS, VS>), dim3(grid),
+                                    dim3(dev::kFusedBlock), args, 0, s);
+}
+
+template <int KS, int VS>
+hipError_t ins_fused_blocks(int device, uint32_t* out) {
+  int per_cu = 0, cus = 0, coop = 0;
+  *out = 0;
+  hipError_t e = hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, device);
+  if (e != hipSuccess || !coop) return e;
+  if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&dev::k_ins_fused<KS, VS>),
+                                                        dev::kFusedBlock, 0)) != hipSuccess ||
+      (e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device)) != hipSuccess)
+    return e;
+  // Two per CU at most: far inside the LDS, wave and register limits, so the
+  // grid is resident even where the limits are not met exactly.
+  *out = static_cast<uint32_t>(std::min(std::max(0, per_cu), 2) * std::max(0, cus));
+  return hipSuccess;
+}
+
 template <int KS, int VS>
 Ops make_ops() {
   Ops o;
@@ -146,6 +177,8 @@ Ops make_ops() {
   o.rem_probe = &launch_rem_probe<KS>;
   o.rem_runs = &launch_rem_runs<KS>;
   o.rem_replay = &launch_rem_replay<KS>;
+  o.ins_fused = &launch_ins_fused<KS, VS>;
+  o.ins_fused_blocks = &ins_fused_blocks<KS, VS>;
   return o;
 }
 
@@ -259,12 +292,26 @@ class GpuBackend final : public Backend {
   fm_status check_insert_status(const Counters& c) {
     if (c.status & dev::kStatusBudget)
       return fail(FM_ERR_KERNEL_LAUNCH, "insert walk found no free slot in a full table (uncounted inserts overfilled it)");
-    if ((c.status & dev::kStatusDryFail) && last_insert_led_) {
-      // Unreachable while len never undercounts occupied slots; the leader
-      // kernels skipped the batch, so only owner[] needs restoring.
+    if (c.status & dev::kStatusStall) {
+      // A grid barrier of the fused insert timed out: no workgroup reached the
+      // phase that writes the table.  Restore owner[] and Q, and use the
+      // multi-kernel path from now on.
+      q_synced_ = false;
+      fused_blocks_ = 0;
       hipLaunchKernelGGL(dev::k_fill_u32, dim3(grid_for(t_.cap, 256)), dim3(256), 0, stream_, t_.owner, t_.cap,
                          dev::kNoOwner);
       (void)hipStreamSynchronize(stream_);
+      return fail(FM_ERR_KERNEL_LAUNCH, "fused insert: workgroups were not all resident; batch not applied");
+    }
+    if ((c.status & dev::kStatusDryFail) && last_insert_ != InsertPath::kSort) {
+      // Unreachable while len never undercounts occupied slots; the leader
+      // kernels skipped the batch, so only owner[] (and Q) need restoring.
+      q_synced_ = false;
+      if (last_insert_ == InsertPath::kLed) {
+        hipLaunchKernelGGL(dev::k_fill_u32, dim3(grid_for(t_.cap, 256)), dim3(256), 0, stream_, t_.owner, t_.cap,
+                           dev::kNoOwner);
+        (void)hipStreamSynchronize(stream_);
+      }
       return fail(FM_ERR_KERNEL_LAUNCH, "insert: free-slot accounting disagrees with the table; batch not applied");
     }
     return FM_OK;
@@ -283,9 +330,16 @@ class GpuBackend final : public Backend {
   Table t_{};
   uint64_t len_ = 0;
   Ops ops_{};
-  fm_tuning tune_{4, 256, 4, 256, 64};
+  fm_tuning tune_{4, 256, 4, 256, 64, 0};
   bool tune_sort_only_ = false;  // FM_INSERT_SORT_PATH=1 forces the sort-grouped insert
-  bool last_insert_led_ = false;
+  enum class InsertPath { kSort, kLed, kFused };
+  InsertPath last_insert_ = InsertPath::kSort;
+  uint32_t fused_blocks_ = 0;  // k_ins_fused workgroups per launch at most (0: no cooperative launch)
+  uint32_t fused_spin_ = 1u << 24;  // grid barrier polls (FM_FUSED_SPIN), ~tens of seconds
+  // Stream-ordered facts about device state, kept by the host:
+  bool q_synced_ = false;      // Q_ == freebits (the fused insert keeps it so)
+  bool ctr_zero_ = false;      // ctr_ holds zeros
+  bool ctr_in_host_ = false;   // the last op left its final counters in h_ctr_
 
   // scratch (grown geometrically, reused)
   uint64_t scratch_n_ = 0;
@@ -308,7 +362,7 @@ class GpuBackend final : public Backend {
   enum {
     kKidGet, kKidInsCopy, kKidInsDry, kKidInsClosure, kKidInsSort, kKidInsReplay, kKidRemProbe, kKidRemCheck,
     kKidRemApply, kKidRemRuns, kKidRemSort, kKidRemReplay, kKidCopyTable, kKidInsLeader, kKidInsBig, kKidInsBits,
-    kNumKid
+    kKidInsFused, kNumKid
   };
   struct PendingTiming {
     int kid;
@@ -446,6 +500,8 @@ fm_status GpuBackend::init(const fm_builder& b, std::string* err) {
     return FM_ERR_GPU_ALLOC;
   }
   ops_ = select_ops(t_.K, t_.V);
+  if (ops_.ins_fused_blocks(dev_, &fused_blocks_) != hipSuccess) fused_blocks_ = 0;  // multi-kernel path instead
+  if (const char* env = std::getenv("FM_FUSED_SPIN")) fused_spin_ = static_cast<uint32_t>(std::strtoul(env, nullptr, 10));
   if (const char* env = std::getenv("FM_INSERT_SORT_PATH")) tune_sort_only_ = env[0] == '1';
   sort_end_bit_ = 1;
   while ((1ull << sort_end_bit_) <= cap) ++sort_end_bit_;  // ids in [0, cap]
@@ -462,8 +518,10 @@ fm_status GpuBackend::init(const fm_builder& b, std::string* err) {
 fm_status GpuBackend::set_tuning(const fm_tuning* t) {
   auto pow2 = [](uint32_t g) { return g >= 1 && g <= 16 && (g & (g - 1)) == 0; };
   auto blk = [](uint32_t b) { return b >= 64 && b <= 1024 && b % 64 == 0; };
-  if (!t || !pow2(t->get_group) || !pow2(t->remove_group) || !blk(t->get_block) || !blk(t->insert_block))
-    return fail(FM_ERR_INVALID_ARGUMENT, "tuning: group must be 1,2,4,8,16; block a multiple of 64 in 64..1024");
+  if (!t || !pow2(t->get_group) || !pow2(t->remove_group) || !blk(t->get_block) || !blk(t->insert_block) ||
+      t->insert_fused > 1)
+    return fail(FM_ERR_INVALID_ARGUMENT,
+                "tuning: group must be 1,2,4,8,16; block a multiple of 64 in 64..1024; insert_fused 0 or 1");
   tune_ = *t;
   return FM_OK;
 }
@@ -526,14 +584,19 @@ fm_status GpuBackend::ensure_staging(uint64_t bk, uint64_t bv, uint64_t bo, uint
   return FM_OK;
 }
 
+// Zeroes the counters for an op that reads them back with read_counters.
 fm_status GpuBackend::zero_counters() {
   hipError_t e = hipMemsetAsync(ctr_, 0, sizeof(Counters), stream_);
+  ctr_zero_ = false;  // the op about to run accumulates into them
+  ctr_in_host_ = false;
   return e == hipSuccess ? FM_OK : fail(FM_ERR_KERNEL_LAUNCH, hip_msg("hipMemsetAsync", e));
 }
 
 fm_status GpuBackend::read_counters(Counters* out) {
-  hipError_t e = hipMemcpyAsync(h_ctr_, ctr_, sizeof(Counters), hipMemcpyDeviceToHost, stream_);
+  hipError_t e = hipSuccess;
+  if (!ctr_in_host_) e = hipMemcpyAsync(h_ctr_, ctr_, sizeof(Counters), hipMemcpyDeviceToHost, stream_);
   if (e == hipSuccess) e = hipStreamSynchronize(stream_);
+  ctr_in_host_ = false;
   if (e != hipSuccess) return fail(FM_ERR_TRANSFER, hip_msg("counter readback", e));
   *out = *h_ctr_;
   return FM_OK;
@@ -557,20 +620,45 @@ fm_status GpuBackend::run_insert(const uint8_t* d_keys, const uint8_t* d_vals, u
   Table t = t_;
   t.outcome = outcome;
   if ((st = ensure_scratch(n)) != FM_OK) return st;
-  if ((st = zero_counters()) != FM_OK) return st;
   const uint64_t words = (t_.cap + 63) / 64;
   const uint32_t blk = tune_.insert_block;
+  // With len + n < cap the dry pass provably leaves a free slot (len never
+  // undercounts occupied slots), so closures are bounded by barriers and are
+  // led without sorting.  Otherwise (the batch may fill the table) group by
+  // closure id with a stable radix sort, which also covers "no barrier".
+  const bool led = len_ + n < t_.cap && !tune_sort_only_;
+  if (led && tune_.insert_fused && fused_blocks_) {
+    last_insert_ = InsertPath::kFused;
+    if (!q_synced_) {
+      Prof p(this, kKidInsCopy);
+      hipError_t e = hipMemcpyAsync(Q_, t_.freebits, words * 8, hipMemcpyDeviceToDevice, stream_);
+      if (e != hipSuccess) return fail(FM_ERR_TRANSFER, hip_msg("bitmap copy", e));
+    }
+    if (!ctr_zero_) {
+      hipError_t e = hipMemsetAsync(ctr_, 0, sizeof(Counters), stream_);
+      if (e != hipSuccess) return fail(FM_ERR_KERNEL_LAUNCH, hip_msg("hipMemsetAsync", e));
+    }
+    dev::FusedInsert a{d_keys, d_vals, n,      Q_,     dslot_, cid_,  ev_, big_, cid_s_, idx_s_, ctr_, h_ctr_,
+                       tune_.insert_coop_arc, fused_spin_};
+    Prof p(this, kKidInsFused);
+    if ((st = check_launch(ops_.ins_fused(t, a, fused_blocks_, stream_), "k_ins_fused")) != FM_OK) {
+      q_synced_ = ctr_zero_ = ctr_in_host_ = false;
+      return st;
+    }
+    q_synced_ = true;  // unless the dry pass failed (check_insert_status)
+    ctr_zero_ = true;
+    ctr_in_host_ = true;
+    return FM_OK;
+  }
+  if ((st = zero_counters()) != FM_OK) return st;
+  q_synced_ = false;  // Q_ is this op's claim bitmap
   {
     Prof p(this, kKidInsCopy);
     hipError_t e = hipMemcpyAsync(Q_, t_.freebits, words * 8, hipMemcpyDeviceToDevice, stream_);
     if (e != hipSuccess) return fail(FM_ERR_TRANSFER, hip_msg("bitmap copy", e));
   }
-  // With len + n < cap the dry pass provably leaves a free slot (len never
-  // undercounts occupied slots), so closures are bounded by barriers and are
-  // led without sorting.  Otherwise (the batch may fill the table) group by
-  // closure id with a stable radix sort, which also covers "no barrier".
-  last_insert_led_ = len_ + n < t_.cap && !tune_sort_only_;
-  if (last_insert_led_) {
+  last_insert_ = led ? InsertPath::kLed : InsertPath::kSort;
+  if (led) {
     {
       Prof p(this, kKidInsDry);
       if ((st = check_launch(ops_.ins_dry(t_, d_keys, n, Q_, nullptr, dslot_, ctr_, blk, stream_), "k_ins_dry")) !=
@@ -619,6 +707,7 @@ fm_status GpuBackend::run_remove(const uint8_t* d_keys, uint64_t n, uint64_t* re
   fm_status st;
   Table t = t_;
   t.outcome = outcome;
+  t.freecopy = q_synced_ ? Q_ : nullptr;  // tombstones free slots in Q_ too
   if ((st = ensure_scratch(n)) != FM_OK) return st;
   if ((st = zero_counters()) != FM_OK) return st;
   const uint32_t blk = tune_.insert_block;
@@ -674,7 +763,8 @@ const char* GpuBackend::kid_name(int k) {
   static const char* names[kNumKid] = {"k_get",        "ins_bitmap_copy", "k_ins_dry",    "k_ins_closure",
                                        "ins_radix_sort", "k_ins_replay",  "k_rem_probe",  "k_rem_check",
                                        "k_rem_apply",  "k_rem_runs",      "rem_radix_sort", "k_rem_replay",
-                                       "copy_table",   "k_ins_leader",    "k_ins_big",    "k_ins_bits"};
+                                       "copy_table",   "k_ins_leader",    "k_ins_big",    "k_ins_bits",
+                                       "k_ins_fused"};
   return names[k];
 }
 
@@ -809,6 +899,7 @@ fm_status GpuBackend::bulk_remove(const void* keys, uint64_t n, uint64_t* out_re
 fm_status GpuBackend::clear() {
   fm_status st;
   if ((st = use_device()) != FM_OK) return st;
+  q_synced_ = false;
   hipLaunchKernelGGL(dev::k_clear, dim3(grid_for(t_.cap, 256)), dim3(256), 0, stream_, t_);
   if ((st = check_launch(hipGetLastError(), "k_clear")) != FM_OK) return st;
   hipError_t e = hipStreamSynchronize(stream_);
@@ -844,6 +935,7 @@ fm_status GpuBackend::import_slots(const uint32_t* flags, const void* keys, cons
       (e = hipMemcpyAsync(t_.keys, keys, t_.cap * t_.K, hipMemcpyHostToDevice, stream_)) != hipSuccess ||
       (e = hipMemcpyAsync(t_.vals, values, t_.cap * t_.V, hipMemcpyHostToDevice, stream_)) != hipSuccess)
     return fail(FM_ERR_TRANSFER, hip_msg("import", e));
+  q_synced_ = false;
   hipLaunchKernelGGL(dev::k_rebuild_bits, dim3(grid_for(t_.cap, 256)), dim3(256), 0, stream_, t_);
   if ((st = check_launch(hipGetLastError(), "k_rebuild_bits")) != FM_OK) return st;
   if ((e = hipStreamSynchronize(stream_)) != hipSuccess) return fail(FM_ERR_TRANSFER, hip_msg("import", e));
@@ -984,6 +1076,8 @@ fm_status GpuBackend::copy_from(Backend& other, uint32_t parts) {
     e = hipMemcpyAsync(t_.vals, src->t_.vals, t_.cap * t_.V, hipMemcpyDeviceToDevice, stream_);
   if (e == hipSuccess) e = hipMemcpyAsync(t_.freebits, src->t_.freebits, words * 8, hipMemcpyDeviceToDevice, stream_);
   if (e == hipSuccess) e = hipMemcpyAsync(t_.emptybits, src->t_.emptybits, words * 8, hipMemcpyDeviceToDevice, stream_);
+  if (e == hipSuccess && Q_) e = hipMemcpyAsync(Q_, src->t_.freebits, words * 8, hipMemcpyDeviceToDevice, stream_);
+  q_synced_ = e == hipSuccess && Q_ != nullptr;
   if (e != hipSuccess) return fail(FM_ERR_TRANSFER, hip_msg("fm_copy_table", e));
   len_ = src->len_;
   return FM_OK;

@@ -52,6 +52,7 @@ constexpr uint32_t kNoOwner = 0xFFFFFFFFu;
 constexpr uint32_t kStatusDryFail = 1u;   // dry pass found no free slot
 constexpr uint32_t kStatusConflict = 2u;  // remove batch needs exact replay
 constexpr uint32_t kStatusBudget = 4u;    // an insert walk exceeded cap+2 steps
+constexpr uint32_t kStatusStall = 8u;     // fused insert: a grid barrier timed out
 
 // Event kinds of a probe (get / remove).
 constexpr uint32_t kEvNone = 0u;      // probed all cap slots
@@ -74,12 +75,21 @@ struct Table {
   // Per-op, not table state: when set, inserts write 1/0 per batch key for
   // "claimed a free slot" and removes 1/0 for "tombstoned a slot".
   uint8_t* outcome;
+  // Per-op: when set, a copy of freebits that removes keep equal to it (the
+  // fused insert's claim bitmap, so the next insert needs no copy).
+  unsigned long long* freecopy;
 };
 
 struct Counters {
   uint32_t status;
-  uint32_t nbig;  // closures queued for k_ins_big
+  uint32_t nbig;  // closures queued for the wave replay
   unsigned long long count;
+  uint32_t done;      // fused insert: workgroups finished
+  uint32_t big_next;  // fused insert: next queued closure to replay
+  uint32_t arrive;    // fused insert: grid barrier arrivals
+  uint32_t gen;       // fused insert: grid barrier generation
+  uint32_t nsmall;    // fused insert: closures queued for the register replay
+  uint32_t small_next;
 };
 
 __device__ __forceinline__ uint32_t make_flag(uint32_t state, uint64_t dist) {
@@ -178,6 +188,20 @@ __device__ inline uint64_t bm_next_set(const unsigned long long* bm, uint64_t st
     if (word) return (w << 6) + static_cast<uint64_t>(__ffsll(word) - 1);
     w = (w + 1 == words) ? 0 : w + 1;
     word = bm[w];
+  }
+  return cap;
+}
+
+// bm_next_set over a bitmap that other workgroups of the same launch modify
+// with atomics (every word read with a device-scope load).
+__device__ inline uint64_t bm_next_set_agent(const unsigned long long* bm, uint64_t start, uint64_t cap) {
+  const uint64_t words = (cap + 63) >> 6;
+  uint64_t w = start >> 6;
+  unsigned long long word = load_word(&bm[w]) & (~0ull << (start & 63));
+  for (uint64_t it = 0; it <= words; ++it) {
+    if (word) return (w << 6) + static_cast<uint64_t>(__ffsll(word) - 1);
+    w = (w + 1 == words) ? 0 : w + 1;
+    word = load_word(&bm[w]);
   }
   return cap;
 }
@@ -282,6 +306,23 @@ __device__ __forceinline__ void wave_add(unsigned long long* dst, uint32_t v) {
   if ((threadIdx.x & 63) == 0 && sum) atomicAdd(dst, sum);
 }
 
+// Adds per-thread counts into *dst with one atomic per workgroup: atomics
+// on one address serialise (about 14 ns each on MI355X), so counters are
+// reduced in the workgroup first.  Called by every thread of the workgroup.
+__device__ inline void block_add(unsigned long long* dst, unsigned long long v) {
+  __shared__ unsigned long long part[16];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  __syncthreads();  // part[] may still be read by a previous call
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long sum = 0;
+    for (uint32_t w = 0; w < (blockDim.x + 63) / 64; ++w) sum += part[w];
+    if (sum) atomicAdd(dst, sum);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // bulk_get (rayon_cpu.rs:82-126).  One group of G lanes per query.
 // out_found may be null (bulk_get_values_device, kernels.cu:481-556).
@@ -365,15 +406,13 @@ k_ins_closure(uint64_t cap, const unsigned long long* __restrict__ Q, const uint
 // kPublishBits: clear the claimed slot's bitmap bits here.  The leader path
 // passes false: there t.freebits is the pre-batch set other lanes are still
 // scanning, and k_ins_bits publishes the claims after the replay.
-template <int KS, int VS, bool kPublishBits = true>
-__device__ inline uint32_t insert_one(const Table& t, const uint8_t* __restrict__ kin, const uint8_t* __restrict__ vin,
-                                      uint32_t K, uint32_t V, Counters* ctr) {
-  Blob<KS> ck;
-  Blob<VS> cv;
-  ck.load(kin, K);
-  cv.load(vin, V);
-  uint64_t home = ck.hash(K, t.mode) & t.mask;
-  uint64_t p = 0;
+// insert_walk is the same walk for a pair already in registers, entered at
+// probe position p (the caller has established that positions 0..p-1 are
+// OCCUPIED with dist >= position and hold no equal key); it reports the
+// slot it claimed in *at.
+template <int KS, int VS, bool kPublishBits>
+__device__ inline uint32_t insert_walk(const Table& t, Blob<KS> ck, Blob<VS> cv, uint64_t home, uint64_t p, uint32_t K,
+                                       uint32_t V, Counters* ctr, uint64_t* at) {
   for (uint64_t step = 0;; ++step) {
     if (p > t.mask) return 0;  // carried pair dropped (rayon_cpu.rs:167-169)
     if (step > t.cap + 1) {
@@ -416,8 +455,23 @@ __device__ inline uint32_t insert_one(const Table& t, const uint8_t* __restrict_
       atomicAnd(&t.freebits[idx >> 6], ~bit);
       atomicAnd(&t.emptybits[idx >> 6], ~bit);
     }
+    *at = idx;
     return 1;
   }
+}
+
+template <int KS, int VS, bool kPublishBits = true>
+__device__ inline uint32_t insert_one(const Table& t, const uint8_t* __restrict__ kin, const uint8_t* __restrict__ vin,
+                                      uint32_t K, uint32_t V, Counters* ctr, uint64_t* at = nullptr) {
+  Blob<KS> ck;
+  Blob<VS> cv;
+  ck.load(kin, K);
+  cv.load(vin, V);
+  const uint64_t home = ck.hash(K, t.mode) & t.mask;
+  uint64_t where = 0;
+  const uint32_t c = insert_walk<KS, VS, kPublishBits>(t, ck, cv, home, 0, K, V, ctr, &where);
+  if (at) *at = where;
+  return c;
 }
 
 // Pass 3: one lane per closure replays its keys in batch order.
@@ -427,10 +481,10 @@ k_ins_replay(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __rest
              const uint32_t* __restrict__ cid_s, const uint32_t* __restrict__ idx_s, Counters* ctr) {
   const uint32_t K = KS ? KS : t.K;
   const uint32_t V = VS ? VS : t.V;
+  unsigned long long claimed = 0;
   for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * blockDim.x;; base += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
     if (base >= n) break;
     const uint64_t s = base + threadIdx.x;
-    uint32_t claimed = 0;
     if (s < n && (s == 0 || cid_s[s] != cid_s[s - 1])) {
       const uint32_t c = cid_s[s];
       for (uint64_t u = s; u < n && cid_s[u] == c; ++u) {
@@ -440,8 +494,8 @@ k_ins_replay(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __rest
         if (t.outcome) t.outcome[i] = static_cast<uint8_t>(c1);
       }
     }
-    wave_add(&ctr->count, claimed);
   }
+  block_add(&ctr->count, claimed);
 }
 
 // ---------------------------------------------------------------------------
@@ -474,6 +528,14 @@ __device__ inline uint64_t bm_prev_set(const unsigned long long* bm, uint64_t st
   return start;
 }
 
+// Orders LDS accesses between the lanes of one wave (a wave's LDS operations
+// complete in issue order; the fences keep the compiler from moving them).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // ---------------------------------------------------------------------------
 // The same sequential insert as insert_one, walked by a whole wave: the 64
 // lanes load 64 consecutive slots (flag + key) in one round trip, the walk's
@@ -494,7 +556,7 @@ __device__ uint32_t insert_one_wave(const Table& t, const uint8_t* __restrict__ 
   const int lane = threadIdx.x & 63;
   for (int w = lane; w < KW; w += 64) carry[w] = reinterpret_cast<const uint32_t*>(kin)[w];
   for (int w = lane; w < VW; w += 64) carry[KW + w] = reinterpret_cast<const uint32_t*>(vin)[w];
-  __syncthreads();
+  wave_sync();
   Blob<KS> ck;  // carried key, identical in every lane
 #pragma unroll
   for (int w = 0; w < KW; ++w) ck.w[w] = carry[w];
@@ -600,14 +662,14 @@ __device__ uint32_t insert_one_wave(const Table& t, const uint8_t* __restrict__ 
     }
     if (end) return claimed;
     // Window exhausted: the pair carried into the next window goes to LDS.
-    __syncthreads();
+    wave_sync();
     if (c_src >= 0 && lane == c_src) {
 #pragma unroll
       for (int w = 0; w < KW; ++w) carry[w] = rk.w[w];
 #pragma unroll
       for (int w = 0; w < VW; ++w) carry[KW + w] = rv.w[w];
     }
-    __syncthreads();
+    wave_sync();
     pos = (pos + 64) & t.mask;
   }
 }
@@ -626,10 +688,10 @@ k_ins_leader(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __rest
   const unsigned long long* P = t.freebits;
   uint32_t* my = lst + threadIdx.x;
   if (ctr->status & kStatusDryFail) return;  // block-uniform
+  unsigned long long claimed = 0;
   for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * blockDim.x;; base += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
     if (base >= n) break;
     const uint64_t i = base + threadIdx.x;
-    uint32_t claimed = 0;
     if (i < n) {
       const uint64_t s = dslot[i];
       const uint64_t b = bm_prev_set(P, s, t.cap);
@@ -672,14 +734,131 @@ k_ins_leader(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __rest
         }
       }
     }
-    wave_add(&ctr->count, claimed);
   }
+  block_add(&ctr->count, claimed);
 }
 
-// One workgroup per large closure: gather the members' indices from the arc
-// into LDS (in windows of kBigMax batch indices when there are more), sort
-// them (bitonic), and replay them in order, each walked by the whole wave
-// (static sizes) or by lane 0.  Launched with 64-thread workgroups.
+// Replays one closure with a whole wave (64 active lanes).  The arc runs
+// from `s`, the dry slot of the closure's first member, up to its barrier
+// (the next bit still set in Q).  The members' batch indices are gathered
+// from owner[] at the arc's dry-claimed slots (P & ~Q), sorted (bitonic, in
+// windows of `lcap` indices when there are more), and inserted in batch
+// order, each walked by the whole wave (static sizes) or by lane 0.  With
+// kPublish the arc's dry claims are settled afterwards: a slot that ended
+// OCCUPIED leaves both bitmaps, any other gets its Q bit back, and its
+// owner[] entry is reset.  Returns the free slots claimed (in every lane).
+template <int KS, int VS, bool kPublish>
+__device__ uint32_t replay_arc(const Table& t, const uint8_t* __restrict__ in_keys,
+                               const uint8_t* __restrict__ in_vals, uint64_t n, unsigned long long* Q, uint64_t s,
+                               uint32_t* lst, uint32_t lcap, uint32_t* carry, uint32_t* cnt, Counters* ctr) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t K = KS ? KS : t.K;
+  const uint32_t V = VS ? VS : t.V;
+  const unsigned long long* P = t.freebits;
+  const uint64_t words = (t.cap + 63) >> 6;
+  long long xb = 0;
+  if (lane == 0) xb = static_cast<long long>(bm_next_set_agent(Q, s, t.cap));
+  const uint64_t x = static_cast<uint64_t>(__shfl(xb, 0, 64));
+  const uint64_t arc = (x + t.cap - s) & t.mask;  // slots s .. x-1
+  const uint64_t w0 = s >> 6;
+  const uint64_t nwords = min(((s & 63) + arc + 63) >> 6, words);
+  uint32_t claimed = 0;
+  uint64_t lo = 0, hi = ~0ull;
+  bool windowed = false;
+  while (lo < n) {
+    if (lane == 0) *cnt = 0;
+    wave_sync();
+    for (uint64_t wi = lane; wi < nwords; wi += 64) {
+      const uint64_t w = (w0 + wi) % words;
+      unsigned long long bits = P[w] & ~load_word(&Q[w]);
+      while (bits) {
+        const int b = __ffsll(bits) - 1;
+        bits &= bits - 1;
+        const uint64_t p = (w << 6) + b;
+        if (((p + t.cap - s) & t.mask) >= arc) continue;
+        const uint32_t idx = __hip_atomic_load(&t.owner[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (idx < lo || idx >= hi) continue;
+        const uint32_t k = atomicAdd(cnt, 1u);
+        if (k < lcap) lst[k] = idx;
+      }
+    }
+    wave_sync();
+    const uint32_t m = *cnt;
+    wave_sync();
+    if (m > lcap) {  // too many for one pass: windows of lcap indices
+      windowed = true;
+      hi = lo + lcap;
+      continue;
+    }
+    uint32_t p2 = 1;
+    while (p2 < m) p2 <<= 1;
+    for (uint32_t k = m + lane; k < p2; k += 64) lst[k] = 0xFFFFFFFFu;
+    wave_sync();
+    for (uint32_t size = 2; size <= p2; size <<= 1) {
+      for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+        for (uint32_t k = lane; k < p2; k += 64) {
+          const uint32_t j = k ^ stride;
+          if (j > k) {
+            const uint32_t a = lst[k], b2 = lst[j];
+            if ((a > b2) == ((k & size) == 0)) {
+              lst[k] = b2;
+              lst[j] = a;
+            }
+          }
+        }
+        wave_sync();
+      }
+    }
+    if constexpr (KS > 0 && VS > 0 && KS % 4 == 0 && VS % 4 == 0) {
+      for (uint32_t j = 0; j < m; ++j) {
+        const uint64_t k = lst[j];
+        const uint32_t c1 = insert_one_wave<KS, VS>(t, in_keys + k * K, in_vals + k * V, carry, ctr);
+        claimed += c1;
+        if (lane == 0 && t.outcome) t.outcome[k] = static_cast<uint8_t>(c1);
+      }
+    } else {
+      uint32_t c0 = 0;
+      if (lane == 0) {
+        for (uint32_t j = 0; j < m; ++j) {
+          const uint64_t k = lst[j];
+          const uint32_t c1 = insert_one<KS, VS, false>(t, in_keys + k * K, in_vals + k * V, K, V, ctr);
+          c0 += c1;
+          if (t.outcome) t.outcome[k] = static_cast<uint8_t>(c1);
+        }
+      }
+      claimed += static_cast<uint32_t>(__shfl(static_cast<int>(c0), 0, 64));
+    }
+    wave_sync();
+    if (!windowed) break;
+    lo = hi;
+    hi = lo + lcap;
+  }
+  if constexpr (kPublish) {
+    __threadfence_block();  // the walks' flag stores, seen by every lane
+    for (uint64_t wi = lane; wi < nwords; wi += 64) {
+      const uint64_t w = (w0 + wi) % words;
+      unsigned long long bits = P[w] & ~load_word(&Q[w]);
+      unsigned long long taken = 0, back = 0;
+      while (bits) {
+        const int b = __ffsll(bits) - 1;
+        bits &= bits - 1;
+        const uint64_t p = (w << 6) + b;
+        if (((p + t.cap - s) & t.mask) >= arc) continue;
+        if ((t.flags[p] & 0xFu) == kOccupied) taken |= 1ull << b;
+        else back |= 1ull << b;
+        __hip_atomic_store(&t.owner[p], kNoOwner, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      if (taken) {
+        atomicAnd(&t.freebits[w], ~taken);
+        atomicAnd(&t.emptybits[w], ~taken);
+      }
+      if (back) atomicOr(&Q[w], back);
+    }
+  }
+  return claimed;
+}
+
+// One single-wave workgroup per queued closure.
 template <int KS, int VS>
 __global__ void __launch_bounds__(64)
 k_ins_big(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __restrict__ in_vals, uint64_t n,
@@ -687,92 +866,11 @@ k_ins_big(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __restric
   __shared__ uint32_t lst[kBigMax];
   __shared__ uint32_t carry[2 * kMaxGenericSize / 4];
   __shared__ uint32_t cnt;
-  __shared__ unsigned long long barrier_at;
-  const uint32_t K = KS ? KS : t.K;
-  const uint32_t V = VS ? VS : t.V;
-  const unsigned long long* P = t.freebits;
-  const uint64_t words = (t.cap + 63) >> 6;
   const uint32_t nbig = __hip_atomic_load(&ctr->nbig, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   for (uint32_t c = blockIdx.x; c < nbig; c += gridDim.x) {
-    const uint64_t s = big[c];
-    if (threadIdx.x == 0) barrier_at = bm_next_set(Q, s, t.cap);
-    __syncthreads();
-    const uint64_t arc = (barrier_at + t.cap - s) & t.mask;  // slots s .. barrier-1
-    const uint64_t w0 = s >> 6;
-    const uint64_t nwords = min(((s & 63) + arc + 63) >> 6, words);
-    uint64_t lo = 0, hi = ~0ull;
-    bool windowed = false;
-    while (lo < n) {
-      if (threadIdx.x == 0) cnt = 0;
-      __syncthreads();
-      for (uint64_t wi = threadIdx.x; wi < nwords; wi += blockDim.x) {
-        const uint64_t w = (w0 + wi) % words;
-        unsigned long long bits = P[w] & ~Q[w];
-        while (bits) {
-          const int b = __ffsll(bits) - 1;
-          bits &= bits - 1;
-          const uint64_t p = (w << 6) + b;
-          if (((p + t.cap - s) & t.mask) >= arc) continue;
-          const uint32_t idx = t.owner[p];
-          if (idx < lo || idx >= hi) continue;
-          const uint32_t k = atomicAdd(&cnt, 1u);
-          if (k < kBigMax) lst[k] = idx;
-        }
-      }
-      __syncthreads();
-      const uint32_t m = cnt;
-      if (m > kBigMax) {  // too many for one pass: windows of kBigMax indices
-        windowed = true;
-        hi = lo + kBigMax;
-        __syncthreads();
-        continue;
-      }
-      uint32_t p2 = 1;
-      while (p2 < m) p2 <<= 1;
-      for (uint32_t k = m + threadIdx.x; k < p2; k += blockDim.x) lst[k] = 0xFFFFFFFFu;
-      __syncthreads();
-      for (uint32_t size = 2; size <= p2; size <<= 1) {
-        for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
-          for (uint32_t k = threadIdx.x; k < p2; k += blockDim.x) {
-            const uint32_t j = k ^ stride;
-            if (j > k) {
-              const uint32_t a = lst[k], b2 = lst[j];
-              if ((a > b2) == ((k & size) == 0)) {
-                lst[k] = b2;
-                lst[j] = a;
-              }
-            }
-          }
-          __syncthreads();
-        }
-      }
-      if constexpr (KS > 0 && VS > 0 && KS % 4 == 0 && VS % 4 == 0) {
-        // Launched as one wave per workgroup: the wave walks each member.
-        uint32_t claimed = 0;
-        for (uint32_t j = 0; j < m; ++j) {
-          const uint64_t k = lst[j];
-          const uint32_t c1 = insert_one_wave<KS, VS>(t, in_keys + k * K, in_vals + k * V, carry, ctr);
-          claimed += c1;
-          if (threadIdx.x == 0 && t.outcome) t.outcome[k] = static_cast<uint8_t>(c1);
-        }
-        if (threadIdx.x == 0 && claimed) atomicAdd(&ctr->count, static_cast<unsigned long long>(claimed));
-      } else {
-        if (threadIdx.x == 0) {
-          uint32_t claimed = 0;
-          for (uint32_t j = 0; j < m; ++j) {
-            const uint64_t k = lst[j];
-            const uint32_t c1 = insert_one<KS, VS, false>(t, in_keys + k * K, in_vals + k * V, K, V, ctr);
-            claimed += c1;
-            if (t.outcome) t.outcome[k] = static_cast<uint8_t>(c1);
-          }
-          if (claimed) atomicAdd(&ctr->count, static_cast<unsigned long long>(claimed));
-        }
-      }
-      __syncthreads();
-      if (!windowed) break;
-      lo = hi;
-      hi = lo + kBigMax;
-    }
+    const uint32_t claimed = replay_arc<KS, VS, false>(t, in_keys, in_vals, n, const_cast<unsigned long long*>(Q),
+                                                       big[c], lst, kBigMax, carry, &cnt, ctr);
+    if (threadIdx.x == 0 && claimed) atomicAdd(&ctr->count, static_cast<unsigned long long>(claimed));
   }
 }
 
@@ -790,6 +888,616 @@ k_ins_bits(Table t, const uint32_t* __restrict__ dslot, uint64_t n, const Counte
       atomicAnd(&t.emptybits[s >> 6], ~bit);
     }
     t.owner[s] = kNoOwner;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Fused insert: the leader path in one cooperative launch.  Three phases over
+// the same key-to-lane mapping, separated by grid-wide barriers:
+//   A  each key dry-claims a Q bit (Q == freebits on entry) and records the
+//      first event of its sequential insert on the pre-batch table;
+//   B  election: a key leads its closure when the nearest pre-batch free slot
+//      below its dry slot is a barrier; a leader counts its closure's members
+//      (dry-claimed slots up to the next barrier) and queues closures too
+//      large or too long for one wave;
+//   C  replay: a one-member closure applies its phase-A event directly (value
+//      update in place, claim, or a shift of the occupied run up to its dry
+//      slot); a closure of several members whose arc fits in 64 slots is
+//      replayed by its whole wave with the arc held in registers; queued
+//      closures are replayed by whole waves from the queue.  Each closure
+//      then settles its own dry claims in freebits/emptybits and in Q, so
+//      that Q == freebits again on exit.
+// Phase A's event is exact for a one-member closure: only its leader writes
+// inside the closure's arc, and only in phase C.  Data that crosses
+// workgroups inside the launch (Q, owner[], the queue) moves through
+// device-scope atomics only, so the barriers need no cache maintenance.  The
+// last workgroup to finish copies the counters to host memory and zeroes them
+// for the next launch.  256-thread workgroups, launched cooperatively (all
+// resident) and at most two per CU; the barriers' waits are bounded.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
+constexpr uint32_t kCodeBig = 0xFFFFFFFFu;  // election code: queued for a wave
+constexpr uint32_t kCodeQueued = 0xFFFFFFFEu;  // election code: queued for the register replay
+constexpr uint32_t kEvFree = 4u;            // insert event: EMPTY or TOMBSTONE (claim)
+constexpr uint64_t kProbeMax = 64;          // positions phase A decides an event within
+constexpr uint32_t kFusedBlock = 256;
+constexpr uint32_t kWaveLst = 1024;         // closure members one wave sorts at a time
+constexpr uint32_t kShiftMax = 16;          // longest run a one-member shift handles
+
+struct FusedInsert {
+  const uint8_t* keys;
+  const uint8_t* vals;
+  uint64_t n;
+  unsigned long long* Q;  // claim bitmap, == freebits on entry and on exit
+  uint32_t* dslot;        // per key: dry slot
+  uint32_t* code;         // per key: election result (0 follower, m, kCodeQueued, kCodeBig)
+  unsigned long long* ev; // per key: phase-A event (kind << 32 | slot)
+  uint32_t* big;          // queued closures (dry slot of their leader)
+  uint32_t* sq_w;         // closures queued for the register replay: first slot of the arc
+  uint32_t* sq_a;         //   and its length
+  Counters* ctr;          // zero on entry, zero again on exit
+  Counters* host_out;     // host-visible copy of the final counters
+  uint32_t coop_arc;
+  uint32_t spin;          // polls a grid barrier waits before giving up
+};
+
+__device__ __forceinline__ uint32_t ld_agent(const uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_agent(uint32_t* p, uint32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Grid-wide barrier number `k` (1, 2, ...) of the cooperative launch, with a
+// bounded wait.  Arrivals accumulate in one counter; the workgroup that
+// brings it to k * gridDim.x opens the next generation.  A workgroup that has
+// polled `spin` times without seeing it flags kStatusStall and returns false,
+// so a launch whose workgroups are not all resident ends instead of hanging.
+// Every thread first waits for its own memory operations to complete; the
+// data that other workgroups read after the barrier was written with
+// device-scope atomics, which are complete at that point.
+__device__ inline bool grid_barrier(Counters* ctr, uint32_t k, uint32_t spin) {
+  __shared__ uint32_t passed;
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t ok = 1;
+    const uint32_t gen = ld_agent(&ctr->gen);
+    if (atomicAdd(&ctr->arrive, 1u) == k * gridDim.x - 1) {
+      __hip_atomic_fetch_add(&ctr->gen, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      uint32_t polls = 0;
+      while (ld_agent(&ctr->gen) == gen) {
+        if (++polls > spin) {
+          atomicOr(&ctr->status, kStatusStall);
+          ok = 0;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+    }
+    passed = ok;
+  }
+  __syncthreads();
+  return passed != 0;
+}
+
+__device__ __forceinline__ unsigned long long pack_ev(uint32_t kind, uint64_t slot) {
+  return (static_cast<unsigned long long>(kind) << 32) | slot;
+}
+
+// First event of the sequential insert of key `k` (rayon_cpu.rs:157-262) on
+// the table as it is, decided within kProbeMax positions of `home`: kEvMatch
+// (equal key at position == dist), kEvTermOcc (OCCUPIED with position > dist,
+// the eviction point), kEvFree (EMPTY or TOMBSTONE, the claim), kEvNone (not
+// decided).  Flags are read four at a time.
+template <int KS>
+__device__ inline unsigned long long insert_event(const Table& t, const Blob<KS>& k, uint32_t K, uint64_t home,
+                                                  uint4 first) {
+  uint64_t base = home & ~3ull;
+  uint32_t j0 = static_cast<uint32_t>(home & 3);
+  uint64_t p = 0;
+  while (p < kProbeMax) {
+    const uint4 f4 = p == 0 ? first : *reinterpret_cast<const uint4*>(t.flags + base);
+    const uint32_t fs[4] = {f4.x, f4.y, f4.z, f4.w};
+#pragma unroll
+    for (uint32_t j = 0; j < 4; ++j) {
+      if (j < j0) continue;
+      if (p > t.mask) return pack_ev(kEvNone, 0);
+      const uint32_t f = fs[j];
+      const uint64_t idx = base + j;
+      if ((f & 0xFu) != kOccupied) return pack_ev(kEvFree, idx);
+      const uint64_t d = f >> 4;
+      if (p > d) return pack_ev(kEvTermOcc, idx);
+      if (d == p && k.eq(t.keys + idx * K, K)) return pack_ev(kEvMatch, idx);
+      ++p;
+    }
+    j0 = 0;
+    base = (base + 4) & t.mask;
+  }
+  return pack_ev(kEvNone, 0);
+}
+
+// Members of the closure whose first dry slot is s: dry-claimed slots
+// (P & ~Q) from s up to the barrier, the first slot at/after s set in both.
+__device__ inline uint32_t closure_members(const unsigned long long* P, const unsigned long long* Q, uint64_t s,
+                                           uint64_t cap, unsigned long long pw0, unsigned long long qw0,
+                                           uint64_t* barrier) {
+  const uint64_t words = (cap + 63) >> 6;
+  const uint64_t w0 = s >> 6;
+  uint64_t w = w0;
+  unsigned long long msk = ~0ull << (s & 63);
+  uint32_t m = 0;
+  for (uint64_t it = 0; it <= words; ++it) {
+    if (it == words) msk = (s & 63) ? (~0ull >> (64 - (s & 63))) : 0ull;  // back at w0: bits below s
+    const unsigned long long pw = it == 0 ? pw0 : P[w];
+    const unsigned long long qw = it == 0 ? qw0 : load_word(&Q[w]);
+    const unsigned long long bar = pw & qw & msk;
+    const unsigned long long mem = pw & ~qw & msk;
+    if (bar) {
+      const int b = __ffsll(bar) - 1;
+      m += static_cast<uint32_t>(__popcll(mem & ((1ull << b) - 1ull)));
+      *barrier = (w << 6) + static_cast<uint64_t>(b);
+      return m;
+    }
+    m += static_cast<uint32_t>(__popcll(mem));
+    msk = ~0ull;
+    w = (w + 1 == words) ? 0 : w + 1;
+  }
+  *barrier = cap;
+  return m;
+}
+
+// Settles one dry claim s of a replayed closure: a claimed slot leaves both
+// bitmaps, an unclaimed dry slot gets its Q bit back.
+__device__ inline void settle_slot(const Table& t, unsigned long long* Q, uint64_t s, bool taken) {
+  const unsigned long long bit = 1ull << (s & 63);
+  if (taken) {
+    atomicAnd(&t.freebits[s >> 6], ~bit);
+    atomicAnd(&t.emptybits[s >> 6], ~bit);
+  } else {
+    atomicOr(&Q[s >> 6], bit);
+  }
+  st_agent(&t.owner[s], kNoOwner);
+}
+
+// One-member closure whose key's first event is OCCUPIED with position >
+// dist at slot e: the sequential insert evicts there and carries residents
+// along the occupied run e..s-1 up to the free slot s (its dry slot).  The
+// eviction points follow from the run's flags alone (a carried resident
+// evicts where its position exceeds the slot's dist), and the moves are done
+// from the last to the first, so each destination's resident has moved on
+// already.  Returns false (nothing written) for a run longer than kShiftMax,
+// a run that is not all OCCUPIED, or a carried resident meeting a slot of
+// equal dist (an equal key there is possible only with duplicates in the
+// table); the caller then walks.
+template <int KS, int VS>
+__device__ inline bool shift_insert(const Table& t, const Blob<KS>& k, const Blob<VS>& v, uint64_t h, uint64_t e,
+                                    uint64_t s, uint32_t K, uint32_t V) {
+  const uint64_t L = (s - e) & t.mask;
+  if (L == 0 || L > kShiftMax || t.cap < 8 * kShiftMax) return false;
+  // the run's flags, in aligned 16-byte chunks (cap is a multiple of 4)
+  const uint64_t base = e & ~3ull;
+  const uint32_t o = static_cast<uint32_t>(e & 3);
+  const uint64_t nch = (o + L + 3) >> 2;
+  uint32_t fs[kShiftMax + 4];
+#pragma unroll
+  for (uint32_t c = 0; c < (kShiftMax + 4) / 4; ++c) {
+    uint4 q = make_uint4(0, 0, 0, 0);
+    if (c < nch) q = *reinterpret_cast<const uint4*>(t.flags + ((base + 4 * c) & t.mask));
+    fs[4 * c] = q.x;
+    fs[4 * c + 1] = q.y;
+    fs[4 * c + 2] = q.z;
+    fs[4 * c + 3] = q.w;
+  }
+  uint32_t fl[kShiftMax];
+#pragma unroll
+  for (uint32_t j = 0; j < kShiftMax; ++j)
+    fl[j] = o == 0 ? fs[j] : o == 1 ? fs[j + 1] : o == 2 ? fs[j + 2] : fs[j + 3];
+  uint32_t evm = 1u;  // eviction points (bit j: slot e+j); the key itself evicts at e
+  bool ok = (fl[0] & 0xFu) == kOccupied;
+  uint64_t p = (fl[0] >> 4) + 1;  // position of the resident carried from e
+#pragma unroll
+  for (uint32_t j = 1; j < kShiftMax; ++j) {
+    if (j < L) {
+      const uint64_t d = fl[j] >> 4;
+      ok = ok && (fl[j] & 0xFu) == kOccupied && p != d;
+      if (p > d) {
+        evm |= 1u << j;
+        p = d + 1;
+      } else {
+        ++p;
+      }
+    }
+  }
+  if (!ok) return false;
+  uint64_t dest = L;  // relative slot the next resident (downwards) moves to
+#pragma unroll
+  for (int j = kShiftMax - 1; j >= 0; --j) {
+    if (static_cast<uint64_t>(j) < L && ((evm >> j) & 1u)) {
+      const uint64_t src = (e + j) & t.mask;
+      const uint64_t dst = (e + dest) & t.mask;
+      Blob<KS> rk;
+      Blob<VS> rv;
+      rk.load(t.keys + src * K, K);
+      rv.load(t.vals + src * V, V);
+      rk.store(t.keys + dst * K, K);
+      rv.store(t.vals + dst * V, V);
+      t.flags[dst] = make_flag(kOccupied, (fl[j] >> 4) + (dest - j));
+      dest = j;
+    }
+  }
+  k.store(t.keys + e * K, K);
+  v.store(t.vals + e * V, V);
+  t.flags[e] = make_flag(kOccupied, (e - h) & t.mask);
+  return true;
+}
+
+// Phase C, one-member closure of key i.
+template <int KS, int VS>
+__device__ inline uint32_t fused_single(const Table& t, const FusedInsert& a, uint64_t i, uint32_t K, uint32_t V) {
+  const uint64_t s = a.dslot[i];
+  const unsigned long long e = a.ev[i];
+  const uint32_t kind = static_cast<uint32_t>(e >> 32);
+  const uint64_t j = e & 0xFFFFFFFFull;
+  Blob<VS> v;
+  v.load(a.vals + i * V, V);
+  uint32_t c1 = 0;
+  uint64_t at = s;
+  if (kind == kEvMatch) {
+    v.store(t.vals + j * V, V);  // update in place, not new
+  } else {
+    Blob<KS> k;
+    k.load(a.keys + i * K, K);
+    const uint64_t h = k.hash(K, t.mode) & t.mask;
+    if (kind == kEvFree && j == s) {
+      k.store(t.keys + s * K, K);
+      v.store(t.vals + s * V, V);
+      t.flags[s] = make_flag(kOccupied, (s - h) & t.mask);
+      c1 = 1;
+    } else if (kind == kEvTermOcc && shift_insert<KS, VS>(t, k, v, h, j, s, K, V)) {
+      c1 = 1;
+    } else {
+      const uint64_t p0 = kind == kEvTermOcc ? ((j - h) & t.mask) : 0;
+      c1 = insert_walk<KS, VS, false>(t, k, v, h, p0, K, V, a.ctr, &at);
+    }
+  }
+  if (t.outcome) t.outcome[i] = static_cast<uint8_t>(c1);
+  if (c1 && at != s) {  // a one-member closure claims its own dry slot; kept exact regardless
+    settle_slot(t, a.Q, at, true);
+    atomicAnd(&a.Q[at >> 6], ~(1ull << (at & 63)));
+    settle_slot(t, a.Q, s, false);
+  } else {
+    settle_slot(t, a.Q, s, c1 != 0);
+  }
+  return c1;
+}
+
+// Phase C, closure of 2..kLeaderMax members led by key i, on one lane (`my`
+// is the lane's LDS column).
+template <int KS, int VS>
+__device__ inline uint32_t fused_small(const Table& t, const FusedInsert& a, uint64_t i, uint32_t m, uint32_t* my,
+                                       uint32_t K, uint32_t V) {
+  const unsigned long long* P = t.freebits;
+  const uint64_t s = a.dslot[i];
+  uint64_t pos = s;
+  for (uint32_t j = 0; j < m; ++j) {
+    const uint64_t x = bm_next_set(P, pos, t.cap);
+    my[j * kFusedBlock] = (x == s) ? static_cast<uint32_t>(i) : ld_agent(&t.owner[x]);
+    pos = (x + 1 == t.cap) ? 0 : x + 1;
+  }
+  for (uint32_t u = 1; u < m; ++u) {  // insertion sort by batch index
+    const uint32_t val = my[u * kFusedBlock];
+    uint32_t c = u;
+    while (c > 0 && my[(c - 1) * kFusedBlock] > val) {
+      my[c * kFusedBlock] = my[(c - 1) * kFusedBlock];
+      --c;
+    }
+    my[c * kFusedBlock] = val;
+  }
+  uint32_t claimed = 0;
+  for (uint32_t j = 0; j < m; ++j) {
+    const uint64_t k = my[j * kFusedBlock];
+    const uint32_t c1 = insert_one<KS, VS, false>(t, a.keys + k * K, a.vals + k * V, K, V, a.ctr);
+    claimed += c1;
+    if (t.outcome) t.outcome[k] = static_cast<uint8_t>(c1);
+  }
+  pos = s;
+  for (uint32_t j = 0; j < m; ++j) {  // the closure's dry slots, in slot order
+    const uint64_t x = bm_next_set(P, pos, t.cap);
+    settle_slot(t, a.Q, x, (t.flags[x] & 0xFu) == kOccupied);
+    pos = (x + 1 == t.cap) ? 0 : x + 1;
+  }
+  return claimed;
+}
+
+// Phase C, a closure of several members whose arc (slots w0 .. w0+A-1,
+// A <= 64) is replayed by a whole wave with the arc in registers: lane j
+// holds slot w0+j (flag, key, value) and lane r the r-th member's pair in
+// batch order.  Every member's sequential walk is decided on wave-uniform
+// state (readlane of the slot it reaches) and applied to the lanes' copies;
+// the changed slots are written back once, then the arc's dry claims are
+// settled.  Within the arc every pre-batch free slot is a dry claim (the arc
+// lies between two consecutive barriers), and no walk leaves it.  Called by
+// all 64 lanes with uniform arguments; returns the claims (in every lane).
+template <int KS, int VS>
+__device__ uint32_t wave_replay_arc(const Table& t, const FusedInsert& a, uint64_t w0, uint32_t A) {
+  static_assert(KS % 4 == 0 && VS % 4 == 0 && KS > 0 && VS > 0, "static word sizes");
+  constexpr int KW = KS / 4, VW = VS / 4;
+  const uint32_t lane = threadIdx.x & 63u;
+  const bool in = lane < A;
+  const uint64_t slot = (w0 + lane) & t.mask;
+  uint32_t f = 0, own = kNoOwner;
+  bool member = false;
+  Blob<KS> sk;
+  Blob<VS> sv;
+#pragma unroll
+  for (int w = 0; w < KW; ++w) sk.w[w] = 0;
+#pragma unroll
+  for (int w = 0; w < VW; ++w) sv.w[w] = 0;
+  if (in) {
+    f = t.flags[slot];
+    sk.load(t.keys + slot * KS, KS);
+    sv.load(t.vals + slot * VS, VS);
+    member = ((t.freebits[slot >> 6] >> (slot & 63)) & 1ull) != 0;
+    own = ld_agent(&t.owner[slot]);
+  }
+  const unsigned long long mm = __ballot(member);
+  const uint32_t m = static_cast<uint32_t>(__popcll(mm));
+  // rank of each member by batch index; lane r then fetches member r's pair
+  uint32_t rank = 0;
+  for (unsigned long long rest = mm; rest; rest &= rest - 1) {
+    const int l = __ffsll(rest) - 1;
+    const uint32_t o = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(own), l));
+    rank += (o < own) ? 1u : 0u;
+  }
+  uint32_t myidx = 0;
+  for (unsigned long long rest = mm; rest; rest &= rest - 1) {
+    const int l = __ffsll(rest) - 1;
+    const uint32_t r = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(rank), l));
+    const uint32_t o = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(own), l));
+    if (lane == r) myidx = o;
+  }
+  Blob<KS> mk;
+  Blob<VS> mv;
+#pragma unroll
+  for (int w = 0; w < KW; ++w) mk.w[w] = 0;
+#pragma unroll
+  for (int w = 0; w < VW; ++w) mv.w[w] = 0;
+  if (lane < m) {
+    mk.load(a.keys + static_cast<uint64_t>(myidx) * KS, KS);
+    mv.load(a.vals + static_cast<uint64_t>(myidx) * VS, VS);
+  }
+  const uint32_t mh = lane < m ? static_cast<uint32_t>(mk.hash(KS, t.mode) & t.mask) : 0u;
+  uint32_t sh = in ? static_cast<uint32_t>(sk.hash(KS, t.mode) & t.mask) : 0u;  // slot key's home
+  bool dirty = false, vdirty = false;
+  uint32_t claimed = 0;
+  for (uint32_t r = 0; r < m; ++r) {
+    Blob<KS> ck;
+    Blob<VS> cv;
+#pragma unroll
+    for (int w = 0; w < KW; ++w) ck.w[w] = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(mk.w[w]), r));
+#pragma unroll
+    for (int w = 0; w < VW; ++w) cv.w[w] = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(mv.w[w]), r));
+    uint32_t ch = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(mh), r));
+    auto eq_ballot = [&]() {
+      uint32_t diff = (in && (f & 0xFu) == kOccupied) ? 0u : 1u;
+#pragma unroll
+      for (int w = 0; w < KW; ++w) diff |= sk.w[w] ^ ck.w[w];
+      return static_cast<unsigned long long>(__ballot(diff == 0));
+    };
+    unsigned long long eqm = eq_ballot();
+    uint64_t p = 0;
+    uint32_t j = static_cast<uint32_t>((ch - w0) & t.mask);
+    uint32_t c1 = 0;
+    for (;;) {
+      if (j >= A) {  // no walk leaves the arc; reaching its end means the state is not what was elected
+        if (lane == 0) atomicOr(&a.ctr->status, kStatusBudget);
+        break;
+      }
+      const uint32_t fj = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(f), j));
+      const uint64_t d = fj >> 4;
+      if ((fj & 0xFu) == kOccupied) {
+        if (d == p && ((eqm >> j) & 1ull)) {  // equal key: update its value in place
+          if (lane == j) {
+            sv = cv;
+            vdirty = true;
+          }
+          break;
+        }
+        if (p > d) {  // Robin Hood eviction: the carried pair in, the resident carried on
+          Blob<KS> nk;
+          Blob<VS> nv;
+#pragma unroll
+          for (int w = 0; w < KW; ++w)
+            nk.w[w] = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(sk.w[w]), j));
+#pragma unroll
+          for (int w = 0; w < VW; ++w)
+            nv.w[w] = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(sv.w[w]), j));
+          const uint32_t nh = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(sh), j));
+          if (lane == j) {
+            sk = ck;
+            sv = cv;
+            sh = ch;
+            f = make_flag(kOccupied, p);
+            dirty = true;
+          }
+          ck = nk;
+          cv = nv;
+          ch = nh;
+          p = d + 1;
+          ++j;
+          eqm = eq_ballot();
+          continue;
+        }
+        ++p;
+        ++j;
+        continue;
+      }
+      if (lane == j) {  // EMPTY or TOMBSTONE: claim
+        sk = ck;
+        sv = cv;
+        sh = ch;
+        f = make_flag(kOccupied, p);
+        dirty = true;
+      }
+      c1 = 1;
+      break;
+    }
+    claimed += c1;
+    if (lane == r && t.outcome) t.outcome[myidx] = static_cast<uint8_t>(c1);
+  }
+  if (dirty) {
+    sk.store(t.keys + slot * KS, KS);
+    sv.store(t.vals + slot * VS, VS);
+    t.flags[slot] = f;
+  } else if (vdirty) {
+    sv.store(t.vals + slot * VS, VS);
+  }
+  if (member) settle_slot(t, a.Q, slot, (f & 0xFu) == kOccupied);
+  return claimed;
+}
+
+// Phase C of the fused insert (whole workgroup; `lst` is its LDS list): the
+// workgroup's own keys, then its share of the two queues (entry c goes to
+// wave c mod waves-in-grid).  Returns this thread's share of the claims.
+template <int KS, int VS>
+__device__ inline unsigned long long fused_replay(const Table& t, const FusedInsert& a, uint32_t* lst, uint32_t K,
+                                                  uint32_t V) {
+  const uint64_t n = a.n;
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t gw = blockIdx.x * (kFusedBlock / 64) + (threadIdx.x >> 6);  // wave id in the grid
+  const uint32_t nw = gridDim.x * (kFusedBlock / 64);
+  uint32_t* my = lst + threadIdx.x;
+  unsigned long long claimed = 0;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const uint32_t code = a.code[i];
+    const uint32_t m = code & 0xFFu;
+    if (code < kCodeQueued && m == 1) claimed += fused_single<KS, VS>(t, a, i, K, V);
+    else if (code < kCodeQueued && m >= 2) claimed += fused_small<KS, VS>(t, a, i, m, my, K, V);
+  }
+  if constexpr (KS > 0 && VS > 0 && KS % 4 == 0 && VS % 4 == 0) {
+    const uint32_t nsmall = ld_agent(&a.ctr->nsmall);
+    uint32_t got = 0;
+    for (uint32_t c = gw; c < nsmall; c += nw) got += wave_replay_arc<KS, VS>(t, a, ld_agent(&a.sq_w[c]), ld_agent(&a.sq_a[c]));
+    if (lane == 0) claimed += got;
+  }
+  const uint32_t nbig = ld_agent(&a.ctr->nbig);
+  if (nbig) {
+    // Queued large closures, one wave each; lst is split into 2048 words per wave.
+    __syncthreads();
+    uint32_t* region = lst + (threadIdx.x >> 6) * (kLeaderMax * kFusedBlock / (kFusedBlock / 64));
+    uint32_t* carry = region;     // 2 * kMaxGenericSize / 4 = 128 words
+    uint32_t* cnt = region + 128;
+    uint32_t* wl = region + 256;  // kWaveLst words
+    for (uint32_t c = gw; c < nbig; c += nw) {
+      const uint32_t got =
+          replay_arc<KS, VS, true>(t, a.keys, a.vals, n, a.Q, ld_agent(&a.big[c]), wl, kWaveLst, carry, cnt, a.ctr);
+      if (lane == 0) claimed += got;
+    }
+  }
+  return claimed;
+}
+
+template <int KS, int VS>
+__global__ void __launch_bounds__(kFusedBlock)
+k_ins_fused(Table t, FusedInsert a) {
+  __shared__ uint32_t lst[kLeaderMax * kFusedBlock];
+  const uint32_t K = KS ? KS : t.K;
+  const uint32_t V = VS ? VS : t.V;
+  const uint64_t n = a.n;
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  const uint64_t tid = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const unsigned long long* P = t.freebits;
+  const uint64_t budget = ((t.cap + 63) >> 6) + n + 2;
+
+  // A: dry claim and pre-batch event.
+  for (uint64_t i = tid; i < n; i += stride) {
+    Blob<KS> k;
+    k.load(a.keys + i * K, K);
+    const uint64_t h = k.hash(K, t.mode) & t.mask;
+    const uint4 f0 = *reinterpret_cast<const uint4*>(t.flags + (h & ~3ull));  // overlaps the claim
+    uint64_t s;
+    if (bm_claim(a.Q, h, t.cap, budget, &s)) {
+      a.dslot[i] = static_cast<uint32_t>(s);
+      st_agent(&t.owner[s], static_cast<uint32_t>(i));
+    } else {
+      atomicOr(&a.ctr->status, kStatusDryFail);
+      a.dslot[i] = kNoSlot;
+    }
+    a.ev[i] = insert_event<KS>(t, k, K, h, f0);
+  }
+
+  // No workgroup reaches phase C, the only one that writes the table, unless
+  // every workgroup has passed both barriers.
+  if (!grid_barrier(a.ctr, 1, a.spin)) {
+    // stalled: nothing applied (the host restores owner[] and Q)
+  } else if (ld_agent(&a.ctr->status) & kStatusDryFail) {
+    // nothing applied; undo the owner[] entries
+    for (uint64_t i = tid; i < n; i += stride) {
+      const uint32_t s = a.dslot[i];
+      if (s != kNoSlot) st_agent(&t.owner[s], kNoOwner);
+    }
+  } else {
+    // B: election.
+    // Queue slots are reserved per workgroup (one atomic each per round).
+    constexpr bool kRegs = KS > 0 && VS > 0 && KS % 4 == 0 && VS % 4 == 0;
+    __shared__ uint32_t q_cnt[2], q_base[2];
+    for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * blockDim.x; base < n; base += stride) {
+      const uint64_t i = base + threadIdx.x;
+      if (threadIdx.x < 2) q_cnt[threadIdx.x] = 0;
+      __syncthreads();
+      uint32_t code = 0, want = 0, local = 0;  // want: 1 register replay, 2 wave replay
+      uint64_t b = 0, arc = 0, s = 0;
+      if (i < n) {
+        s = a.dslot[i];
+        const unsigned long long pw = P[s >> 6], qw = load_word(&a.Q[s >> 6]);
+        const unsigned long long below = pw & ((1ull << (s & 63)) - 1ull);
+        b = below ? ((s & ~63ull) + static_cast<uint64_t>(63 - __clzll(below))) : bm_prev_set(P, s, t.cap);
+        const unsigned long long qb = (b >> 6) == (s >> 6) ? qw : load_word(&a.Q[b >> 6]);
+        if (b != s && ((qb >> (b & 63)) & 1ull)) {
+          uint64_t x;
+          const uint32_t m = closure_members(P, a.Q, s, t.cap, pw, qw, &x);
+          arc = (x - b - 1) & t.mask;
+          if (m > kLeaderMax || arc > a.coop_arc) {
+            want = 2;
+            code = kCodeBig;
+          } else if (kRegs && m >= 2 && arc <= 64 && t.cap > 128) {
+            want = 1;
+            code = kCodeQueued;
+          } else {
+            code = m;
+          }
+        }
+        a.code[i] = code;
+        if (want) local = atomicAdd(&q_cnt[want - 1], 1u);
+      }
+      __syncthreads();
+      if (threadIdx.x < 2 && q_cnt[threadIdx.x])
+        q_base[threadIdx.x] = atomicAdd(threadIdx.x == 0 ? &a.ctr->nsmall : &a.ctr->nbig, q_cnt[threadIdx.x]);
+      __syncthreads();
+      if (want == 1) {
+        st_agent(&a.sq_w[q_base[0] + local], static_cast<uint32_t>((b + 1) & t.mask));
+        st_agent(&a.sq_a[q_base[0] + local], static_cast<uint32_t>(arc));
+      } else if (want == 2) {
+        st_agent(&a.big[q_base[1] + local], static_cast<uint32_t>(s));
+      }
+    }
+    if (grid_barrier(a.ctr, 2, a.spin)) block_add(&a.ctr->count, fused_replay<KS, VS>(t, a, lst, K, V));
+  }
+
+  // The last workgroup out publishes the counters and zeroes them.
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  if (threadIdx.x == 0 && atomicAdd(&a.ctr->done, 1u) == gridDim.x - 1) {
+    Counters out{};
+    out.status = ld_agent(&a.ctr->status);
+    out.nbig = ld_agent(&a.ctr->nbig);
+    out.count = __hip_atomic_load(&a.ctr->count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    *a.host_out = out;
+    *a.ctr = Counters{};
+    __threadfence_system();
   }
 }
 
@@ -850,6 +1558,7 @@ static __global__ void __launch_bounds__(1024)
 k_rem_apply(Table t, const unsigned long long* __restrict__ ev, const uint8_t* __restrict__ action, uint64_t n,
             Counters* ctr) {
   const bool conflict = (ctr->status & kStatusConflict) != 0;
+  unsigned long long total = 0;
   for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * blockDim.x;; base += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
     if (base >= n) break;
     const uint64_t i = base + threadIdx.x;
@@ -859,13 +1568,15 @@ k_rem_apply(Table t, const unsigned long long* __restrict__ ev, const uint8_t* _
       if (!conflict && action[i]) {
         t.flags[slot] = kTombstone;  // raw 2: distance bits cleared
         atomicOr(&t.freebits[slot >> 6], 1ull << (slot & 63));
+        if (t.freecopy) atomicOr(&t.freecopy[slot >> 6], 1ull << (slot & 63));
         removed = 1;
       }
       t.owner[slot] = kNoOwner;
     }
     if (t.outcome && i < n && !conflict) t.outcome[i] = static_cast<uint8_t>(removed);
-    wave_add(&ctr->count, removed);
+    total += removed;
   }
+  block_add(&ctr->count, total);
 }
 
 // Exact replay, step 1: run id = first EMPTY slot at/after home (removes
@@ -902,6 +1613,7 @@ __device__ inline uint32_t remove_one(const Table& t, const uint8_t* __restrict_
       if (d == p && q.eq(t.keys + idx * K, K)) {
         t.flags[idx] = kTombstone;
         atomicOr(&t.freebits[idx >> 6], 1ull << (idx & 63));
+        if (t.freecopy) atomicOr(&t.freecopy[idx >> 6], 1ull << (idx & 63));
         return 1;
       }
     }
@@ -915,10 +1627,10 @@ __global__ void __launch_bounds__(1024)
 k_rem_replay(Table t, const uint8_t* __restrict__ qkeys, uint64_t n, const uint32_t* __restrict__ cid_s,
              const uint32_t* __restrict__ idx_s, Counters* ctr) {
   const uint32_t K = KS ? KS : t.K;
+  unsigned long long removed = 0;
   for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * blockDim.x;; base += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
     if (base >= n) break;
     const uint64_t s = base + threadIdx.x;
-    uint32_t removed = 0;
     if (s < n && (s == 0 || cid_s[s] != cid_s[s - 1])) {
       const uint32_t c = cid_s[s];
       for (uint64_t u = s; u < n && cid_s[u] == c; ++u) {
@@ -927,8 +1639,8 @@ k_rem_replay(Table t, const uint8_t* __restrict__ qkeys, uint64_t n, const uint3
         if (t.outcome) t.outcome[idx_s[u]] = static_cast<uint8_t>(r);
       }
     }
-    wave_add(&ctr->count, removed);
   }
+  block_add(&ctr->count, removed);
 }
 
 // ---------------------------------------------------------------------------
@@ -974,13 +1686,13 @@ static __global__ void k_rebuild_bits(Table t) {
 
 // recount (kernels.cu:328-337): OCCUPIED slots, one atomic per wave.
 static __global__ void k_count(Table t, Counters* ctr) {
+  unsigned long long occ = 0;
   for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * blockDim.x;; base += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
     if (base >= t.cap) break;
     const uint64_t i = base + threadIdx.x;
-    const bool occ = i < t.cap && (t.flags[i] & 0xFu) == kOccupied;
-    const unsigned long long b = __ballot(occ);
-    if ((threadIdx.x & 63u) == 0 && b) atomicAdd(&ctr->count, static_cast<unsigned long long>(__popcll(b)));
+    occ += (i < t.cap && (t.flags[i] & 0xFu) == kOccupied) ? 1u : 0u;
   }
+  block_add(&ctr->count, occ);
 }
 
 }  // namespace dev

@@ -117,6 +117,7 @@ class Tuning(ctypes.Structure):
         ("remove_group", ctypes.c_uint32),
         ("insert_block", ctypes.c_uint32),
         ("insert_coop_arc", ctypes.c_uint32),
+        ("insert_fused", ctypes.c_uint32),
     ]
 
 
@@ -449,11 +450,11 @@ class FlashMap:
         return t
 
     def set_tuning(self, get_group=None, get_block=None, remove_group=None, insert_block=None,
-                   insert_coop_arc=None):
+                   insert_coop_arc=None, insert_fused=None):
         t = self.tuning()
         for name, val in (("get_group", get_group), ("get_block", get_block),
                           ("remove_group", remove_group), ("insert_block", insert_block),
-                          ("insert_coop_arc", insert_coop_arc)):
+                          ("insert_coop_arc", insert_coop_arc), ("insert_fused", insert_fused)):
             if val is not None:
                 setattr(t, name, val)
         _check(lib().fm_set_tuning(self._h, ctypes.byref(t)))

@@ -210,6 +210,9 @@ typedef struct fm_tuning {
   uint32_t insert_block;
   uint32_t insert_coop_arc; /* closures whose barrier-to-barrier arc is longer
                                are walked by a whole wave (0: all of them) */
+  uint32_t insert_fused;    /* 1: batches that cannot fill the table run as one
+                               cooperative launch with grid-wide barriers;
+                               0 (default): as a sequence of kernels */
 } fm_tuning;
 fm_status fm_get_tuning(const fm_map* m, fm_tuning* out);
 fm_status fm_set_tuning(fm_map* m, const fm_tuning* t);
