@@ -356,6 +356,7 @@ class GpuBackend final : public Backend {
   size_t sort_tmp_bytes_ = 0;
   unsigned long long* Q_ = nullptr;
   Counters* ctr_ = nullptr;
+  dev::GridBarrier* bar_ = nullptr;  // the fused insert's barrier words (zero between launches)
   Counters* h_ctr_ = nullptr;  // pinned
 
   // per-kernel event timing
@@ -430,6 +431,7 @@ void GpuBackend::release() {
   sort_tmp_ = nullptr;
   dfree(Q_);
   dfree(ctr_);
+  dfree(bar_);
   if (h_ctr_) (void)hipHostFree(h_ctr_);
   h_ctr_ = nullptr;
   dfree(st_keys_);
@@ -495,6 +497,8 @@ fm_status GpuBackend::init(const fm_builder& b, std::string* err) {
       (e = hipMalloc(&t_.emptybits, words * 8)) != hipSuccess ||
       (e = hipMalloc(&t_.owner, cap * sizeof(uint32_t))) != hipSuccess ||
       (e = hipMalloc(&ctr_, sizeof(Counters))) != hipSuccess ||
+      (e = hipMalloc(&bar_, sizeof(dev::GridBarrier))) != hipSuccess ||
+      (e = hipMemset(bar_, 0, sizeof(dev::GridBarrier))) != hipSuccess ||
       (e = hipHostMalloc(&h_ctr_, sizeof(Counters), hipHostMallocDefault)) != hipSuccess) {
     *err = hip_msg("hipMalloc (table)", e);
     return FM_ERR_GPU_ALLOC;
@@ -638,8 +642,8 @@ fm_status GpuBackend::run_insert(const uint8_t* d_keys, const uint8_t* d_vals, u
       hipError_t e = hipMemsetAsync(ctr_, 0, sizeof(Counters), stream_);
       if (e != hipSuccess) return fail(FM_ERR_KERNEL_LAUNCH, hip_msg("hipMemsetAsync", e));
     }
-    dev::FusedInsert a{d_keys, d_vals, n,      Q_,     dslot_, cid_,  ev_, big_, cid_s_, idx_s_, ctr_, h_ctr_,
-                       tune_.insert_coop_arc, fused_spin_};
+    dev::FusedInsert a{d_keys, d_vals, n,      Q_,     dslot_, cid_,  ev_,   big_,
+                       cid_s_, idx_s_, ctr_,   bar_,   h_ctr_, tune_.insert_coop_arc, fused_spin_};
     Prof p(this, kKidInsFused);
     if ((st = check_launch(ops_.ins_fused(t, a, fused_blocks_, stream_), "k_ins_fused")) != FM_OK) {
       q_synced_ = ctr_zero_ = ctr_in_host_ = false;

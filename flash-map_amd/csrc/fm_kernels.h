@@ -85,11 +85,16 @@ struct Counters {
   uint32_t nbig;  // closures queued for the wave replay
   unsigned long long count;
   uint32_t done;      // fused insert: workgroups finished
-  uint32_t big_next;  // fused insert: next queued closure to replay
-  uint32_t arrive;    // fused insert: grid barrier arrivals
-  uint32_t gen;       // fused insert: grid barrier generation
   uint32_t nsmall;    // fused insert: closures queued for the register replay
-  uint32_t small_next;
+};
+
+// Grid barrier words of the fused insert, each on a cache line of its own
+// (every workgroup polls `gen` while the others arrive).
+struct GridBarrier {
+  uint32_t arrive;
+  uint32_t pad0[31];
+  uint32_t gen;
+  uint32_t pad1[31];
 };
 
 __device__ __forceinline__ uint32_t make_flag(uint32_t state, uint64_t dist) {
@@ -146,6 +151,12 @@ struct Blob {
     for (int i = 0; i < W; ++i) diff |= w[i] ^ o.w[i];
     return diff == 0;
   }
+  __device__ __forceinline__ bool eqb(const Blob<N>& o) const {
+    uint32_t diff = 0;
+#pragma unroll
+    for (int i = 0; i < W; ++i) diff |= w[i] ^ o.w[i];
+    return diff == 0;
+  }
   __device__ __forceinline__ uint64_t hash(uint32_t, uint32_t mode) const {
     if (mode == 0) {
       if constexpr (N >= 8) return static_cast<uint64_t>(w[0]) | (static_cast<uint64_t>(w[1]) << 32);
@@ -167,6 +178,11 @@ struct Blob<0> {
   __device__ __forceinline__ bool eq(const uint8_t* __restrict__ p, uint32_t n) const {
     for (uint32_t i = 0; i < n; ++i)
       if (b[i] != p[i]) return false;
+    return true;
+  }
+  __device__ __forceinline__ bool eqb(const Blob<0>& o) const {
+    for (uint32_t i = 0; i < kMaxGenericSize; ++i)
+      if (b[i] != o.b[i]) return false;
     return true;
   }
   __device__ __forceinline__ uint64_t hash(uint32_t n, uint32_t mode) const { return hash_key(b, n, mode); }
@@ -923,6 +939,7 @@ constexpr uint64_t kProbeMax = 64;          // positions phase A decides an even
 constexpr uint32_t kFusedBlock = 256;
 constexpr uint32_t kWaveLst = 1024;         // closure members one wave sorts at a time
 constexpr uint32_t kShiftMax = 16;          // longest run a one-member shift handles
+constexpr uint32_t kShiftChecks = 4;        // same-home key comparisons a shift plan makes
 
 struct FusedInsert {
   const uint8_t* keys;
@@ -936,6 +953,7 @@ struct FusedInsert {
   uint32_t* sq_w;         // closures queued for the register replay: first slot of the arc
   uint32_t* sq_a;         //   and its length
   Counters* ctr;          // zero on entry, zero again on exit
+  GridBarrier* bar;       // zero on entry, zero again on exit
   Counters* host_out;     // host-visible copy of the final counters
   uint32_t coop_arc;
   uint32_t spin;          // polls a grid barrier waits before giving up
@@ -956,24 +974,24 @@ __device__ __forceinline__ void st_agent(uint32_t* p, uint32_t v) {
 // Every thread first waits for its own memory operations to complete; the
 // data that other workgroups read after the barrier was written with
 // device-scope atomics, which are complete at that point.
-__device__ inline bool grid_barrier(Counters* ctr, uint32_t k, uint32_t spin) {
+__device__ inline bool grid_barrier(GridBarrier* bar, Counters* ctr, uint32_t k, uint32_t spin) {
   __shared__ uint32_t passed;
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   if (threadIdx.x == 0) {
     uint32_t ok = 1;
-    const uint32_t gen = ld_agent(&ctr->gen);
-    if (atomicAdd(&ctr->arrive, 1u) == k * gridDim.x - 1) {
-      __hip_atomic_fetch_add(&ctr->gen, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t gen = ld_agent(&bar->gen);
+    if (atomicAdd(&bar->arrive, 1u) == k * gridDim.x - 1) {
+      __hip_atomic_fetch_add(&bar->gen, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     } else {
       uint32_t polls = 0;
-      while (ld_agent(&ctr->gen) == gen) {
+      while (ld_agent(&bar->gen) == gen) {
         if (++polls > spin) {
           atomicOr(&ctr->status, kStatusStall);
           ok = 0;
           break;
         }
-        __builtin_amdgcn_s_sleep(1);
+        __builtin_amdgcn_s_sleep(4);
       }
     }
     passed = ok;
@@ -993,7 +1011,7 @@ __device__ __forceinline__ unsigned long long pack_ev(uint32_t kind, uint64_t sl
 // decided).  Flags are read four at a time.
 template <int KS>
 __device__ inline unsigned long long insert_event(const Table& t, const Blob<KS>& k, uint32_t K, uint64_t home,
-                                                  uint4 first) {
+                                                  uint4 first, uint64_t cslot, const Blob<KS>& ckey) {
   uint64_t base = home & ~3ull;
   uint32_t j0 = static_cast<uint32_t>(home & 3);
   uint64_t p = 0;
@@ -1009,7 +1027,7 @@ __device__ inline unsigned long long insert_event(const Table& t, const Blob<KS>
       if ((f & 0xFu) != kOccupied) return pack_ev(kEvFree, idx);
       const uint64_t d = f >> 4;
       if (p > d) return pack_ev(kEvTermOcc, idx);
-      if (d == p && k.eq(t.keys + idx * K, K)) return pack_ev(kEvMatch, idx);
+      if (d == p && (idx == cslot ? k.eqb(ckey) : k.eq(t.keys + idx * K, K))) return pack_ev(kEvMatch, idx);
       ++p;
     }
     j0 = 0;
@@ -1071,12 +1089,18 @@ __device__ inline void settle_slot(const Table& t, unsigned long long* Q, uint64
 // a run that is not all OCCUPIED, or a carried resident meeting a slot of
 // equal dist (an equal key there is possible only with duplicates in the
 // table); the caller then walks.
-template <int KS, int VS>
-__device__ inline bool shift_insert(const Table& t, const Blob<KS>& k, const Blob<VS>& v, uint64_t h, uint64_t e,
-                                    uint64_t s, uint32_t K, uint32_t V) {
-  const uint64_t L = (s - e) & t.mask;
-  if (L == 0 || L > kShiftMax || t.cap < 8 * kShiftMax) return false;
-  // the run's flags, in aligned 16-byte chunks (cap is a multiple of 4)
+// Plans the shift: reads the run's flags (fl[j] = slot e+j, in aligned
+// 16-byte chunks; cap is a multiple of 4) and returns the eviction points
+// (bit j: slot e+j), or 0 when the run cannot be shifted this way.  Where a
+// carried resident reaches a slot whose dist equals its position (a resident
+// with the same home) the walk compares keys.  Up to kShiftChecks such
+// comparisons are made on the keys' first 8 bytes (or the whole key when it
+// is shorter), loaded together after the flags; equal prefixes, or more
+// comparisons, return 0 and leave the exact comparison to the walk.  (Equal
+// keys there are possible only with duplicate keys in the table.)
+template <int KS>
+__device__ __forceinline__ uint32_t shift_plan(const Table& t, uint64_t e, uint64_t L, uint32_t (&fl)[kShiftMax]) {
+  if (L == 0 || L > kShiftMax || t.cap < 8 * kShiftMax) return 0;
   const uint64_t base = e & ~3ull;
   const uint32_t o = static_cast<uint32_t>(e & 3);
   const uint64_t nch = (o + L + 3) >> 2;
@@ -1090,27 +1114,61 @@ __device__ inline bool shift_insert(const Table& t, const Blob<KS>& k, const Blo
     fs[4 * c + 2] = q.z;
     fs[4 * c + 3] = q.w;
   }
-  uint32_t fl[kShiftMax];
 #pragma unroll
   for (uint32_t j = 0; j < kShiftMax; ++j)
     fl[j] = o == 0 ? fs[j] : o == 1 ? fs[j + 1] : o == 2 ? fs[j + 2] : fs[j + 3];
-  uint32_t evm = 1u;  // eviction points (bit j: slot e+j); the key itself evicts at e
+  uint32_t evm = 1u;  // the key itself evicts at e
   bool ok = (fl[0] & 0xFu) == kOccupied;
   uint64_t p = (fl[0] >> 4) + 1;  // position of the resident carried from e
+  uint32_t carrier = 0;           // slot (relative) the carried resident came from
+  uint32_t chk = 0, nchk = 0;     // comparisons: byte c = carrier | checked slot << 4
 #pragma unroll
   for (uint32_t j = 1; j < kShiftMax; ++j) {
     if (j < L) {
       const uint64_t d = fl[j] >> 4;
-      ok = ok && (fl[j] & 0xFu) == kOccupied && p != d;
+      ok = ok && (fl[j] & 0xFu) == kOccupied;
+      if (p == d) {
+        if (nchk < kShiftChecks) chk |= (carrier | (j << 4)) << (8 * nchk);
+        ++nchk;
+      }
       if (p > d) {
         evm |= 1u << j;
         p = d + 1;
+        carrier = j;
       } else {
         ++p;
       }
     }
   }
-  if (!ok) return false;
+  if (!ok || nchk > kShiftChecks) return 0u;
+  if constexpr (KS >= 8 && KS % 8 == 0) {
+    uint2 x[kShiftChecks], y[kShiftChecks];
+#pragma unroll
+    for (uint32_t c = 0; c < kShiftChecks; ++c) {
+      if (c < nchk) {
+        const uint32_t b = (chk >> (8 * c)) & 0xFFu;
+        x[c] = *reinterpret_cast<const uint2*>(t.keys + ((e + (b & 0xFu)) & t.mask) * KS);
+        y[c] = *reinterpret_cast<const uint2*>(t.keys + ((e + (b >> 4)) & t.mask) * KS);
+      }
+    }
+    bool same = false;
+#pragma unroll
+    for (uint32_t c = 0; c < kShiftChecks; ++c)
+      if (c < nchk) same = same || (x[c].x == y[c].x && x[c].y == y[c].y);
+    if (same) return 0u;
+  } else {
+    if (nchk) return 0u;
+  }
+  return evm;
+}
+
+template <int KS, int VS>
+__device__ inline bool shift_insert(const Table& t, const Blob<KS>& k, const Blob<VS>& v, uint64_t h, uint64_t e,
+                                    uint64_t s, uint32_t K, uint32_t V) {
+  const uint64_t L = (s - e) & t.mask;
+  uint32_t fl[kShiftMax];
+  const uint32_t evm = shift_plan<KS>(t, e, L, fl);
+  if (!evm) return false;
   uint64_t dest = L;  // relative slot the next resident (downwards) moves to
 #pragma unroll
   for (int j = kShiftMax - 1; j >= 0; --j) {
@@ -1131,6 +1189,17 @@ __device__ inline bool shift_insert(const Table& t, const Blob<KS>& k, const Blo
   v.store(t.vals + e * V, V);
   t.flags[e] = make_flag(kOccupied, (e - h) & t.mask);
   return true;
+}
+
+// Whether the one-member closure of a key with phase-A event `ev` and dry
+// slot s is applied directly (update in place, claim of s, or a shift of at
+// most kShiftMax slots) rather than walked.
+__device__ __forceinline__ bool single_is_direct(const Table& t, unsigned long long ev, uint64_t s) {
+  const uint32_t kind = static_cast<uint32_t>(ev >> 32);
+  const uint64_t j = ev & 0xFFFFFFFFull;
+  if (kind == kEvMatch) return true;
+  if (kind == kEvFree) return j == s;
+  return kind == kEvTermOcc && ((s - j) & t.mask) <= kShiftMax && t.cap >= 8 * kShiftMax;
 }
 
 // Phase C, one-member closure of key i.
@@ -1169,6 +1238,134 @@ __device__ inline uint32_t fused_single(const Table& t, const FusedInsert& a, ui
     settle_slot(t, a.Q, s, false);
   } else {
     settle_slot(t, a.Q, s, c1 != 0);
+  }
+  return c1;
+}
+
+constexpr uint32_t kFromBatch = 0x80000000u;  // move source: a batch pair, not a slot
+constexpr uint32_t kWaveMoves = 2048 / 3;      // moves one wave's share of lst holds
+
+// Phase C, the one-member closures of one wave (static sizes; every lane
+// calls, `mine` marks the lanes that hold one, key i).  Updates in place and
+// claims of the dry slot are written directly.  The eviction shifts of all
+// lanes go into one move list in the wave's LDS share `ml`: each lane lists
+// its run's moves from the last eviction point down (the resident carried
+// furthest first, the batch pair last), so a move's destination has always
+// been read by an earlier move of the list; the list is then applied 64
+// moves at a time, every lane loading its pair before any lane stores.  A
+// lane whose moves do not fit shifts on its own; an event that needs a walk
+// is walked.  Returns the lane's claim.
+template <int KS, int VS>
+__device__ inline uint32_t fused_singles_wave(const Table& t, const FusedInsert& a, uint64_t i, uint32_t* ml) {
+  const uint32_t lane = threadIdx.x & 63u;
+  uint64_t s = 0, j = 0, h = 0, L = 0;
+  uint32_t kind = kEvNone, c1 = 0, evm = 0;
+  Blob<KS> k;
+  Blob<VS> v;
+  uint32_t fl[kShiftMax];
+  // Everything a one-member closure may need is loaded at once, for every
+  // key of the wave (the election code decides afterwards).
+  uint32_t code = 0;
+  unsigned long long e = 0;
+  if (i < a.n) {
+    code = a.code[i];
+    s = a.dslot[i];
+    e = a.ev[i];
+    v.load(a.vals + i * VS, VS);
+    k.load(a.keys + i * KS, KS);
+  }
+  const bool mine = i < a.n && code < kCodeQueued && (code & 0xFFu) == 1u;
+  bool done = !mine;
+  if (mine) {
+    kind = static_cast<uint32_t>(e >> 32);
+    j = e & 0xFFFFFFFFull;
+    if (kind == kEvMatch) {
+      v.store(t.vals + j * VS, VS);  // update in place, not new
+      done = true;
+    } else {
+      h = k.hash(KS, t.mode) & t.mask;
+      if (kind == kEvFree && j == s) {
+        k.store(t.keys + s * KS, KS);
+        v.store(t.vals + s * VS, VS);
+        t.flags[s] = make_flag(kOccupied, (s - h) & t.mask);
+        c1 = 1;
+        done = true;
+      } else if (kind == kEvTermOcc) {
+        L = (s - j) & t.mask;
+        evm = shift_plan<KS>(t, j, L, fl);
+      }
+    }
+  }
+  // moves of this lane: one per eviction point, plus the batch pair
+  const uint32_t nmov = (!done && evm) ? static_cast<uint32_t>(__popc(evm)) + 1u : 0u;
+  uint32_t incl = nmov;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t y = static_cast<uint32_t>(__shfl_up(static_cast<int>(incl), off, 64));
+    if (lane >= static_cast<uint32_t>(off)) incl += y;
+  }
+  const uint32_t first = incl - nmov;
+  const bool listed = nmov != 0 && incl <= kWaveMoves;
+  if (listed) {
+    uint64_t dest = L;
+    uint32_t w = first;
+#pragma unroll
+    for (int jj = kShiftMax - 1; jj >= 0; --jj) {
+      if (static_cast<uint64_t>(jj) < L && ((evm >> jj) & 1u)) {
+        ml[3 * w] = static_cast<uint32_t>((j + jj) & t.mask);
+        ml[3 * w + 1] = static_cast<uint32_t>((j + dest) & t.mask);
+        ml[3 * w + 2] = make_flag(kOccupied, (fl[jj] >> 4) + (dest - jj));
+        ++w;
+        dest = jj;
+      }
+    }
+    ml[3 * w] = kFromBatch | static_cast<uint32_t>(i);
+    ml[3 * w + 1] = static_cast<uint32_t>(j);
+    ml[3 * w + 2] = make_flag(kOccupied, (j - h) & t.mask);
+    c1 = 1;
+    done = true;
+  }
+  uint32_t used = listed ? incl : 0u;  // listed lanes are a prefix of the wave
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) used = max(used, static_cast<uint32_t>(__shfl_xor(static_cast<int>(used), off, 64)));
+  wave_sync();
+  for (uint32_t base = 0; base < used; base += 64) {
+    const uint32_t e = base + lane;
+    if (e < used) {
+      const uint32_t src = ml[3 * e], dst = ml[3 * e + 1], fg = ml[3 * e + 2];
+      Blob<KS> mk;
+      Blob<VS> mv;
+      if (src & kFromBatch) {
+        const uint64_t bi = src & ~kFromBatch;
+        mk.load(a.keys + bi * KS, KS);
+        mv.load(a.vals + bi * VS, VS);
+      } else {
+        mk.load(t.keys + static_cast<uint64_t>(src) * KS, KS);
+        mv.load(t.vals + static_cast<uint64_t>(src) * VS, VS);
+      }
+      mk.store(t.keys + static_cast<uint64_t>(dst) * KS, KS);
+      mv.store(t.vals + static_cast<uint64_t>(dst) * VS, VS);
+      t.flags[dst] = fg;
+    }
+  }
+  uint64_t at = s;
+  if (!done) {
+    if (evm && shift_insert<KS, VS>(t, k, v, h, j, s, KS, VS)) {
+      c1 = 1;
+    } else {
+      const uint64_t p0 = kind == kEvTermOcc ? ((j - h) & t.mask) : 0;
+      c1 = insert_walk<KS, VS, false>(t, k, v, h, p0, KS, VS, a.ctr, &at);
+    }
+  }
+  if (mine) {
+    if (t.outcome) t.outcome[i] = static_cast<uint8_t>(c1);
+    if (c1 && at != s) {  // a one-member closure claims its own dry slot; kept exact regardless
+      settle_slot(t, a.Q, at, true);
+      atomicAnd(&a.Q[at >> 6], ~(1ull << (at & 63)));
+      settle_slot(t, a.Q, s, false);
+    } else {
+      settle_slot(t, a.Q, s, c1 != 0);
+    }
   }
   return c1;
 }
@@ -1291,10 +1488,19 @@ __device__ uint32_t wave_replay_arc(const Table& t, const FusedInsert& a, uint64
     uint32_t j = static_cast<uint32_t>((ch - w0) & t.mask);
     uint32_t c1 = 0;
     for (;;) {
-      if (j >= A) {  // no walk leaves the arc; reaching its end means the state is not what was elected
+      // Jump to the next slot where something happens: the carried pair is at
+      // position p + (l - j) when it reaches lane l; it stops at a free slot
+      // or where dist <= that position (eviction, or an equal-dist resident
+      // whose key is compared).  Slots before it are passed over.
+      const bool stop = in && lane >= j && ((f & 0xFu) != kOccupied || (f >> 4) <= p + (lane - j));
+      const unsigned long long sm = __ballot(stop);
+      if (!sm) {  // no walk leaves the arc; reaching its end means the state is not what was elected
         if (lane == 0) atomicOr(&a.ctr->status, kStatusBudget);
         break;
       }
+      const uint32_t nj = static_cast<uint32_t>(__ffsll(sm) - 1);
+      p += nj - j;
+      j = nj;
       const uint32_t fj = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(f), j));
       const uint64_t d = fj >> 4;
       if ((fj & 0xFu) == kOccupied) {
@@ -1371,16 +1577,31 @@ __device__ inline unsigned long long fused_replay(const Table& t, const FusedIns
   const uint32_t nw = gridDim.x * (kFusedBlock / 64);
   uint32_t* my = lst + threadIdx.x;
   unsigned long long claimed = 0;
-  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
-    const uint32_t code = a.code[i];
-    const uint32_t m = code & 0xFFu;
-    if (code < kCodeQueued && m == 1) claimed += fused_single<KS, VS>(t, a, i, K, V);
-    else if (code < kCodeQueued && m >= 2) claimed += fused_small<KS, VS>(t, a, i, m, my, K, V);
+  // this wave's first register-replay entry, fetched ahead of the lane work
+  const uint32_t nsmall = ld_agent(&a.ctr->nsmall);
+  uint32_t w_first = 0, a_first = 0;
+  if (gw < nsmall) {
+    w_first = ld_agent(&a.sq_w[gw]);
+    a_first = ld_agent(&a.sq_a[gw]);
+  }
+  for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * blockDim.x; base < n; base += stride) {
+    const uint64_t i = base + threadIdx.x;
+    if constexpr (KS > 0 && VS > 0 && KS % 4 == 0 && VS % 4 == 0) {
+      // m >= 2 closures were all queued in phase B
+      claimed += fused_singles_wave<KS, VS>(t, a, i,
+                                            lst + (threadIdx.x >> 6) * (kLeaderMax * kFusedBlock / (kFusedBlock / 64)));
+    } else {
+      const uint32_t code = i < n ? a.code[i] : 0u;
+      const uint32_t m = code & 0xFFu;
+      if (i < n && code < kCodeQueued && m == 1) claimed += fused_single<KS, VS>(t, a, i, K, V);
+      else if (i < n && code < kCodeQueued && m >= 2) claimed += fused_small<KS, VS>(t, a, i, m, my, K, V);
+    }
   }
   if constexpr (KS > 0 && VS > 0 && KS % 4 == 0 && VS % 4 == 0) {
-    const uint32_t nsmall = ld_agent(&a.ctr->nsmall);
     uint32_t got = 0;
-    for (uint32_t c = gw; c < nsmall; c += nw) got += wave_replay_arc<KS, VS>(t, a, ld_agent(&a.sq_w[c]), ld_agent(&a.sq_a[c]));
+    for (uint32_t c = gw; c < nsmall; c += nw)
+      got += wave_replay_arc<KS, VS>(t, a, c == gw ? w_first : ld_agent(&a.sq_w[c]),
+                                     c == gw ? a_first : ld_agent(&a.sq_a[c]));
     if (lane == 0) claimed += got;
   }
   const uint32_t nbig = ld_agent(&a.ctr->nbig);
@@ -1418,6 +1639,29 @@ k_ins_fused(Table t, FusedInsert a) {
     k.load(a.keys + i * K, K);
     const uint64_t h = k.hash(K, t.mode) & t.mask;
     const uint4 f0 = *reinterpret_cast<const uint4*>(t.flags + (h & ~3ull));  // overlaps the claim
+    // The first slot of that chunk whose dist equals its probe position is the
+    // likely match: its key is loaded before the claim too.
+    uint64_t cslot = ~0ull;
+    Blob<KS> ckey;
+    if constexpr (KS > 0) {
+      const uint32_t fs0[4] = {f0.x, f0.y, f0.z, f0.w};
+      const uint32_t j0 = static_cast<uint32_t>(h & 3);
+      bool stop = false;
+#pragma unroll
+      for (uint32_t jj = 0; jj < 4; ++jj) {
+        if (jj >= j0 && !stop) {
+          const uint32_t f = fs0[jj];
+          const uint32_t pp = jj - j0;
+          if ((f & 0xFu) != kOccupied || pp > (f >> 4)) {
+            stop = true;
+          } else if ((f >> 4) == pp) {
+            cslot = (h & ~3ull) + jj;
+            stop = true;
+          }
+        }
+      }
+      if (cslot != ~0ull) ckey.load(t.keys + cslot * K, K);
+    }
     uint64_t s;
     if (bm_claim(a.Q, h, t.cap, budget, &s)) {
       a.dslot[i] = static_cast<uint32_t>(s);
@@ -1426,12 +1670,12 @@ k_ins_fused(Table t, FusedInsert a) {
       atomicOr(&a.ctr->status, kStatusDryFail);
       a.dslot[i] = kNoSlot;
     }
-    a.ev[i] = insert_event<KS>(t, k, K, h, f0);
+    a.ev[i] = insert_event<KS>(t, k, K, h, f0, cslot, ckey);
   }
 
   // No workgroup reaches phase C, the only one that writes the table, unless
   // every workgroup has passed both barriers.
-  if (!grid_barrier(a.ctr, 1, a.spin)) {
+  if (!grid_barrier(a.bar, a.ctr, 1, a.spin)) {
     // stalled: nothing applied (the host restores owner[] and Q)
   } else if (ld_agent(&a.ctr->status) & kStatusDryFail) {
     // nothing applied; undo the owner[] entries
@@ -1460,10 +1704,16 @@ k_ins_fused(Table t, FusedInsert a) {
           uint64_t x;
           const uint32_t m = closure_members(P, a.Q, s, t.cap, pw, qw, &x);
           arc = (x - b - 1) & t.mask;
-          if (m > kLeaderMax || arc > a.coop_arc) {
+          // Static sizes: anything a lane cannot apply directly goes to a
+          // wave (register replay for arcs of up to 64 slots, else the queue
+          // of large closures); a walk is left on a lane only for small
+          // tables or one-member arcs longer than 64 slots.
+          const bool direct = m == 1 && single_is_direct(t, a.ev[i], s);
+          const bool regs_ok = kRegs && arc <= 64 && t.cap > 128;
+          if (m > kLeaderMax || arc > a.coop_arc || (kRegs && m >= 2 && !regs_ok)) {
             want = 2;
             code = kCodeBig;
-          } else if (kRegs && m >= 2 && arc <= 64 && t.cap > 128) {
+          } else if (!direct && regs_ok) {
             want = 1;
             code = kCodeQueued;
           } else {
@@ -1484,7 +1734,7 @@ k_ins_fused(Table t, FusedInsert a) {
         st_agent(&a.big[q_base[1] + local], static_cast<uint32_t>(s));
       }
     }
-    if (grid_barrier(a.ctr, 2, a.spin)) block_add(&a.ctr->count, fused_replay<KS, VS>(t, a, lst, K, V));
+    if (grid_barrier(a.bar, a.ctr, 2, a.spin)) block_add(&a.ctr->count, fused_replay<KS, VS>(t, a, lst, K, V));
   }
 
   // The last workgroup out publishes the counters and zeroes them.
@@ -1497,6 +1747,8 @@ k_ins_fused(Table t, FusedInsert a) {
     out.count = __hip_atomic_load(&a.ctr->count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     *a.host_out = out;
     *a.ctr = Counters{};
+    a.bar->arrive = 0;
+    a.bar->gen = 0;
     __threadfence_system();
   }
 }

@@ -306,3 +306,26 @@ def test_gpu_fused_barrier_timeout_leaves_table(monkeypatch):
         assert_same_slots(m, o, "fused insert whose barriers all met")
     apply_and_compare(m, o, [("ins", rows[2000:6000], rows[2000:6000]), ("rem", rows[:1000]),
                              ("ins", rows[5000:], rows[:3000]), ("get", rows)])
+
+
+@pytest.mark.parametrize("fused", [1, 0])
+def test_gpu_shift_past_duplicate_key(fused):
+    """A shift whose carried resident meets its own duplicate (left by the
+    tombstone anomaly): the sequential insert updates the duplicate's value
+    and claims nothing.  Identity hash, home = key mod 1024."""
+    cap, h = 1024, 500
+    say(f"shift past duplicate fused={fused}")
+
+    def key(home, tag):
+        return np.array([home + cap * tag], dtype=np.uint64).view(np.uint8).reshape(1, 8)
+
+    W, X, Y, Z = key(h - 1, 3), key(h, 1), key(h, 2), key(h - 1, 4)
+    v = lambda x: np.full((1, 8), x, dtype=np.uint8)  # noqa: E731
+    ops = [("ins", np.concatenate([W, X, Y]), np.concatenate([v(1), v(2), v(3)])),
+           ("rem", X),                      # tombstone at h
+           ("ins", Y, v(4)),                # claims the tombstone: Y at h and at h+1
+           ("ins", Z, v(5)),                # evicts Y at h; the carried Y meets its duplicate
+           ("get", np.concatenate([W, X, Y, Z]))]
+    m = gpu_make(cap, 8, 8)
+    m.set_tuning(insert_fused=fused)
+    apply_and_compare(m, OracleAdapter(cap, 8, 8), ops)
