@@ -45,6 +45,7 @@ struct Ops {
   hipError_t (*rem_replay)(const Table&, const uint8_t*, uint64_t, const uint32_t*, const uint32_t*, Counters*,
                            uint32_t, hipStream_t);
   hipError_t (*ins_fused)(const Table&, const dev::FusedInsert&, uint32_t, hipStream_t);
+  hipError_t (*ins_dedup)(const Table&, const uint8_t*, uint64_t, uint32_t, hipStream_t);
   hipError_t (*ins_fused_blocks)(int, uint32_t*);
 };
 
@@ -76,6 +77,12 @@ template <int KS>
 hipError_t launch_ins_dry(const Table& t, const uint8_t* k, uint64_t n, unsigned long long* Q, uint32_t* home,
                           uint32_t* dslot, Counters* c, uint32_t block, hipStream_t s) {
   hipLaunchKernelGGL((dev::k_ins_dry<KS>), dim3(grid_for(n, block)), dim3(block), 0, s, t, k, n, Q, home, dslot, c);
+  return hipGetLastError();
+}
+
+template <int KS>
+hipError_t launch_ins_dedup(const Table& t, const uint8_t* k, uint64_t n, uint32_t block, hipStream_t s) {
+  hipLaunchKernelGGL((dev::k_ins_dedup<KS>), dim3(grid_for(n, block)), dim3(block), 0, s, t, k, n);
   return hipGetLastError();
 }
 
@@ -178,6 +185,7 @@ Ops make_ops() {
   o.rem_runs = &launch_rem_runs<KS>;
   o.rem_replay = &launch_rem_replay<KS>;
   o.ins_fused = &launch_ins_fused<KS, VS>;
+  o.ins_dedup = &launch_ins_dedup<KS>;
   o.ins_fused_blocks = &ins_fused_blocks<KS, VS>;
   return o;
 }
@@ -257,6 +265,7 @@ class GpuBackend final : public Backend {
     return FM_OK;
   }
   void serve_settle_insert(uint64_t n, uint64_t claimed) override { len_ -= std::min(len_, n - claimed); }
+  void serve_mode(bool on) override { serving_ = on; }
   fm_status serve_remove(const void* d_keys, uint64_t n, uint8_t* d_removed, uint64_t* removed) override {
     fm_status st;
     if ((st = use_device()) != FM_OK) return st;
@@ -330,7 +339,15 @@ class GpuBackend final : public Backend {
   Table t_{};
   uint64_t len_ = 0;
   Ops ops_{};
-  fm_tuning tune_{4, 256, 4, 256, 64, 0};
+  fm_tuning tune_{4, 256, 4, 256, 64, 0, 0};
+  bool serving_ = false;  // inside fm_serve_log: inserts deduplicate as with insert_dedup
+  // insert deduplication buckets (allocated on first use)
+  uint64_t dd_s_ = 0;
+  uint32_t* dd_first_ = nullptr;
+  uint32_t* dd_last_ = nullptr;
+  uint32_t* dd_bucket_ = nullptr;
+  uint64_t dd_n_ = 0;
+  fm_status ensure_dedup(uint64_t n);
   bool tune_sort_only_ = false;  // FM_INSERT_SORT_PATH=1 forces the sort-grouped insert
   enum class InsertPath { kSort, kLed, kFused };
   InsertPath last_insert_ = InsertPath::kSort;
@@ -363,7 +380,7 @@ class GpuBackend final : public Backend {
   enum {
     kKidGet, kKidInsCopy, kKidInsDry, kKidInsClosure, kKidInsSort, kKidInsReplay, kKidRemProbe, kKidRemCheck,
     kKidRemApply, kKidRemRuns, kKidRemSort, kKidRemReplay, kKidCopyTable, kKidInsLeader, kKidInsBig, kKidInsBits,
-    kKidInsFused, kNumKid
+    kKidInsFused, kKidInsDedup, kNumKid
   };
   struct PendingTiming {
     int kid;
@@ -432,6 +449,9 @@ void GpuBackend::release() {
   dfree(Q_);
   dfree(ctr_);
   dfree(bar_);
+  dfree(dd_first_);
+  dfree(dd_last_);
+  dfree(dd_bucket_);
   if (h_ctr_) (void)hipHostFree(h_ctr_);
   h_ctr_ = nullptr;
   dfree(st_keys_);
@@ -523,9 +543,10 @@ fm_status GpuBackend::set_tuning(const fm_tuning* t) {
   auto pow2 = [](uint32_t g) { return g >= 1 && g <= 16 && (g & (g - 1)) == 0; };
   auto blk = [](uint32_t b) { return b >= 64 && b <= 1024 && b % 64 == 0; };
   if (!t || !pow2(t->get_group) || !pow2(t->remove_group) || !blk(t->get_block) || !blk(t->insert_block) ||
-      t->insert_fused > 1)
+      t->insert_fused > 1 || t->insert_dedup > 1)
     return fail(FM_ERR_INVALID_ARGUMENT,
-                "tuning: group must be 1,2,4,8,16; block a multiple of 64 in 64..1024; insert_fused 0 or 1");
+                "tuning: group must be 1,2,4,8,16; block a multiple of 64 in 64..1024; insert_fused and "
+                "insert_dedup 0 or 1");
   tune_ = *t;
   return FM_OK;
 }
@@ -619,6 +640,37 @@ fm_status GpuBackend::run_get(const uint8_t* d_keys, uint64_t n, uint8_t* d_out,
   return check_launch(ops_.get(t_, d_keys, n, d_out, d_found, tune_.get_group, tune_.get_block, stream_), "k_get");
 }
 
+// Buckets for insert deduplication: 2^k >= 2n entries, first = kNoOwner and
+// last = 0 between ops (k_ins_bits / k_dedup_reset restore the used ones).
+fm_status GpuBackend::ensure_dedup(uint64_t n) {
+  uint64_t want = 1024;
+  while (want < 2 * n) want <<= 1;
+  hipError_t e = hipSuccess;
+  if (want > dd_s_) {
+    (void)hipStreamSynchronize(stream_);
+    dfree(dd_first_);
+    dfree(dd_last_);
+    dd_s_ = 0;
+    if ((e = hipMalloc(&dd_first_, want * 4)) != hipSuccess || (e = hipMalloc(&dd_last_, want * 4)) != hipSuccess)
+      return fail(FM_ERR_GPU_ALLOC, hip_msg("hipMalloc (dedup)", e));
+    hipLaunchKernelGGL(dev::k_fill_u32, dim3(grid_for(want, 256)), dim3(256), 0, stream_, dd_first_, want,
+                       dev::kNoOwner);
+    hipLaunchKernelGGL(dev::k_fill_u32, dim3(grid_for(want, 256)), dim3(256), 0, stream_, dd_last_, want, 0u);
+    if ((e = hipGetLastError()) != hipSuccess) return fail(FM_ERR_KERNEL_LAUNCH, hip_msg("dedup init", e));
+    dd_s_ = want;
+  }
+  if (n > dd_n_) {
+    (void)hipStreamSynchronize(stream_);
+    dfree(dd_bucket_);
+    dd_n_ = 0;
+    const uint64_t m = std::max<uint64_t>(n, 1024);
+    if ((e = hipMalloc(&dd_bucket_, m * 4)) != hipSuccess)
+      return fail(FM_ERR_GPU_ALLOC, hip_msg("hipMalloc (dedup)", e));
+    dd_n_ = m;
+  }
+  return FM_OK;
+}
+
 fm_status GpuBackend::run_insert(const uint8_t* d_keys, const uint8_t* d_vals, uint64_t n, uint8_t* outcome) {
   fm_status st;
   Table t = t_;
@@ -626,12 +678,20 @@ fm_status GpuBackend::run_insert(const uint8_t* d_keys, const uint8_t* d_vals, u
   if ((st = ensure_scratch(n)) != FM_OK) return st;
   const uint64_t words = (t_.cap + 63) / 64;
   const uint32_t blk = tune_.insert_block;
+  const bool dedup = (tune_.insert_dedup || serving_) && n > 1;
+  if (dedup) {
+    if ((st = ensure_dedup(n)) != FM_OK) return st;
+    t.dd_first = dd_first_;
+    t.dd_last = dd_last_;
+    t.dd_bucket = dd_bucket_;
+    t.dd_smask = dd_s_ - 1;
+  }
   // With len + n < cap the dry pass provably leaves a free slot (len never
   // undercounts occupied slots), so closures are bounded by barriers and are
   // led without sorting.  Otherwise (the batch may fill the table) group by
   // closure id with a stable radix sort, which also covers "no barrier".
   const bool led = len_ + n < t_.cap && !tune_sort_only_;
-  if (led && tune_.insert_fused && fused_blocks_) {
+  if (led && tune_.insert_fused && fused_blocks_ && !dedup) {
     last_insert_ = InsertPath::kFused;
     if (!q_synced_) {
       Prof p(this, kKidInsCopy);
@@ -661,11 +721,15 @@ fm_status GpuBackend::run_insert(const uint8_t* d_keys, const uint8_t* d_vals, u
     hipError_t e = hipMemcpyAsync(Q_, t_.freebits, words * 8, hipMemcpyDeviceToDevice, stream_);
     if (e != hipSuccess) return fail(FM_ERR_TRANSFER, hip_msg("bitmap copy", e));
   }
+  if (dedup) {
+    Prof p(this, kKidInsDedup);
+    if ((st = check_launch(ops_.ins_dedup(t, d_keys, n, blk, stream_), "k_ins_dedup")) != FM_OK) return st;
+  }
   last_insert_ = led ? InsertPath::kLed : InsertPath::kSort;
   if (led) {
     {
       Prof p(this, kKidInsDry);
-      if ((st = check_launch(ops_.ins_dry(t_, d_keys, n, Q_, nullptr, dslot_, ctr_, blk, stream_), "k_ins_dry")) !=
+      if ((st = check_launch(ops_.ins_dry(t, d_keys, n, Q_, nullptr, dslot_, ctr_, blk, stream_), "k_ins_dry")) !=
           FM_OK)
         return st;
     }
@@ -682,12 +746,12 @@ fm_status GpuBackend::run_insert(const uint8_t* d_keys, const uint8_t* d_vals, u
         return st;
     }
     Prof p(this, kKidInsBits);
-    hipLaunchKernelGGL(dev::k_ins_bits, dim3(grid_for(n, blk)), dim3(blk), 0, stream_, t_, dslot_, n, ctr_);
+    hipLaunchKernelGGL(dev::k_ins_bits, dim3(grid_for(n, blk)), dim3(blk), 0, stream_, t, dslot_, n, ctr_);
     return check_launch(hipGetLastError(), "k_ins_bits");
   }
   {
     Prof p(this, kKidInsDry);
-    if ((st = check_launch(ops_.ins_dry(t_, d_keys, n, Q_, home_, nullptr, ctr_, blk, stream_), "k_ins_dry")) != FM_OK)
+    if ((st = check_launch(ops_.ins_dry(t, d_keys, n, Q_, home_, nullptr, ctr_, blk, stream_), "k_ins_dry")) != FM_OK)
       return st;
   }
   {
@@ -703,8 +767,15 @@ fm_status GpuBackend::run_insert(const uint8_t* d_keys, const uint8_t* d_vals, u
                                                       0, sort_end_bit_, stream_);
     if (e != hipSuccess) return fail(FM_ERR_KERNEL_LAUNCH, hip_msg("radix sort", e));
   }
-  Prof p(this, kKidInsReplay);
-  return check_launch(ops_.ins_replay(t, d_keys, d_vals, n, cid_s_, idx_s_, ctr_, blk, stream_), "k_ins_replay");
+  {
+    Prof p(this, kKidInsReplay);
+    if ((st = check_launch(ops_.ins_replay(t, d_keys, d_vals, n, cid_s_, idx_s_, ctr_, blk, stream_),
+                           "k_ins_replay")) != FM_OK)
+      return st;
+  }
+  if (!dedup) return FM_OK;
+  hipLaunchKernelGGL(dev::k_dedup_reset, dim3(grid_for(n, blk)), dim3(blk), 0, stream_, t, n);
+  return check_launch(hipGetLastError(), "k_dedup_reset");
 }
 
 fm_status GpuBackend::run_remove(const uint8_t* d_keys, uint64_t n, uint64_t* removed, uint8_t* outcome) {
@@ -768,7 +839,7 @@ const char* GpuBackend::kid_name(int k) {
                                        "ins_radix_sort", "k_ins_replay",  "k_rem_probe",  "k_rem_check",
                                        "k_rem_apply",  "k_rem_runs",      "rem_radix_sort", "k_rem_replay",
                                        "copy_table",   "k_ins_leader",    "k_ins_big",    "k_ins_bits",
-                                       "k_ins_fused"};
+                                       "k_ins_fused",  "k_ins_dedup"};
   return names[k];
 }
 

@@ -78,7 +78,36 @@ struct Table {
   // Per-op: when set, a copy of freebits that removes keep equal to it (the
   // fused insert's claim bitmap, so the next insert needs no copy).
   unsigned long long* freecopy;
+  // Per-op insert deduplication (null when off): keys are bucketed by home;
+  // dd_first[b] = smallest batch index in bucket b, dd_last[b] = largest
+  // index of a later copy of that key; dd_bucket[i] = bucket of key i, with
+  // kDupKey set when key i is a later copy of its bucket's first key.
+  uint32_t* dd_first;
+  uint32_t* dd_last;
+  uint32_t* dd_bucket;
+  uint64_t dd_smask;
 };
+
+constexpr uint32_t kDupKey = 0x80000000u;
+constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
+
+// Insert deduplication.  With no removes inside a batch, every later copy of
+// a key only overwrites the value that its first copy inserted or matched
+// (the first copy leaves no free slot between the key's home and the key, and
+// later inserts keep it reachable), so the batch is applied exactly as its
+// first copies, each carrying the value of its key's last copy; later copies
+// claim nothing.  Only copies of each bucket's first key are folded (keys
+// that merely share a bucket stay separate batch entries).
+__device__ __forceinline__ bool dd_is_copy(const Table& t, uint64_t i) {
+  return t.dd_first && (t.dd_bucket[i] & kDupKey);
+}
+__device__ __forceinline__ uint64_t dd_value_index(const Table& t, uint64_t i) {
+  if (!t.dd_first) return i;
+  const uint32_t b = t.dd_bucket[i] & ~kDupKey;
+  if (t.dd_first[b] != i) return i;
+  const uint64_t last = t.dd_last[b];
+  return last > i ? last : i;
+}
 
 struct Counters {
   uint32_t status;
@@ -373,6 +402,32 @@ k_get(Table t, const uint8_t* __restrict__ qkeys, uint64_t n, uint8_t* __restric
 // Pass 1: hash every key and dry-claim one free bit of Q (a copy of
 // freebits) at/after its home.  With `dslot` given, the claimed slot is
 // recorded per key and the key's index is written to owner[slot].
+// Deduplication pass 1: each key's bucket, and the first index per bucket.
+template <int KS>
+__global__ void __launch_bounds__(1024)
+k_ins_dedup(Table t, const uint8_t* __restrict__ in_keys, uint64_t n) {
+  const uint32_t K = KS ? KS : t.K;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    Blob<KS> k;
+    k.load(in_keys + i * K, K);
+    const uint32_t b = static_cast<uint32_t>((k.hash(K, t.mode) & t.mask) & t.dd_smask);
+    t.dd_bucket[i] = b;
+    atomicMin(&t.dd_first[b], static_cast<uint32_t>(i));
+  }
+}
+
+// Deduplication reset (sort path; the leader path resets in k_ins_bits).
+static __global__ void __launch_bounds__(1024)
+k_dedup_reset(Table t, uint64_t n) {
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    const uint32_t b = t.dd_bucket[i] & ~kDupKey;
+    t.dd_first[b] = kNoOwner;
+    t.dd_last[b] = 0;
+  }
+}
+
 template <int KS>
 __global__ void __launch_bounds__(1024)
 k_ins_dry(Table t, const uint8_t* __restrict__ in_keys, uint64_t n, unsigned long long* __restrict__ Q,
@@ -385,6 +440,17 @@ k_ins_dry(Table t, const uint8_t* __restrict__ in_keys, uint64_t n, unsigned lon
     k.load(in_keys + i * K, K);
     const uint64_t h = k.hash(K, t.mode) & t.mask;
     if (home_out) home_out[i] = static_cast<uint32_t>(h);
+    if (t.dd_first) {  // a later copy of its bucket's first key claims nothing
+      const uint32_t b = t.dd_bucket[i];
+      const uint32_t f = t.dd_first[b];
+      if (f != i && k.eq(in_keys + static_cast<uint64_t>(f) * K, K)) {
+        atomicMax(&t.dd_last[b], static_cast<uint32_t>(i));
+        t.dd_bucket[i] = b | kDupKey;
+        if (dslot) dslot[i] = kNoSlot;
+        if (t.outcome) t.outcome[i] = 0;
+        continue;
+      }
+    }
     uint64_t s;
     if (!bm_claim(Q, h, t.cap, budget, &s)) {
       atomicOr(&ctr->status, kStatusDryFail);
@@ -505,7 +571,8 @@ k_ins_replay(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __rest
       const uint32_t c = cid_s[s];
       for (uint64_t u = s; u < n && cid_s[u] == c; ++u) {
         const uint64_t i = idx_s[u];
-        const uint32_t c1 = insert_one<KS, VS>(t, in_keys + i * K, in_vals + i * V, K, V, ctr);
+        if (dd_is_copy(t, i)) continue;
+        const uint32_t c1 = insert_one<KS, VS>(t, in_keys + i * K, in_vals + dd_value_index(t, i) * V, K, V, ctr);
         claimed += c1;
         if (t.outcome) t.outcome[i] = static_cast<uint8_t>(c1);
       }
@@ -708,7 +775,7 @@ k_ins_leader(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __rest
   for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * blockDim.x;; base += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
     if (base >= n) break;
     const uint64_t i = base + threadIdx.x;
-    if (i < n) {
+    if (i < n && dslot[i] != kNoSlot) {
       const uint64_t s = dslot[i];
       const uint64_t b = bm_prev_set(P, s, t.cap);
       if (b != s && bit_at(Q, b)) {
@@ -743,7 +810,8 @@ k_ins_leader(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __rest
           }
           for (uint32_t j = 0; j < m; ++j) {
             const uint64_t k = my[j * 256];
-            const uint32_t c1 = insert_one<KS, VS, false>(t, in_keys + k * K, in_vals + k * V, K, V, ctr);
+            const uint32_t c1 =
+                insert_one<KS, VS, false>(t, in_keys + k * K, in_vals + dd_value_index(t, k) * V, K, V, ctr);
             claimed += c1;
             if (t.outcome) t.outcome[k] = static_cast<uint8_t>(c1);
           }
@@ -828,7 +896,7 @@ __device__ uint32_t replay_arc(const Table& t, const uint8_t* __restrict__ in_ke
     if constexpr (KS > 0 && VS > 0 && KS % 4 == 0 && VS % 4 == 0) {
       for (uint32_t j = 0; j < m; ++j) {
         const uint64_t k = lst[j];
-        const uint32_t c1 = insert_one_wave<KS, VS>(t, in_keys + k * K, in_vals + k * V, carry, ctr);
+        const uint32_t c1 = insert_one_wave<KS, VS>(t, in_keys + k * K, in_vals + dd_value_index(t, k) * V, carry, ctr);
         claimed += c1;
         if (lane == 0 && t.outcome) t.outcome[k] = static_cast<uint8_t>(c1);
       }
@@ -837,7 +905,8 @@ __device__ uint32_t replay_arc(const Table& t, const uint8_t* __restrict__ in_ke
       if (lane == 0) {
         for (uint32_t j = 0; j < m; ++j) {
           const uint64_t k = lst[j];
-          const uint32_t c1 = insert_one<KS, VS, false>(t, in_keys + k * K, in_vals + k * V, K, V, ctr);
+          const uint32_t c1 =
+              insert_one<KS, VS, false>(t, in_keys + k * K, in_vals + dd_value_index(t, k) * V, K, V, ctr);
           c0 += c1;
           if (t.outcome) t.outcome[k] = static_cast<uint8_t>(c1);
         }
@@ -894,10 +963,17 @@ k_ins_big(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __restric
 // bitmaps and resets owner[] for the next op.
 static __global__ void __launch_bounds__(1024)
 k_ins_bits(Table t, const uint32_t* __restrict__ dslot, uint64_t n, const Counters* ctr) {
-  if (ctr->status & kStatusDryFail) return;
+  const bool failed = (ctr->status & kStatusDryFail) != 0;
   for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
        i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    if (t.dd_first) {
+      const uint32_t b = t.dd_bucket[i] & ~kDupKey;
+      t.dd_first[b] = kNoOwner;
+      t.dd_last[b] = 0;
+    }
+    if (failed) continue;
     const uint64_t s = dslot[i];
+    if (s == kNoSlot) continue;
     if ((t.flags[s] & 0xFu) == kOccupied) {
       const unsigned long long bit = 1ull << (s & 63);
       atomicAnd(&t.freebits[s >> 6], ~bit);
@@ -931,7 +1007,6 @@ k_ins_bits(Table t, const uint32_t* __restrict__ dslot, uint64_t n, const Counte
 // for the next launch.  256-thread workgroups, launched cooperatively (all
 // resident) and at most two per CU; the barriers' waits are bounded.
 // ---------------------------------------------------------------------------
-constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
 constexpr uint32_t kCodeBig = 0xFFFFFFFFu;  // election code: queued for a wave
 constexpr uint32_t kCodeQueued = 0xFFFFFFFEu;  // election code: queued for the register replay
 constexpr uint32_t kEvFree = 4u;            // insert event: EMPTY or TOMBSTONE (claim)

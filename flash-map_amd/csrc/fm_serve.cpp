@@ -351,6 +351,11 @@ void Server::completion_loop() {
 fm_status Server::run(fm_serve_stats* stats) {
   fm_status st = setup();
   if (st != FM_OK) return st;
+  struct ServeMode {  // hot keys are expected: inserts deduplicate while serving
+    Backend& be;
+    explicit ServeMode(Backend& b) : be(b) { be.serve_mode(true); }
+    ~ServeMode() { be.serve_mode(false); }
+  } mode(*m_->be);
   t0_ = Clock::now();
   std::thread arrivals(&Server::arrival_loop, this);
   std::thread completions(&Server::completion_loop, this);

@@ -118,6 +118,7 @@ class Tuning(ctypes.Structure):
         ("insert_block", ctypes.c_uint32),
         ("insert_coop_arc", ctypes.c_uint32),
         ("insert_fused", ctypes.c_uint32),
+        ("insert_dedup", ctypes.c_uint32),
     ]
 
 
@@ -450,11 +451,12 @@ class FlashMap:
         return t
 
     def set_tuning(self, get_group=None, get_block=None, remove_group=None, insert_block=None,
-                   insert_coop_arc=None, insert_fused=None):
+                   insert_coop_arc=None, insert_fused=None, insert_dedup=None):
         t = self.tuning()
         for name, val in (("get_group", get_group), ("get_block", get_block),
                           ("remove_group", remove_group), ("insert_block", insert_block),
-                          ("insert_coop_arc", insert_coop_arc), ("insert_fused", insert_fused)):
+                          ("insert_coop_arc", insert_coop_arc), ("insert_fused", insert_fused),
+                          ("insert_dedup", insert_dedup)):
             if val is not None:
                 setattr(t, name, val)
         _check(lib().fm_set_tuning(self._h, ctypes.byref(t)))

@@ -213,6 +213,10 @@ typedef struct fm_tuning {
   uint32_t insert_fused;    /* 1: batches that cannot fill the table run as one
                                cooperative launch with grid-wide barriers;
                                0 (default): as a sequence of kernels */
+  uint32_t insert_dedup;    /* 1: fold later copies of a key within an insert
+                               batch into its first copy before claiming slots
+                               (exact; pays off for batches with hot keys;
+                               always on inside fm_serve_log); 0 (default) */
 } fm_tuning;
 fm_status fm_get_tuning(const fm_map* m, fm_tuning* out);
 fm_status fm_set_tuning(fm_map* m, const fm_tuning* t);

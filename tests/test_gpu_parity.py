@@ -329,3 +329,69 @@ def test_gpu_shift_past_duplicate_key(fused):
     m = gpu_make(cap, 8, 8)
     m.set_tuning(insert_fused=fused)
     apply_and_compare(m, OracleAdapter(cap, 8, 8), ops)
+
+
+def _hot_batch(rng, pool, n, K):
+    """n keys drawn with heavy repetition from `pool` (rows), skewed."""
+    ranks = np.minimum((rng.pareto(0.8, size=n) * 3).astype(np.int64), len(pool) - 1)
+    return pool[ranks]
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_gpu_insert_dedup_hot_keys(seed):
+    """insert_dedup = 1: batches with many copies of hot keys (new and
+    existing, with tombstones in the table) fold later copies into the first
+    one; counts, gets, removes and the slot array must equal the oracle's."""
+    rng = np.random.default_rng(5000 + seed)
+    cap = [1 << 12, 1 << 14, 1 << 16, 256][seed % 4]
+    K, V = [(8, 8), (32, 8), (32, 128), (16, 16)][seed % 4]
+    say(f"insert dedup seed={seed} cap={cap} K={K} V={V}")
+    pool = rng.integers(0, 256, size=(max(8, cap // 16), K), dtype=np.uint8)
+    m = gpu_make(cap, K, V, seed % 2)
+    m.set_tuning(insert_dedup=1, insert_coop_arc=[64, 0][seed % 2])
+    ops = []
+    for r in range(8):
+        n = int(rng.integers(1, cap // 3))
+        ks = _hot_batch(rng, pool, n, K)
+        ops.append(("ins", ks, rng.integers(0, 256, size=(n, V), dtype=np.uint8)))
+        if r % 3 == 1:
+            ops.append(("rem", _hot_batch(rng, pool, max(1, n // 4), K)))
+        ops.append(("get", pool))
+    apply_and_compare(m, OracleAdapter(cap, K, V, seed % 2), ops)
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_gpu_insert_dedup_fill_to_full(seed):
+    """Deduplicated batches on the sort-grouped path (the batch could fill
+    the table), including a batch that fills it exactly."""
+    rng = np.random.default_rng(5100 + seed)
+    cap = 512
+    say(f"insert dedup fill seed={seed}")
+    ks = rng.choice(1 << 30, size=cap + 200, replace=False).astype(np.uint64).view(np.uint8).reshape(-1, 8)
+    m, o = gpu_make(cap, 8, 8, seed % 2), OracleAdapter(cap, 8, 8, seed % 2)
+    m.set_tuning(insert_dedup=1)
+    batch = np.concatenate([ks[:cap - 100], ks[:50], ks[10:60]])
+    ops = [("ins", ks[:100], ks[:100]), ("rem", ks[:20]),
+           ("ins", batch, rng.integers(0, 256, size=(len(batch), 8), dtype=np.uint8)),
+           ("get", ks)]
+    apply_and_compare(m, o, ops)
+
+
+def test_gpu_serve_hot_insert_batch_matches_oracle():
+    """fm_serve_log turns deduplication on: a served hot-key insert batch and
+    gets after it agree with the oracle."""
+    rng = np.random.default_rng(5200)
+    cap, K, V = 1 << 16, 32, 8
+    say("serve hot insert batch")
+    pool = rng.integers(0, 256, size=(400, K), dtype=np.uint8)
+    m, o = gpu_make(cap, K, V), OracleAdapter(cap, K, V)
+    ins = _hot_batch(rng, pool, 20_000, K)
+    vals = rng.integers(0, 256, size=(len(ins), V), dtype=np.uint8)
+    from flashmap_amd.workloads import Request
+
+    reqs = [Request("insert", 0.0, ins, vals), Request("get", 0.0, pool)]
+    results, _ = fm.serve_log(m, reqs, staging_bytes=1 << 26, max_batch_keys=1 << 20, realtime=False)
+    assert results[0].count == o.bulk_insert(ins, vals)
+    ev, ef = o.bulk_get(pool)
+    assert np.array_equal(results[1].values, ev) and np.array_equal(results[1].found.astype(bool), ef)
+    assert_same_slots(m, o, "after served hot inserts")
