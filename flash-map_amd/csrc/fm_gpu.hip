@@ -266,6 +266,11 @@ class GpuBackend final : public Backend {
   }
   void serve_settle_insert(uint64_t n, uint64_t claimed) override { len_ -= std::min(len_, n - claimed); }
   void serve_mode(bool on) override { serving_ = on; }
+  fm_status serve_reserve(uint64_t max_keys) override {
+    fm_status st;
+    if ((st = use_device()) != FM_OK || (st = ensure_scratch(max_keys)) != FM_OK) return st;
+    return ensure_dedup(max_keys);
+  }
   fm_status serve_remove(const void* d_keys, uint64_t n, uint8_t* d_removed, uint64_t* removed) override {
     fm_status st;
     if ((st = use_device()) != FM_OK) return st;

@@ -60,6 +60,7 @@ class Backend {
   virtual fm_status serve_insert(const void*, const void*, uint64_t, uint8_t*) { return gpu_required(); }
   virtual void serve_settle_insert(uint64_t, uint64_t) {}
   virtual void serve_mode(bool) {}  // inside fm_serve_log: inserts deduplicate hot keys
+  virtual fm_status serve_reserve(uint64_t /*max_keys*/) { return FM_OK; }  // per-batch scratch up front
   virtual fm_status serve_remove(const void*, uint64_t, uint8_t*, uint64_t*) { return gpu_required(); }
   virtual fm_status profile_enable(bool) { return gpu_required(); }
   virtual fm_status profile_read(fm_kernel_stat*, uint32_t, uint32_t*) { return gpu_required(); }

@@ -78,6 +78,89 @@ class Ring {
   std::deque<uint64_t> live_;
 };
 
+// Host copy workers.  Request buffers are ordinary (pageable) memory: one
+// thread moves them into and out of the pinned staging ring at a few GB/s,
+// so large copies are cut into pieces that the workers and the caller share.
+class CopyPool {
+ public:
+  struct Span {
+    uint8_t* dst;
+    const uint8_t* src;
+    uint64_t bytes;
+  };
+  explicit CopyPool(unsigned workers) {
+    for (unsigned i = 0; i < workers; ++i) threads_.emplace_back([this] { loop(); });
+  }
+  ~CopyPool() {
+    {
+      std::lock_guard<std::mutex> l(mu_);
+      stop_ = true;
+    }
+    cv_.notify_all();
+    for (std::thread& t : threads_) t.join();
+  }
+  // Copies every span; returns when all bytes are in place.
+  void run(const std::vector<Span>& spans) {
+    constexpr uint64_t kPiece = 256 << 10;
+    Call call;
+    {
+      std::lock_guard<std::mutex> l(mu_);
+      for (const Span& sp : spans)
+        for (uint64_t o = 0; o < sp.bytes; o += kPiece) {
+          jobs_.push_back(Job{sp.dst + o, sp.src + o, std::min(kPiece, sp.bytes - o), &call});
+          ++call.left;
+        }
+    }
+    cv_.notify_all();
+    for (;;) {  // the caller copies too
+      Job j;
+      {
+        std::lock_guard<std::mutex> l(mu_);
+        if (jobs_.empty()) break;
+        j = jobs_.front();
+        jobs_.pop_front();
+      }
+      do_job(j);
+    }
+    std::unique_lock<std::mutex> l(mu_);
+    done_cv_.wait(l, [&] { return call.left == 0; });
+  }
+
+ private:
+  struct Call {
+    uint64_t left = 0;  // pieces not yet copied (under mu_)
+  };
+  struct Job {
+    uint8_t* dst = nullptr;
+    const uint8_t* src = nullptr;
+    uint64_t bytes = 0;
+    Call* call = nullptr;
+  };
+  void do_job(const Job& j) {
+    std::memcpy(j.dst, j.src, j.bytes);
+    std::lock_guard<std::mutex> l(mu_);
+    if (--j.call->left == 0) done_cv_.notify_all();
+  }
+  void loop() {
+    for (;;) {
+      Job j;
+      {
+        std::unique_lock<std::mutex> l(mu_);
+        cv_.wait(l, [&] { return stop_ || !jobs_.empty(); });
+        if (jobs_.empty()) return;
+        j = jobs_.front();
+        jobs_.pop_front();
+      }
+      do_job(j);
+    }
+  }
+  std::mutex mu_;
+  std::condition_variable cv_, done_cv_;
+  std::deque<Job> jobs_;
+  bool stop_ = false;
+  std::vector<std::thread> threads_;
+};
+
 struct Batch {
   uint32_t op = 0;
   std::vector<uint64_t> reqs;  // request indices, arrival order
@@ -91,7 +174,7 @@ struct Batch {
 class Server {
  public:
   Server(fm_map* m, fm_serve_request* reqs, uint64_t n, const fm_serve_config& cfg)
-      : m_(m), reqs_(reqs), n_(n), cfg_(cfg), ring_(cfg.staging_bytes) {}
+      : m_(m), reqs_(reqs), n_(n), cfg_(cfg), ring_(cfg.staging_bytes), copies_(kCopyWorkers) {}
   ~Server();
   fm_status run(fm_serve_stats* stats);
 
@@ -129,6 +212,8 @@ class Server {
   bool stop_ = false;
   fm_status async_err_ = FM_OK;
   uint64_t batches_ = 0, max_inflight_ = 0, staging_waits_ = 0;
+  static constexpr unsigned kCopyWorkers = 6;
+  CopyPool copies_;
 };
 
 Server::~Server() {
@@ -166,7 +251,12 @@ fm_status Server::setup() {
     r.batch = -1;
     r.admit_s = r.done_s = 0;
   }
-  return FM_OK;
+  // Size the map's per-batch scratch for the largest batch now, not mid-run.
+  uint64_t largest = 0;
+  for (uint64_t i = 0; i < n_; ++i) largest = std::max<uint64_t>(largest, reqs_[i].n);
+  const uint64_t most = std::max(largest, std::min<uint64_t>(cfg_.max_batch_keys, ring_.size() / (K_ + 1)));
+  std::lock_guard<std::mutex> ml(m_->mu);
+  return be.serve_reserve(most);
 }
 
 uint64_t Server::batch_bytes(uint32_t op, uint64_t n) const {
@@ -254,13 +344,16 @@ fm_status Server::dispatch_one(uint64_t first) {
   (op == FM_OP_GET ? b.found_at : b.flags_at) = at;
   const double admit = now();
   uint64_t pos = 0;
+  std::vector<CopyPool::Span> spans;
   for (uint64_t i : b.reqs) {
     fm_serve_request& r = reqs_[i];
     r.admit_s = admit;
-    std::memcpy(h_stage_ + b.keys_at + pos * K_, r.keys, r.n * K_);
-    if (op == FM_OP_INSERT) std::memcpy(h_stage_ + b.vals_at + pos * V_, r.values, r.n * V_);
+    spans.push_back({h_stage_ + b.keys_at + pos * K_, static_cast<const uint8_t*>(r.keys), r.n * K_});
+    if (op == FM_OP_INSERT)
+      spans.push_back({h_stage_ + b.vals_at + pos * V_, static_cast<const uint8_t*>(r.values), r.n * V_});
     pos += r.n;
   }
+  copies_.run(spans);
   b.id = static_cast<int32_t>(batches_++);
   hipError_t e;
   if ((e = hipEventCreateWithFlags(&b.h2d, hipEventDisableTiming)) != hipSuccess ||
@@ -308,21 +401,26 @@ void Server::completion_loop() {
       b = inflight_.front();
     }
     hipError_t e = hipEventSynchronize(b.d2h);
-    const double done = now();
     uint64_t pos = 0, claimed = 0;
+    std::vector<CopyPool::Span> spans;
+    for (uint64_t i : b.reqs) {
+      fm_serve_request& r = reqs_[i];
+      if (b.op == FM_OP_GET) {
+        spans.push_back({static_cast<uint8_t*>(r.out_values), h_stage_ + b.out_at + pos * V_, r.n * V_});
+        if (r.out_found) spans.push_back({r.out_found, h_stage_ + b.found_at + pos, r.n});
+      } else if (r.out_flags) {
+        spans.push_back({r.out_flags, h_stage_ + b.flags_at + pos, r.n});
+      }
+      pos += r.n;
+    }
+    copies_.run(spans);
+    const double done = now();
+    pos = 0;
     for (uint64_t i : b.reqs) {
       fm_serve_request& r = reqs_[i];
       uint64_t count = 0;
-      if (b.op == FM_OP_GET) {
-        std::memcpy(r.out_values, h_stage_ + b.out_at + pos * V_, r.n * V_);
-        const uint8_t* f = h_stage_ + b.found_at + pos;
-        for (uint64_t k = 0; k < r.n; ++k) count += f[k];
-        if (r.out_found) std::memcpy(r.out_found, f, r.n);
-      } else {
-        const uint8_t* f = h_stage_ + b.flags_at + pos;
-        for (uint64_t k = 0; k < r.n; ++k) count += f[k];
-        if (r.out_flags) std::memcpy(r.out_flags, f, r.n);
-      }
+      const uint8_t* f = h_stage_ + (b.op == FM_OP_GET ? b.found_at : b.flags_at) + pos;
+      for (uint64_t k = 0; k < r.n; ++k) count += f[k];
       if (b.op == FM_OP_INSERT) claimed += count;
       r.count = count;
       r.batch = b.id;
