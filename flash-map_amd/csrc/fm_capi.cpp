@@ -4,11 +4,14 @@
 // backend arm and dispatches every method to it.  fm_build mirrors
 // FlashMapBuilder::build (lib.rs:414-449): GPU first, CPU on failure with a
 // message on stderr.
+#include <algorithm>
+#include <chrono>
 #include <cinttypes>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "fm_internal.h"
 
@@ -323,6 +326,57 @@ fm_status fm_get_tuning(const fm_map* m, fm_tuning* out) {
 fm_status fm_set_tuning(fm_map* m, const fm_tuning* t) {
   FM_LOCK(m);
   return m->be->set_tuning(t);
+}
+
+// Exhaustive over get_group x get_block (35 shapes): gets change nothing,
+// so every shape runs on the live table.  Time per shape: median wall time
+// of `reps` bulk_get_device calls, each synchronised, after one untimed call.
+fm_status fm_autotune_get(fm_map* m, const void* d_keys, uint64_t count, uint32_t reps, fm_tuning* out_best,
+                          double* out_get_us) {
+  FM_LOCK(m);
+  if (m->be->kind() != FM_BACKEND_GPU) return bad_arg("fm_autotune_get: the map is not on a GPU");
+  if (!d_keys || count == 0 || reps == 0 || reps > 64) return bad_arg("fm_autotune_get: need keys, count > 0, reps 1..64");
+  fm_tuning cur{};
+  fm_status st = m->be->get_tuning(&cur);
+  if (st != FM_OK) return st;
+  const uint64_t V = m->be->value_size();
+  void* d_out = nullptr;
+  if ((st = m->be->alloc_device(count * V + count, &d_out)) != FM_OK) return st;
+  uint8_t* d_found = static_cast<uint8_t*>(d_out) + count * V;
+  static const uint32_t kGroups[] = {1, 2, 4, 8, 16};
+  static const uint32_t kBlocks[] = {64, 128, 256, 512, 1024};
+  fm_tuning top = cur;
+  double top_us = -1;
+  for (uint32_t g : kGroups) {
+    for (uint32_t b : kBlocks) {
+      fm_tuning t = cur;
+      t.get_group = g;
+      t.get_block = b;
+      if ((st = m->be->set_tuning(&t)) != FM_OK) break;
+      std::vector<double> us;
+      for (uint32_t r = 0; r <= reps && st == FM_OK; ++r) {
+        const auto t0 = std::chrono::steady_clock::now();
+        st = m->be->bulk_get_device(d_keys, count, d_out, d_found);
+        if (st == FM_OK) st = m->be->synchronize();
+        const auto t1 = std::chrono::steady_clock::now();
+        if (r) us.push_back(std::chrono::duration<double, std::micro>(t1 - t0).count());
+      }
+      if (st != FM_OK) break;
+      std::sort(us.begin(), us.end());
+      if (top_us < 0 || us[us.size() / 2] < top_us) top = t, top_us = us[us.size() / 2];
+    }
+    if (st != FM_OK) break;
+  }
+  const fm_status fst = m->be->free_device(d_out);
+  if (st != FM_OK) {
+    (void)m->be->set_tuning(&cur);
+    return st;
+  }
+  if (fst != FM_OK) return fst;
+  if ((st = m->be->set_tuning(&top)) != FM_OK) return st;
+  if (out_best) *out_best = top;
+  if (out_get_us) *out_get_us = top_us;
+  return FM_OK;
 }
 
 }  // extern "C"

@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -149,7 +150,7 @@ kg_ins_leader(const Table* __restrict__ tables, OpDesc op, const uint8_t* __rest
         }
       }
     }
-    wave_add(&ctr[ch.table].count, claimed);
+    block_add(&ctr[ch.table].count, claimed);  // chunk-uniform table
   }
 }
 
@@ -393,7 +394,7 @@ kg_rem_apply(const Table* __restrict__ tables, OpDesc op, const unsigned long lo
         t.owner[slot] = kNoOwner;
       }
     }
-    wave_add(&ctr[ch.table].count, removed);
+    block_add(&ctr[ch.table].count, removed);  // chunk-uniform table
   }
 }
 
@@ -538,6 +539,10 @@ class GroupImpl {
   fm_status export_slots(uint32_t table, uint32_t* flags, void* keys, void* values);
   fm_status copy_from(GroupImpl& src);
   fm_status set_tuning(const fm_group_tuning& t);
+  fm_status autotune(GroupImpl* scratch, const uint8_t* ik, const uint8_t* iv, const uint64_t* ic, const uint8_t* gk,
+                     const uint64_t* gc, uint8_t* ov, uint8_t* of, const uint8_t* rk, const uint64_t* rc,
+                     const fm_group_tuning* cand, uint32_t n_cand, uint32_t reps, fm_group_tuning* best,
+                     double* best_us, uint32_t* evaluated);
 
   uint32_t n_ = 0;
   int dev_ = -1;
@@ -913,7 +918,8 @@ fm_status GroupImpl::step(const uint8_t* ik, const uint8_t* iv, const uint64_t* 
   std::vector<const void*> key = {ik, iv, gk, ov, of, rk, d_chunks_[0], d_chunks_[1], d_chunks_[2], dslot_,
                                   reinterpret_cast<const void*>(static_cast<uintptr_t>(tune_.group)),
                                   reinterpret_cast<const void*>(static_cast<uintptr_t>(tune_.block)),
-                                  reinterpret_cast<const void*>(static_cast<uintptr_t>(tune_.grid))};
+                                  reinterpret_cast<const void*>(static_cast<uintptr_t>(tune_.grid)),
+                                  reinterpret_cast<const void*>(static_cast<uintptr_t>(tune_.coop_arc))};
   hipError_t e;
   if (!exec_ || key != graph_key_) {
     if (exec_) (void)hipGraphExecDestroy(exec_);
@@ -972,6 +978,106 @@ fm_status GroupImpl::copy_from(GroupImpl& src) {
   if (e == hipSuccess) e = hipMemcpyAsync(empty_all_, src.empty_all_, total_words_ * 8, hipMemcpyDeviceToDevice, stream_);
   if (e != hipSuccess) return fail(FM_ERR_TRANSFER, hmsg("fm_group_copy", e));
   len_ = src.len_;
+  return FM_OK;
+}
+
+// Launch-shape search.  Every run restores `scratch` from this group's
+// tables, so each candidate steps from the same state and must give the same
+// per-table counts and statuses; the wall time of the whole step (descriptor
+// upload, graph launch, counter read-back) is what is compared.
+fm_status GroupImpl::autotune(GroupImpl* scratch, const uint8_t* ik, const uint8_t* iv, const uint64_t* ic,
+                              const uint8_t* gk, const uint64_t* gc, uint8_t* ov, uint8_t* of, const uint8_t* rk,
+                              const uint64_t* rc, const fm_group_tuning* cand, uint32_t n_cand, uint32_t reps,
+                              fm_group_tuning* best, double* best_us, uint32_t* evaluated) {
+  fm_status st;
+  if ((st = use()) != FM_OK) return st;
+  if (reps == 0 || reps > 64) return fail(FM_ERR_INVALID_ARGUMENT, "fm_group_autotune: reps must be 1..64");
+  std::unique_ptr<GroupImpl> own;
+  if (!scratch) {
+    std::vector<uint64_t> caps(n_);
+    for (uint32_t t = 0; t < n_; ++t) caps[t] = h_tables_[t].cap;
+    own.reset(new (std::nothrow) GroupImpl());
+    if (!own) return fail(FM_ERR_GPU_ALLOC, "fm_group_autotune: host allocation failed");
+    std::string err;
+    if ((st = own->init(n_, caps.data(), K_, V_, mode_, dev_, &err)) != FM_OK) return fail(st, err);
+    scratch = own.get();
+  }
+  std::vector<uint64_t> ref_new(n_), ref_rem(n_), cur_new(n_), cur_rem(n_);
+  std::vector<int32_t> ref_status(n_), cur_status(n_);
+  bool have_ref = false;
+  uint32_t runs = 0;
+  // Median wall time of `reps` restored steps (after one untimed step that
+  // captures the graph for this shape), or a negative value on error.
+  auto measure = [&](const fm_group_tuning& t, double* us) -> fm_status {
+    fm_status s;
+    if ((s = scratch->set_tuning(t)) != FM_OK) return s;
+    std::vector<double> times;
+    for (uint32_t r = 0; r <= reps; ++r) {
+      if ((s = scratch->copy_from(*this)) != FM_OK) return s;
+      hipError_t e = hipStreamSynchronize(scratch->stream_);
+      if (e != hipSuccess) return fail(FM_ERR_TRANSFER, hmsg("fm_group_autotune: restore", e));
+      const auto t0 = std::chrono::steady_clock::now();
+      s = scratch->step(ik, iv, ic, gk, gc, ov, of, rk, rc, cur_new.data(), cur_status.data(), cur_rem.data());
+      const auto t1 = std::chrono::steady_clock::now();
+      if (s != FM_OK) return s;
+      if (r) times.push_back(std::chrono::duration<double, std::micro>(t1 - t0).count());
+    }
+    ++runs;
+    if (!have_ref) {
+      ref_new = cur_new;
+      ref_status = cur_status;
+      ref_rem = cur_rem;
+      have_ref = true;
+    } else if (cur_new != ref_new || cur_status != ref_status || cur_rem != ref_rem) {
+      return fail(FM_ERR_KERNEL_LAUNCH, "fm_group_autotune: launch shapes gave different per-table results");
+    }
+    std::sort(times.begin(), times.end());
+    *us = times[times.size() / 2];
+    return FM_OK;
+  };
+  fm_group_tuning top = tune_;
+  double top_us = 0;
+  if (n_cand) {  // exhaustive over the caller's list
+    bool any = false;
+    for (uint32_t i = 0; i < n_cand; ++i) {
+      double us;
+      if ((st = measure(cand[i], &us)) != FM_OK) return st;
+      if (!any || us < top_us) top = cand[i], top_us = us, any = true;
+    }
+  } else {  // coordinate descent from the current tuning, one knob at a time
+    static const uint32_t kGroups[] = {1, 2, 4, 8, 16};
+    static const uint32_t kBlocks[] = {64, 128, 256, 512};
+    static const uint32_t kGrids[] = {256, 512, 1024, 2048, 4096, 8192};
+    static const uint32_t kArcs[] = {0, 16, 64, 256, 4096};
+    struct Knob {
+      uint32_t fm_group_tuning::*field;
+      const uint32_t* vals;
+      uint32_t n;
+    };
+    const Knob knobs[] = {{&fm_group_tuning::block, kBlocks, 4},
+                          {&fm_group_tuning::grid, kGrids, 6},
+                          {&fm_group_tuning::group, kGroups, 5},
+                          {&fm_group_tuning::coop_arc, kArcs, 5}};
+    if ((st = measure(top, &top_us)) != FM_OK) return st;
+    for (int pass = 0; pass < 3; ++pass) {
+      bool moved = false;
+      for (const Knob& k : knobs) {
+        for (uint32_t i = 0; i < k.n; ++i) {
+          if (top.*k.field == k.vals[i]) continue;
+          fm_group_tuning t = top;
+          t.*k.field = k.vals[i];
+          double us;
+          if ((st = measure(t, &us)) != FM_OK) return st;
+          if (us < top_us * 0.97) top = t, top_us = us, moved = true;  // 3%: above run-to-run noise
+        }
+      }
+      if (!moved) break;
+    }
+  }
+  tune_ = top;
+  if (best) *best = top;
+  if (best_us) *best_us = top_us;
+  if (evaluated) *evaluated = runs;
   return FM_OK;
 }
 
@@ -1100,6 +1206,29 @@ fm_status fm_group_copy(fm_group* dst, fm_group* src) {
   if (dst == src) return FM_OK;
   std::scoped_lock l(dst->mu, src->mu);
   return dst->impl.copy_from(src->impl);
+}
+
+fm_status fm_group_autotune(fm_group* g, fm_group* scratch, const void* d_ins_keys, const void* d_ins_values,
+                            const uint64_t* ins_counts, const void* d_get_keys, const uint64_t* get_counts,
+                            void* d_out_values, uint8_t* d_out_found, const void* d_rem_keys,
+                            const uint64_t* rem_counts, const fm_group_tuning* candidates, uint32_t n_candidates,
+                            uint32_t reps, fm_group_tuning* out_best, double* out_step_us, uint32_t* out_evaluated) {
+  if (!g || !ins_counts || !get_counts || !rem_counts || (n_candidates && !candidates))
+    return group_bad("fm_group_autotune: null argument");
+  if (scratch == g) return group_bad("fm_group_autotune: scratch must be another group");
+  if (scratch) {
+    std::scoped_lock l(g->mu, scratch->mu);
+    return g->impl.autotune(&scratch->impl, static_cast<const uint8_t*>(d_ins_keys),
+                            static_cast<const uint8_t*>(d_ins_values), ins_counts,
+                            static_cast<const uint8_t*>(d_get_keys), get_counts, static_cast<uint8_t*>(d_out_values),
+                            d_out_found, static_cast<const uint8_t*>(d_rem_keys), rem_counts, candidates, n_candidates,
+                            reps, out_best, out_step_us, out_evaluated);
+  }
+  std::lock_guard<std::mutex> l(g->mu);
+  return g->impl.autotune(nullptr, static_cast<const uint8_t*>(d_ins_keys), static_cast<const uint8_t*>(d_ins_values),
+                          ins_counts, static_cast<const uint8_t*>(d_get_keys), get_counts,
+                          static_cast<uint8_t*>(d_out_values), d_out_found, static_cast<const uint8_t*>(d_rem_keys),
+                          rem_counts, candidates, n_candidates, reps, out_best, out_step_us, out_evaluated);
 }
 
 fm_status fm_group_get_tuning(const fm_group* g, fm_group_tuning* out) {

@@ -193,11 +193,17 @@ def lib() -> ctypes.CDLL:
             "fm_group_copy": (ctypes.c_int, [vp, vp]),
             "fm_group_get_tuning": (ctypes.c_int, [vp, ctypes.POINTER(GroupTuning)]),
             "fm_group_set_tuning": (ctypes.c_int, [vp, ctypes.POINTER(GroupTuning)]),
+            "fm_group_autotune": (ctypes.c_int, [vp, vp, vp, vp, pu64, vp, pu64, vp, vp, vp, pu64,
+                                                 ctypes.POINTER(GroupTuning), u32, u32,
+                                                 ctypes.POINTER(GroupTuning), ctypes.POINTER(ctypes.c_double),
+                                                 ctypes.POINTER(ctypes.c_uint32)]),
             "fm_profile_enable": (ctypes.c_int, [vp, ctypes.c_int]),
             "fm_profile_read": (ctypes.c_int, [vp, ctypes.POINTER(KernelStat), u32,
                                                ctypes.POINTER(ctypes.c_uint32)]),
             "fm_get_tuning": (ctypes.c_int, [vp, ctypes.POINTER(Tuning)]),
             "fm_set_tuning": (ctypes.c_int, [vp, ctypes.POINTER(Tuning)]),
+            "fm_autotune_get": (ctypes.c_int, [vp, vp, u64, u32, ctypes.POINTER(Tuning),
+                                               ctypes.POINTER(ctypes.c_double)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -461,6 +467,14 @@ class FlashMap:
                 setattr(t, name, val)
         _check(lib().fm_set_tuning(self._h, ctypes.byref(t)))
 
+    def autotune_get(self, d_keys: DeviceBuffer, count: int, reps: int = 5) -> dict:
+        """Sets the fastest get_group x get_block for gets of these keys
+        (fm_autotune_get); the table is not changed."""
+        best, us = Tuning(), ctypes.c_double(0)
+        _check(lib().fm_autotune_get(self._h, ctypes.c_void_p(d_keys.ptr), count, reps, ctypes.byref(best),
+                                     ctypes.byref(us)))
+        return {"get_group": best.get_group, "get_block": best.get_block, "get_us": round(us.value, 1)}
+
 
 class ServeResult:
     """Outputs of one served request."""
@@ -653,6 +667,59 @@ class FlashMapGroup:
             if val is not None:
                 setattr(t, name, val)
         _check(lib().fm_group_set_tuning(self._h, ctypes.byref(t)))
+
+    def tuning_key(self, ins_counts, get_counts, rem_counts) -> str:
+        """Cache key of a step shape: sizes, table count, mean load (0.1
+        steps) and keys per step (powers of two)."""
+        caps = np.array([self.capacity(t) for t in range(self.n)], dtype=np.float64)
+        lens = np.array([self.len(t) for t in range(self.n)], dtype=np.float64)
+        keys = int(_u64s(ins_counts).sum() + _u64s(get_counts).sum() + _u64s(rem_counts).sum())
+        load = float((lens / caps).mean())
+        return f"K{self.key_size}-V{self.value_size}-T{self.n}-load{load:.1f}-keys2^{max(0, keys).bit_length()}"
+
+    def autotune(self, ins_keys, ins_vals, ins_counts, get_keys, get_counts, d_out, d_found, rem_keys,
+                 rem_counts, reps: int = 3, candidates=None, scratch: "FlashMapGroup" = None,
+                 cache: Optional[str] = None) -> dict:
+        """Picks the fastest launch shape for this step (fm_group_autotune)
+        and sets it.  ``cache``: a JSON file of tunings by step shape
+        (tuning_key); a hit sets the cached tuning without measuring, a miss
+        measures and records the result.  The tables are left unchanged."""
+        import json
+
+        key = self.tuning_key(ins_counts, get_counts, rem_counts)
+        if cache and os.path.exists(cache):
+            with open(cache) as f:
+                known = json.load(f)
+            if key in known:
+                self.set_tuning(**known[key]["tuning"])
+                return dict(known[key], key=key, cached=True)
+        ic, gc, rc = _u64s(ins_counts), _u64s(get_counts), _u64s(rem_counts)
+        cand = None
+        if candidates:
+            cand = (GroupTuning * len(candidates))()
+            for i, c in enumerate(candidates):
+                cand[i] = GroupTuning(c["group"], c["block"], c["grid"], c.get("coop_arc", 64))
+        best, us, ran = GroupTuning(), ctypes.c_double(0), ctypes.c_uint32(0)
+        vp = ctypes.c_void_p
+        _check(lib().fm_group_autotune(self._h, scratch._h if scratch else None, vp(ins_keys.ptr), vp(ins_vals.ptr),
+                                       self._p64(ic), vp(get_keys.ptr), self._p64(gc), vp(d_out.ptr),
+                                       vp(d_found.ptr), vp(rem_keys.ptr), self._p64(rc), cand,
+                                       len(candidates) if candidates else 0, reps, ctypes.byref(best),
+                                       ctypes.byref(us), ctypes.byref(ran)))
+        rec = {"tuning": {k: getattr(best, k) for k in ("group", "block", "grid", "coop_arc")},
+               "step_us": round(us.value, 1), "evaluated": int(ran.value)}
+        if cache:
+            known = {}
+            if os.path.exists(cache):
+                with open(cache) as f:
+                    known = json.load(f)
+            known[key] = rec
+            os.makedirs(os.path.dirname(os.path.abspath(cache)), exist_ok=True)
+            tmp = cache + ".tmp"
+            with open(tmp, "w") as f:
+                json.dump(known, f, indent=1, sort_keys=True)
+            os.replace(tmp, cache)
+        return dict(rec, key=key, cached=False)
 
 
 class FlashMapBuilder:

@@ -215,6 +215,7 @@ class Config3:
     row_ids: np.ndarray  # (n,) u64
     probes: np.ndarray   # (m, 32) u8
     probe_hit: np.ndarray  # (m,) bool -- True where the probe is an inserted key
+    probe_row: np.ndarray = None  # (m,) int64 -- row id of an inserted probe, -1 for a miss
 
 
 def config3(n_load=40_000_000, n_probe=100_000_000, zipf_s=1.0, hit=0.9, capacity=50_000_000, seed=3,
@@ -230,7 +231,8 @@ def config3(n_load=40_000_000, n_probe=100_000_000, zipf_s=1.0, hit=0.9, capacit
     order = st.permutation(n_probe)
     probes = np.concatenate([hits, miss])[order]
     is_hit = np.concatenate([np.ones(n_hit, bool), np.zeros(n_probe - n_hit, bool)])[order]
-    return Config3(capacity, keys, rows, probes, is_hit)
+    row = np.concatenate([perm[ranks].astype(np.int64), np.full(n_probe - n_hit, -1, np.int64)])[order]
+    return Config3(capacity, keys, rows, probes, is_hit, row)
 
 
 # ---------------------------------------------------------------------------

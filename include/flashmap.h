@@ -201,7 +201,7 @@ fm_status fm_profile_read(fm_map* m, fm_kernel_stat* out, uint32_t max,
                           uint32_t* count);
 
 /* ---- launch tuning (no reference counterpart) --------------------------
- * group: lanes that probe one key together (1..64, power of two);
+ * group: lanes that probe one key together (1,2,4,8,16);
  * block: threads per workgroup (64..1024, multiple of 64). */
 typedef struct fm_tuning {
   uint32_t get_group;
@@ -220,6 +220,15 @@ typedef struct fm_tuning {
 } fm_tuning;
 fm_status fm_get_tuning(const fm_map* m, fm_tuning* out);
 fm_status fm_set_tuning(fm_map* m, const fm_tuning* t);
+/* Get autotuner (no reference counterpart; bulk_get launches 8 queries of
+ * 32 lanes per 256-thread block, kernels.cu:365-468 / gpu.rs:141-167).
+ * Times fm_bulk_get_device on count keys at d_keys for every get_group in
+ * {1,2,4,8,16} x get_block in {64,...,1024} (median wall time of `reps`
+ * synchronised calls, 1..64, after one untimed call) and sets the fastest;
+ * the table is not changed.  GPU maps only. */
+fm_status fm_autotune_get(fm_map* m, const void* d_keys, uint64_t count,
+                          uint32_t reps, fm_tuning* out_best,
+                          double* out_get_us);
 
 /* ---- serving loop (no reference counterpart) ---------------------------
  * Closest reference analogue: AsyncFlashMap (src/async_map.rs:32-106), which
@@ -336,6 +345,27 @@ fm_status fm_group_export_slots(fm_group* g, uint32_t table, uint32_t* flags,
 fm_status fm_group_copy(fm_group* dst, fm_group* src);
 fm_status fm_group_get_tuning(const fm_group* g, fm_group_tuning* out);
 fm_status fm_group_set_tuning(fm_group* g, const fm_group_tuning* t);
+/* Autotuner (no reference counterpart).  Times fm_group_step_device on the
+ * given batches for candidate tunings and sets the fastest on g.  Each run
+ * steps a scratch group restored from g's tables first, so g's tables are
+ * left unchanged; every candidate must give the same per-table counts and
+ * statuses (else FM_ERR_KERNEL_LAUNCH).  candidates/n_candidates: a list to
+ * try in full, or NULL/0 for a coordinate descent from g's current tuning
+ * over group {1,2,4,8,16}, block {64..512}, grid {256..8192} and coop_arc
+ * {0,16,64,256,4096}.  Per candidate: one untimed step, then the median wall
+ * time of `reps` (1..64) steps.  scratch: a group of g's shape, or NULL to
+ * allocate one for the call (as much device memory as g).  d_out_values /
+ * d_out_found receive one step's get results on g's current state. */
+fm_status fm_group_autotune(fm_group* g, fm_group* scratch,
+                            const void* d_ins_keys, const void* d_ins_values,
+                            const uint64_t* ins_counts, const void* d_get_keys,
+                            const uint64_t* get_counts, void* d_out_values,
+                            uint8_t* d_out_found, const void* d_rem_keys,
+                            const uint64_t* rem_counts,
+                            const fm_group_tuning* candidates,
+                            uint32_t n_candidates, uint32_t reps,
+                            fm_group_tuning* out_best, double* out_step_us,
+                            uint32_t* out_evaluated);
 
 #ifdef __cplusplus
 }

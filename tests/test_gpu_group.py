@@ -49,7 +49,7 @@ def _tables(n, seed, cap_hi, ins_hi, get_hi, rem_hi):
     return tabs + [full, fill]
 
 
-def _run_and_check(tabs, use_step, tuning=None, steps=1):
+def _run_and_check(tabs, use_step, tuning=None, steps=1, autotune=None):
     n = len(tabs)
     g = fm.FlashMapGroup([t.capacity for t in tabs], 8, V)
     if tuning:
@@ -71,6 +71,13 @@ def _run_and_check(tabs, use_step, tuning=None, steps=1):
     rk = g.upload(_concat([_rows64(t.rem_keys) for t in tabs], 8))
     out = g.alloc_device(max(1, sum(gc)) * V)
     found = g.alloc_device(max(1, sum(gc)))
+    if autotune is not None:
+        # the search steps a scratch copy: the steps below must still see the
+        # preloaded state, and run with the tuning it picked
+        rec = g.autotune(ik, iv, ic, gk, gc, out, found, rk, rc, **autotune)
+        say(f"autotune: {rec}")
+        assert {k: getattr(g.tuning(), k) for k in rec["tuning"]} == rec["tuning"]
+        assert rec["evaluated"] >= 1 or rec["cached"]
     for s in range(steps):
         if use_step:
             new, status, removed = g.step_device(ik, iv, ic, gk, gc, out, found, rk, rc)
@@ -126,3 +133,31 @@ def test_group_large_closures():
     say("group large closures")
     tabs = _tables(6, 14, 1 << 15, 1 << 15, 1 << 10, 1 << 8)
     _run_and_check(tabs, True)
+
+
+def test_group_autotune_search():
+    """Coordinate-descent search on a scratch group, then two checked steps
+    with the chosen shape."""
+    say("group autotune search")
+    _run_and_check(_tables(12, 15, 1 << 13, 1 << 12, 1 << 11, 1 << 9), True, steps=2, autotune=dict(reps=1))
+
+
+def test_group_autotune_candidates_and_cache(tmp_path):
+    """An explicit candidate list with a caller-provided scratch group; the
+    result is cached and a second group of the same shape takes it from the
+    cache without measuring."""
+    say("group autotune cache")
+    tabs = _tables(8, 16, 1 << 12, 1 << 10, 1 << 10, 1 << 8)
+    cache = str(tmp_path / "tune.json")
+    cand = [dict(group=2, block=128, grid=512, coop_arc=16), dict(group=8, block=256, grid=1024, coop_arc=64),
+            dict(group=4, block=64, grid=256, coop_arc=0)]
+    caps = [t.capacity for t in tabs]
+    scratch = fm.FlashMapGroup(caps, 8, V)
+    _run_and_check(tabs, True, autotune=dict(reps=2, candidates=cand, scratch=scratch, cache=cache))
+    import json
+    with open(cache) as f:
+        known = json.load(f)
+    assert len(known) == 1
+    rec = next(iter(known.values()))
+    assert rec["evaluated"] == 3 and rec["tuning"] in cand
+    _run_and_check(tabs, True, autotune=dict(reps=2, candidates=cand, cache=cache))

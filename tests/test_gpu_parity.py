@@ -395,3 +395,65 @@ def test_gpu_serve_hot_insert_batch_matches_oracle():
     ev, ef = o.bulk_get(pool)
     assert np.array_equal(results[1].values, ev) and np.array_equal(results[1].found.astype(bool), ef)
     assert_same_slots(m, o, "after served hot inserts")
+
+
+def test_config3_reduced_bit_exact():
+    """Config 3 shape ([u8;32] -> u64 row id, zipf probes) at 1/200 size vs the oracle."""
+    from flashmap_amd.workloads import config3
+
+    say("config3 reduced")
+    c = config3(n_load=200_000, n_probe=500_000, capacity=250_000)
+    rows = c.row_ids.view(np.uint8).reshape(-1, 8)
+    m, o = gpu_make(c.capacity, 32, 8), OracleAdapter(c.capacity, 32, 8)
+    apply_and_compare(m, o, [("ins", c.keys, rows), ("get", c.probes)])
+
+
+def test_config3_full_row_ids():
+    """Config 3 at full load: 40M keys -> row id in a 50M map, 20M zipf probes.
+
+    The oracle cannot run this size in test time, so the check is the
+    property the workload fixes: every hit returns exactly the row id its key
+    was loaded with, every miss is not found, and the map holds 40M entries.
+    """
+    from flashmap_amd.workloads import config3
+
+    say("config3 full: generating")
+    c = config3(n_probe=20_000_000)
+    n, m_probe = len(c.keys), len(c.probes)
+    m = gpu_make(c.capacity, 32, 8)
+    dk = m.upload(c.keys)
+    dv = m.upload(c.row_ids.view(np.uint8))
+    say("config3 full: insert")
+    assert m.bulk_insert_device(dk, dv, n) == n
+    assert len(m) == n
+    del dk, dv
+    dp = m.upload(c.probes)
+    dout, dfound = m.alloc_device(m_probe * 8), m.alloc_device(m_probe)
+    say("config3 full: get")
+    m.bulk_get_device(dp, m_probe, dout, dfound)
+    found = m.download(dfound).astype(bool)
+    vals = m.download(dout).view("<u8")
+    assert np.array_equal(found, c.probe_row >= 0)
+    hit = c.probe_row >= 0
+    assert np.array_equal(vals[hit], c.probe_row[hit].astype(np.uint64))
+    assert not vals[~hit].any()
+    say("config3 full: ok")
+
+
+def test_gpu_autotune_get():
+    """The get autotuner leaves the table alone and picks a valid shape;
+    gets with it still match the oracle."""
+    say("autotune get")
+    rng = np.random.default_rng(21)
+    cap = 1 << 16
+    m, o = gpu_make(cap, 32, 8), OracleAdapter(cap, 32, 8)
+    keys = rng.integers(0, 256, (50_000, 32), dtype=np.uint8)
+    vals = rng.integers(0, 256, (50_000, 8), dtype=np.uint8)
+    assert m.bulk_insert(keys, vals) == o.bulk_insert(keys, vals)
+    q = np.concatenate([keys[::3], rng.integers(0, 256, (20_000, 32), dtype=np.uint8)])
+    dq = m.upload(q)
+    rec = m.autotune_get(dq, len(q), reps=2)
+    t = m.tuning()
+    assert (t.get_group, t.get_block) == (rec["get_group"], rec["get_block"])
+    assert rec["get_group"] in (1, 2, 4, 8, 16) and rec["get_block"] in (64, 128, 256, 512, 1024)
+    apply_and_compare(m, o, [("get", q)])
