@@ -7,9 +7,9 @@ pairs.  One step applies, in order, 20 bulk_insert batches of 100,000 pairs
 (50% overwrites of existing keys, 50% new keys, no duplicates inside a batch),
 one bulk_get of 1,000,000 queries at a 90% hit rate and one bulk_remove of
 200,000 existing keys: 3,200,000 key-ops.  Every step starts from the
-preloaded state: two working tables alternate, and while one runs a step the
-other is restored from a device snapshot on its own stream.  Inputs are
-resident in HBM before the timed region; results stay on the device.
+preloaded state: two working tables alternate and are restored from a device
+snapshot between steps, outside the timed brackets.  Inputs are resident in
+HBM before the timed region; results stay on the device.
 
 The first step is checked bit for bit against the sequential CPU oracle
 (counts, every get value and found flag, final length).  The CPU baseline
@@ -80,21 +80,28 @@ class Step:
         return new, removed
 
 
-def cpu_baseline(cfg, threads):
-    """Rayon-port timing of one step (after an untimed preload)."""
+def cpu_baseline(cfg, threads, target_s=10.0, max_reps=400):
+    """Rayon-port timing: the step repeated from the preloaded state (restored
+    between repetitions, untimed) until about target_s seconds are timed."""
     from oracle.binding import RayonPortMap
 
+    snap = RayonPortMap(cfg.capacity, K_BYTES, V_BYTES, 0, threads=threads)
+    snap.bulk_insert(cfg.preload_keys, cfg.preload_vals)
     p = RayonPortMap(cfg.capacity, K_BYTES, V_BYTES, 0, threads=threads)
-    p.bulk_insert(cfg.preload_keys, cfg.preload_vals)
-    t0 = time.perf_counter()
-    for k, v in cfg.batches:
-        p.bulk_insert(k, v)
-    p.bulk_get(cfg.get_keys)
-    p.bulk_remove(cfg.remove_keys)
-    dt = time.perf_counter() - t0
+    timed, reps = 0.0, 0
+    while timed < target_s and reps < max_reps:
+        p.copy_from(snap, structure_only=True)
+        t0 = time.perf_counter()
+        for k, v in cfg.batches:
+            p.bulk_insert(k, v)
+        p.bulk_get(cfg.get_keys)
+        p.bulk_remove(cfg.remove_keys)
+        timed += time.perf_counter() - t0
+        reps += 1
     keys = sum(len(k) for k, _ in cfg.batches) + len(cfg.get_keys) + len(cfg.remove_keys)
     p.close()
-    return keys / dt / 1e6, dt
+    snap.close()
+    return keys * reps / timed / 1e6, timed, reps
 
 
 def oracle_step(cfg):
@@ -121,12 +128,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--cpu-threads", type=int, default=0, help="0: min(16, host cores)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="timed length of the CPU-baseline sample")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--scale", type=float, default=1.0, help="shrink the workload (testing only)")
     ap.add_argument("--config", choices=["c2", "c4", "c5"], default="c2",
                     help="c2: account-state mix (default, the metric line); c4: 512-table fused vs plain; "
                          "c5: serving loop at several offered loads")
     ap.add_argument("--tables", type=int, default=512, help="c4: number of tables")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--requests", type=int, default=400, help="c5: requests per offered load")
     ap.add_argument("--rates", default="2000,10000,50000", help="c5: offered loads, requests/s")
     args = ap.parse_args()
@@ -181,6 +190,13 @@ def main():
     # on or indexes by a value, so each step does the same work; the checked
     # step above ran on a fully restored table.
     maps = [A, B]
+    get_tune = None
+    if not args.no_autotune:
+        # get launch shape (group width x block) for this table and query mix
+        get_tune = snap.autotune_get(step.get_keys, step.n_get, reps=5)
+        for m in maps:
+            m.set_tuning(get_group=get_tune["get_group"], get_block=get_tune["get_block"])
+        log(f"get autotune: {get_tune}")
     A.copy_from(snap, structure_only=True)
     B.copy_from(snap, structure_only=True)
     for i in range(args.warmup):
@@ -250,10 +266,10 @@ def main():
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         log(f"CPU baseline (Rayon restatement, {threads} threads)")
-        rate, dt = cpu_baseline(cfg, threads)
+        rate, dt, reps = cpu_baseline(cfg, threads, target_s=args.cpu_seconds)
         cpu = {"value": round(rate, 3), "unit": "Mkeys/s", "cores": threads, "kind": "port",
-               "sample": f"one step ({step.keys_per_step} key-ops) after an untimed 8M preload; "
-                         f"{dt:.2f} s; host '{host_cpu_model()}'"}
+               "sample": f"{reps} steps of {step.keys_per_step} key-ops, each from the 8M preloaded state "
+                         f"(restored untimed between steps); {dt:.2f} s timed; host '{host_cpu_model()}'"}
 
     out = {
         "metric": "bulk_get/insert Mkeys/s + % HBM peak; batched vs plain; serving p99 latency",
@@ -284,6 +300,7 @@ def main():
                 "insert_hbm_frac": ins_rate and round(ins_rate * 1e6 * B_INS / 1e9 / HBM_PEAK_GBS, 5)},
         "kernels_ms": {k: {"launches": v[0], "avg_us": round(v[1] / v[0] * 1e3, 3)} for k, v in sorted(kstats.items())},
         "preload_mkeys_s": round(len(cfg.preload_keys) / preload_s / 1e6, 2),
+        "get_autotune": get_tune,
         "check": check,
     }
     print(json.dumps(out), flush=True)
@@ -325,6 +342,16 @@ def run_c4(args):
     gk, rk = work.upload(cat([rows64(t.get_keys) for t in tabs], 8)), work.upload(cat([rows64(t.rem_keys) for t in tabs], 8))
     out, found = work.alloc_device(max(1, sum(gc)) * V), work.alloc_device(max(1, sum(gc)))
     keys_per_step = sum(ic) + sum(gc) + sum(rc)
+
+    # launch shapes: coordinate-descent search, each run on `work` restored
+    # from the preloaded snapshot (fm_group_autotune), then used by `work`
+    tune = None
+    if not args.no_autotune:
+        log("c4: autotuning the step's launch shapes")
+        t0 = time.perf_counter()
+        tune = snap.autotune(ik, iv, ic, gk, gc, out, found, rk, rc, reps=3, scratch=work)
+        tune["search_s"] = round(time.perf_counter() - t0, 2)
+        work.set_tuning(**tune["tuning"])
 
     # fused: warm up (captures the graph), then time restored repetitions
     reps = max(1, args.steps)
@@ -415,7 +442,8 @@ def run_c4(args):
                                "70% hits) + remove (log-uniform [1,2^12]) per table",
                    "keys_per_step": keys_per_step, "slots": int(sum(caps)),
                    "table_full_batches": int(sum(1 for s in f_status if s))},
-        "tuning": {k: getattr(work.tuning(), k) for k in ("group", "block", "grid")},
+        "tuning": {k: getattr(work.tuning(), k) for k in ("group", "block", "grid", "coop_arc")},
+        "autotune": tune,
         "check": {"fused_equals_plain": exact, "oracle_sample_tables": len(sample), "oracle_exact": bool(oracle_ok)},
     }
     print(json.dumps(out_line), flush=True)

@@ -60,6 +60,8 @@ def lib():
         L.or_par_bulk_remove.restype = ctypes.c_uint64
         L.or_par_bulk_remove.argtypes = [ctypes.c_void_p, u8p, ctypes.c_uint64]
         L.or_par_len.restype = ctypes.c_uint64
+        L.or_par_copy_from.restype = ctypes.c_int
+        L.or_par_copy_from.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
         L.or_par_len.argtypes = [ctypes.c_void_p]
         _LIB = L
     return _LIB
@@ -192,3 +194,7 @@ class RayonPortMap:
     def bulk_remove(self, keys):
         k = _u8(keys)
         return int(lib().or_par_bulk_remove(self.h, _ptr(k), k.size // self.K))
+
+    def copy_from(self, src: "RayonPortMap", structure_only: bool = False):
+        if lib().or_par_copy_from(self.h, src.h, 1 if structure_only else 0) != 0:
+            raise ValueError("rayon-port maps differ in shape")

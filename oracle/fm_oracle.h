@@ -96,6 +96,7 @@ int or_par_bulk_insert(or_par_map* m, const uint8_t* keys,
 uint64_t or_par_bulk_remove(or_par_map* m, const uint8_t* keys, uint64_t n);
 uint64_t or_par_len(const or_par_map* m);
 int or_par_threads(const or_par_map* m);
+int or_par_copy_from(or_par_map* dst, const or_par_map* src, int structure_only);
 
 #ifdef __cplusplus
 }

@@ -302,3 +302,19 @@ uint64_t or_par_len(const or_par_map* m) {
     return atomic_load_explicit(&((or_par_map*)m)->len, memory_order_relaxed);
 }
 int or_par_threads(const or_par_map* m) { return m->threads; }
+
+/* Copies src's table into dst (same shape): flags, keys and len; the value
+ * bytes too unless structure_only (no probe branches on a value, so a timing
+ * loop may leave them).  Lets a timing loop restart every repetition from
+ * one preloaded state. */
+int or_par_copy_from(or_par_map* dst, const or_par_map* src, int structure_only) {
+    if (dst->capacity != src->capacity || dst->key_size != src->key_size ||
+        dst->value_size != src->value_size || dst->hash_mode != src->hash_mode)
+        return -1;
+    memcpy((void*)dst->flags, (const void*)src->flags, dst->capacity * sizeof(uint32_t));
+    memcpy(dst->keys, src->keys, dst->capacity * (dst->key_size ? dst->key_size : 1));
+    if (!structure_only)
+        memcpy(dst->values, src->values, dst->capacity * (dst->value_size ? dst->value_size : 1));
+    atomic_store(&dst->len, atomic_load(&((or_par_map*)src)->len));
+    return 0;
+}

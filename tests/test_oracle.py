@@ -359,3 +359,27 @@ def test_rayon_port_agrees_where_order_cannot_matter():
     vo, fo = o.bulk_get(c.queries.view(np.uint8))
     vp, fp = p.bulk_get(c.queries.view(np.uint8))
     assert np.array_equal(fo, fp) and np.array_equal(vo, vp)
+
+
+@pytest.mark.parametrize("structure_only", [False, True])
+def test_rayon_port_copy_restores_state(structure_only):
+    """The restore the CPU-baseline timing loop uses: a copy answers gets and
+    len like its source, and an op on the copy leaves the source alone."""
+    rng = np.random.default_rng(13)
+    keys = rng.integers(0, 256, (3000, 16), dtype=np.uint8)
+    vals = rng.integers(0, 256, (3000, 8), dtype=np.uint8)
+    src = RayonPortMap(1 << 13, 16, 8, 0, threads=3)
+    dst = RayonPortMap(1 << 13, 16, 8, 0, threads=3)
+    assert src.bulk_insert(keys, vals) == 3000
+    dst.bulk_insert(keys[:10], vals[:10] ^ 1)
+    dst.copy_from(src, structure_only=structure_only)
+    assert len(dst) == len(src) == 3000
+    vs, fs = src.bulk_get(keys)
+    vd, fd = dst.bulk_get(keys)
+    assert fd.all() and np.array_equal(fs, fd)
+    if not structure_only:
+        assert np.array_equal(vs, vd)
+    assert dst.bulk_remove(keys[:500]) == 500
+    assert len(src) == 3000 and src.bulk_get(keys[:500])[1].all()
+    with pytest.raises(ValueError):
+        dst.copy_from(RayonPortMap(1 << 12, 16, 8, 0, threads=1))
