@@ -25,6 +25,7 @@ using dev::Counters;
 using dev::Table;
 
 constexpr uint64_t kMaxGpuCapacity = 1ull << 31;  // u32 slot ids in scratch
+constexpr uint64_t kQKeepMax = 1u << 15;          // removes up to this size keep Q_ == freebits
 
 // ---------------------------------------------------------------------------
 // Per-(K,V) launchers.  Specialised sizes get register-resident keys and
@@ -360,6 +361,7 @@ class GpuBackend final : public Backend {
   uint32_t fused_spin_ = 1u << 24;  // grid barrier polls (FM_FUSED_SPIN), ~tens of seconds
   // Stream-ordered facts about device state, kept by the host:
   bool q_synced_ = false;      // Q_ == freebits (the fused insert keeps it so)
+  bool q_pending_ = false;     // the last led insert restored Q_; valid once its status is clean
   bool ctr_zero_ = false;      // ctr_ holds zeros
   bool ctr_in_host_ = false;   // the last op left its final counters in h_ctr_
 
@@ -616,7 +618,7 @@ fm_status GpuBackend::ensure_staging(uint64_t bk, uint64_t bv, uint64_t bo, uint
 
 // Zeroes the counters for an op that reads them back with read_counters.
 fm_status GpuBackend::zero_counters() {
-  hipError_t e = hipMemsetAsync(ctr_, 0, sizeof(Counters), stream_);
+  hipError_t e = ctr_zero_ ? hipSuccess : hipMemsetAsync(ctr_, 0, sizeof(Counters), stream_);
   ctr_zero_ = false;  // the op about to run accumulates into them
   ctr_in_host_ = false;
   return e == hipSuccess ? FM_OK : fail(FM_ERR_KERNEL_LAUNCH, hip_msg("hipMemsetAsync", e));
@@ -714,18 +716,22 @@ fm_status GpuBackend::run_insert(const uint8_t* d_keys, const uint8_t* d_vals, u
       q_synced_ = ctr_zero_ = ctr_in_host_ = false;
       return st;
     }
-    q_synced_ = true;  // unless the dry pass failed (check_insert_status)
+    q_synced_ = false;
+    q_pending_ = true;  // Q_ == freebits once the status shows no failed dry pass or stall
     ctr_zero_ = true;
     ctr_in_host_ = true;
     return FM_OK;
   }
   if ((st = zero_counters()) != FM_OK) return st;
-  q_synced_ = false;  // Q_ is this op's claim bitmap
-  {
+  // Q_ is this op's claim bitmap: a copy of freebits, unless the last led
+  // insert (or a remove after it) left it equal to freebits already.
+  if (!(q_synced_ && led)) {
     Prof p(this, kKidInsCopy);
     hipError_t e = hipMemcpyAsync(Q_, t_.freebits, words * 8, hipMemcpyDeviceToDevice, stream_);
     if (e != hipSuccess) return fail(FM_ERR_TRANSFER, hip_msg("bitmap copy", e));
   }
+  q_synced_ = false;
+  q_pending_ = false;
   if (dedup) {
     Prof p(this, kKidInsDedup);
     if ((st = check_launch(ops_.ins_dedup(t, d_keys, n, blk, stream_), "k_ins_dedup")) != FM_OK) return st;
@@ -751,8 +757,14 @@ fm_status GpuBackend::run_insert(const uint8_t* d_keys, const uint8_t* d_vals, u
         return st;
     }
     Prof p(this, kKidInsBits);
-    hipLaunchKernelGGL(dev::k_ins_bits, dim3(grid_for(n, blk)), dim3(blk), 0, stream_, t, dslot_, n, ctr_);
-    return check_launch(hipGetLastError(), "k_ins_bits");
+    // 1024-thread workgroups: one completion ticket per 1024 keys
+    hipLaunchKernelGGL(dev::k_ins_bits, dim3(grid_for(n, 1024)), dim3(1024), 0, stream_, t, dslot_, n, ctr_, Q_,
+                       h_ctr_);
+    if ((st = check_launch(hipGetLastError(), "k_ins_bits")) != FM_OK) return st;
+    q_pending_ = true;     // Q_ == freebits once the status shows no failed dry pass
+    ctr_zero_ = true;      // k_ins_bits zeroed the counters ...
+    ctr_in_host_ = true;   // ... after writing them to h_ctr_
+    return FM_OK;
   }
   {
     Prof p(this, kKidInsDry);
@@ -787,7 +799,10 @@ fm_status GpuBackend::run_remove(const uint8_t* d_keys, uint64_t n, uint64_t* re
   fm_status st;
   Table t = t_;
   t.outcome = outcome;
-  t.freecopy = q_synced_ ? Q_ : nullptr;  // tombstones free slots in Q_ too
+  // Keeping Q_ equal to freebits costs one atomic per removed key; above
+  // kQKeepMax keys the next insert's copy of freebits is cheaper.
+  t.freecopy = (q_synced_ && n <= kQKeepMax) ? Q_ : nullptr;
+  if (!t.freecopy) q_synced_ = false;
   if ((st = ensure_scratch(n)) != FM_OK) return st;
   if ((st = zero_counters()) != FM_OK) return st;
   const uint32_t blk = tune_.insert_block;
@@ -954,6 +969,7 @@ fm_status GpuBackend::bulk_insert(const void* keys, const void* values, uint64_t
   Counters c;
   if ((st = read_counters(&c)) != FM_OK) return st;
   if ((st = check_insert_status(c)) != FM_OK) return st;
+  q_synced_ = q_pending_;  // a clean led insert left Q_ equal to freebits
   len_ += c.count;
   *out_new = c.count;
   return FM_OK;
@@ -1099,6 +1115,7 @@ fm_status GpuBackend::bulk_insert_device(const void* d_keys, const void* d_vals,
   Counters c;
   if ((st = read_counters(&c)) != FM_OK) return st;
   if ((st = check_insert_status(c)) != FM_OK) return st;
+  q_synced_ = q_pending_;  // a clean led insert left Q_ equal to freebits
   len_ += c.count;
   if (out_new) *out_new = c.count;
   return FM_OK;

@@ -223,6 +223,13 @@ struct Blob<0> {
 __device__ __forceinline__ unsigned long long load_word(const unsigned long long* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+__device__ __forceinline__ uint32_t ld_agent(const uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_agent(uint32_t* p, uint32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 
 // First set bit at or after `start`, cyclically; cap if there is none.
 __device__ inline uint64_t bm_next_set(const unsigned long long* bm, uint64_t start, uint64_t cap) {
@@ -960,9 +967,14 @@ k_ins_big(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __restric
 }
 
 // Publishes the batch's real claims (a subset of its dry claims) in both
-// bitmaps and resets owner[] for the next op.
+// bitmaps and resets owner[] for the next op.  Q (optional): the dry pass's
+// claim bitmap; a dry claim that ended unused (the key updated an existing
+// slot) gets its Q bit back, which leaves Q equal to the new freebits, so
+// the next insert needs no copy.  host_out (optional): the last workgroup
+// writes the op's final counters there and zeroes them for the next op.
 static __global__ void __launch_bounds__(1024)
-k_ins_bits(Table t, const uint32_t* __restrict__ dslot, uint64_t n, const Counters* ctr) {
+k_ins_bits(Table t, const uint32_t* __restrict__ dslot, uint64_t n, Counters* ctr, unsigned long long* Q,
+           Counters* host_out) {
   const bool failed = (ctr->status & kStatusDryFail) != 0;
   for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
        i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
@@ -974,12 +986,28 @@ k_ins_bits(Table t, const uint32_t* __restrict__ dslot, uint64_t n, const Counte
     if (failed) continue;
     const uint64_t s = dslot[i];
     if (s == kNoSlot) continue;
+    const unsigned long long bit = 1ull << (s & 63);
     if ((t.flags[s] & 0xFu) == kOccupied) {
-      const unsigned long long bit = 1ull << (s & 63);
       atomicAnd(&t.freebits[s >> 6], ~bit);
       atomicAnd(&t.emptybits[s >> 6], ~bit);
+    } else if (Q) {
+      atomicOr(&Q[s >> 6], bit);
     }
     t.owner[s] = kNoOwner;
+  }
+  // No fence: this kernel changes none of the published fields, every
+  // workgroup's read of ctr->status precedes its ticket, and the host reads
+  // host_out after the kernel has completed.  (An agent-scope fence writes
+  // back the L2 on this part, which costs more than the whole kernel.)
+  if (!host_out) return;
+  __syncthreads();
+  if (threadIdx.x == 0 && atomicAdd(&ctr->done, 1u) == gridDim.x - 1) {
+    Counters out{};
+    out.status = ld_agent(&ctr->status);
+    out.nbig = ld_agent(&ctr->nbig);
+    out.count = __hip_atomic_load(&ctr->count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    *host_out = out;
+    *ctr = Counters{};
   }
 }
 
@@ -1033,13 +1061,6 @@ struct FusedInsert {
   uint32_t coop_arc;
   uint32_t spin;          // polls a grid barrier waits before giving up
 };
-
-__device__ __forceinline__ uint32_t ld_agent(const uint32_t* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void st_agent(uint32_t* p, uint32_t v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
 
 // Grid-wide barrier number `k` (1, 2, ...) of the cooperative launch, with a
 // bounded wait.  Arrivals accumulate in one counter; the workgroup that
