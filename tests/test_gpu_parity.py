@@ -457,3 +457,53 @@ def test_gpu_autotune_get():
     assert (t.get_group, t.get_block) == (rec["get_group"], rec["get_block"])
     assert rec["get_group"] in (1, 2, 4, 8, 16) and rec["get_block"] in (64, 128, 256, 512, 1024)
     apply_and_compare(m, o, [("get", q)])
+
+
+def test_gpu_claim_bitmap_reuse_sequence():
+    """The claim bitmap is reused between inserts when the last insert left it
+    equal to the free-slot bitmap.  This history crosses every event that
+    must keep it equal or force a fresh copy: consecutive inserts, small
+    removes (kept in step), large removes (copy next time), duplicate-heavy
+    and fill-to-near-full batches, an uncounted insert, clear, slot import
+    and table copies, each followed by inserts checked against the oracle."""
+    say("claim bitmap reuse")
+    rng = np.random.default_rng(77)
+    cap = 1 << 17
+    m, o = gpu_make(cap, 32, 8), OracleAdapter(cap, 32, 8)
+    pool = rng.integers(0, 256, (60_000, 32), dtype=np.uint8)
+
+    def batch(n, dup=False):
+        idx = rng.integers(0, len(pool) if not dup else 500, n)
+        return pool[idx], rng.integers(0, 256, (n, 8), dtype=np.uint8)
+
+    ops = []
+    for _ in range(3):
+        ops.append(("ins",) + batch(20_000))
+    ops.append(("rem", pool[rng.integers(0, len(pool), 5_000)]))     # small remove: Q kept
+    ops.append(("ins",) + batch(20_000, dup=True))
+    ops.append(("rem", pool[rng.integers(0, len(pool), 40_000)]))    # large remove: copy next insert
+    ops.append(("ins",) + batch(30_000))
+    ops.append(("get", pool))
+    apply_and_compare(m, o, ops)
+    # uncounted insert (no status check), then counted inserts
+    k, v = batch(10_000)
+    dk, dv = m.upload(k), m.upload(v)
+    m.bulk_insert_device_uncounted(dk, dv, len(k))
+    o.bulk_insert(k, v)
+    m.recount()
+    apply_and_compare(m, o, [("ins",) + batch(20_000), ("ins",) + batch(20_000)])
+    # a batch that fills the table (sort path), then led inserts again
+    apply_and_compare(m, o, [("rem", pool[:30_000])])
+    room = cap - len(o)
+    fresh = rng.integers(0, 256, (room, 32), dtype=np.uint8)
+    apply_and_compare(m, o, [("ins", fresh, np.zeros((room, 8), np.uint8)), ("rem", fresh[:50_000]),
+                             ("ins",) + batch(20_000), ("ins",) + batch(20_000)])
+    # clear, then slot import and a full table copy
+    apply_and_compare(m, o, [("clear",), ("ins",) + batch(20_000), ("ins",) + batch(20_000)])
+    flags, keys, vals = o.export_slots()
+    m2 = gpu_make(cap, 32, 8)
+    m2.import_slots(flags, keys, vals, len(o))
+    apply_and_compare(m2, o, [("ins",) + batch(20_000)])
+    m.copy_from(m2)
+    apply_and_compare(m, o, [("ins",) + batch(20_000), ("rem", pool[rng.integers(0, len(pool), 3_000)]),
+                             ("ins",) + batch(20_000)])
