@@ -163,7 +163,8 @@ kg_ins_big(const Table* __restrict__ tables, OpDesc op, const uint8_t* __restric
   __shared__ uint32_t lst[kBigMax];
   __shared__ uint32_t carry[2 * kMaxGenericSize / 4];
   __shared__ uint32_t cnt;
-  __shared__ unsigned long long barrier_at;
+  __shared__ unsigned long long barrier_at, barrier_before;
+  __shared__ uint32_t arc_lds[(KS > 0 && VS > 0 && KS % 4 == 0 && VS % 4 == 0) ? kArcLdsWords : 1];
   const uint32_t nbig = *nbig_p;
   for (uint32_t c = blockIdx.x; c < nbig; c += gridDim.x) {
     const uint32_t table = static_cast<uint32_t>(big[c] >> 32);
@@ -227,10 +228,21 @@ kg_ins_big(const Table* __restrict__ tables, OpDesc op, const uint8_t* __restric
         }
       }
       if constexpr (KS > 0 && VS > 0 && KS % 4 == 0 && VS % 4 == 0) {
-        uint32_t claimed = 0;  // one wave per workgroup walks each member
-        for (uint32_t j = 0; j < m; ++j) {
-          const uint64_t gk = to.begin + lst[j];
-          claimed += insert_one_wave<KS, VS>(t, keys + gk * K, vals + gk * V, carry, &ctr[table]);
+        // The whole arc (from the barrier before s to the one after) in LDS
+        // when it fits; otherwise one wave walks each member in HBM.
+        if (threadIdx.x == 0) barrier_before = windowed ? s : bm_prev_set(P, s, t.cap);
+        __syncthreads();
+        const uint64_t b = barrier_before;
+        const uint64_t L = (barrier_at - b - 1) & t.mask;
+        uint32_t claimed = 0;
+        if (!windowed && b != s && L <= arc_lds_cap<KS>()) {
+          claimed = replay_arc_lds<KS, VS>(t, keys + to.begin * K, vals + to.begin * V, (b + 1) & t.mask,
+                                           static_cast<uint32_t>(L), lst, m, arc_lds, &ctr[table]);
+        } else {
+          for (uint32_t j = 0; j < m; ++j) {  // one wave per workgroup walks each member
+            const uint64_t gk = to.begin + lst[j];
+            claimed += insert_one_wave<KS, VS>(t, keys + gk * K, vals + gk * V, carry, &ctr[table]);
+          }
         }
         if (threadIdx.x == 0 && claimed) atomicAdd(&ctr[table].count, static_cast<unsigned long long>(claimed));
       } else if (threadIdx.x == 0) {

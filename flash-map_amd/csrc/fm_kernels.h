@@ -764,6 +764,209 @@ __device__ uint32_t insert_one_wave(const Table& t, const uint8_t* __restrict__ 
   }
 }
 
+// ---------------------------------------------------------------------------
+// A whole closure replayed in LDS.  A closure owns its arc: the slots
+// strictly between two barriers (free slots no walk of the batch reaches),
+// and its members' walks start at homes inside the arc and never leave it.
+// So the arc's flags and keys are read into LDS once, every member's walk
+// (the walk of insert_one_wave: 64 slots per window, decisions in slot
+// order) runs on LDS, and the arc is written back once.  Values stay in
+// HBM during the replay: each arc slot carries a value source instead (the
+// arc-relative slot whose pre-replay value it holds, or kArcBatch | the
+// batch index of the value), and the write-back copies each moved value
+// once.  Values only move forward along the arc (Robin Hood evictions push
+// residents to later slots), so writing back from the end of the arc down,
+// each chunk of 64 slots reading all its sources before writing, reads
+// every source before it is overwritten.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kArcLdsWords = 28672;       // 112 KiB: flags, value sources and keys of one arc
+constexpr uint32_t kArcBatch = 0x80000000u;    // value source: a batch value, not an arc slot
+
+template <int KS>
+__host__ __device__ constexpr uint32_t arc_lds_cap() {
+  return kArcLdsWords / (2 + KS / 4);
+}
+
+// Replays members lst[0..m) (batch indices into in_keys, in batch order) of
+// the closure whose arc is the L slots from r0.  Called by all 64 lanes of
+// a single-wave workgroup; `lds` holds kArcLdsWords words.  Returns the free
+// slots claimed (in every lane).
+template <int KS, int VS>
+__device__ uint32_t replay_arc_lds(const Table& t, const uint8_t* __restrict__ in_keys,
+                                   const uint8_t* __restrict__ in_vals, uint64_t r0, uint32_t L,
+                                   const uint32_t* lst, uint32_t m, uint32_t* lds, Counters* ctr) {
+  static_assert(KS % 4 == 0 && VS % 4 == 0 && KS > 0 && VS > 0, "static word sizes");
+  constexpr int KW = KS / 4;
+  const uint32_t lane = threadIdx.x & 63u;
+  uint32_t* fl = lds;          // flags
+  uint32_t* vs = lds + L;      // value sources
+  uint32_t* ky = lds + 2 * L;  // keys, KW words per slot
+  for (uint32_t q = lane; q < L; q += 64) {
+    const uint64_t slot = (r0 + q) & t.mask;
+    fl[q] = t.flags[slot];
+    vs[q] = q;
+    Blob<KS> k;
+    k.load(t.keys + slot * KS, KS);
+#pragma unroll
+    for (int w = 0; w < KW; ++w) ky[q * KW + w] = k.w[w];
+  }
+  wave_sync();
+  uint32_t claimed = 0;
+  Blob<KS> mk;       // lane l: key of member (j & ~63) + l, read 64 members at a time
+  uint32_t mvi = 0;  // and its value index
+  for (uint32_t j = 0; j < m; ++j) {
+    if ((j & 63u) == 0) {
+      const uint32_t jl = j + lane;
+      if (jl < m) {
+        const uint64_t kl = lst[jl];
+        mk.load(in_keys + kl * KS, KS);
+        mvi = static_cast<uint32_t>(dd_value_index(t, kl));
+      }
+    }
+    const uint64_t kb = lst[j];
+    Blob<KS> ck;  // carried key, identical in every lane
+#pragma unroll
+    for (int w = 0; w < KW; ++w)
+      ck.w[w] = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(mk.w[w]), static_cast<int>(j & 63u)));
+    const uint32_t cvs =
+        kArcBatch | static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(mvi), static_cast<int>(j & 63u)));
+    Blob<KS> ck0 = ck;  // the pair carried into the current window
+    uint32_t cvs0 = cvs;
+    // The walk as prefix minima.  Along a window, a carried pair that
+    // entered at slot e with position p_e has position p_e + (k - e) at slot
+    // k, and evicts the resident there iff that exceeds the resident's dist
+    // d_k, i.e. iff d_k - k < p_e - e.  The evicted resident is carried on
+    // with position d_k + 1 at slot k + 1, i.e. the same quantity d_k - k.
+    // So with a_k = d_k - k per occupied slot and a = p for the pair carried
+    // into the window (slot 0), the eviction points are the slots where a_k
+    // is a strict new minimum of the a's before it, the pair arriving at slot
+    // k is the one holding that running minimum, and its position there is
+    // the minimum + k.  An equal key ends the walk only where a_k equals the
+    // running minimum (position == dist); a free slot ends it; so does a
+    // position past cap - 1 (the carried pair is dropped, rayon_cpu.rs:
+    // 167-169).  One wave scan per window finds all of it.
+    int64_t p = 0;
+    uint32_t pos = static_cast<uint32_t>(((ck.hash(KS, t.mode) & t.mask) - r0) & t.mask);
+    uint32_t c1 = 0;
+    for (;;) {
+      const uint32_t q = pos + lane;
+      const bool in = q < L;
+      const uint32_t f = in ? fl[q] : 0u;
+      const bool occ = (f & 0xFu) == kOccupied;
+      Blob<KS> rk;
+#pragma unroll
+      for (int w = 0; w < KW; ++w) rk.w[w] = occ ? ky[q * KW + w] : 0u;
+      const uint32_t rvs = in ? vs[q] : 0u;
+      const int64_t ak = (in && occ) ? static_cast<int64_t>(f >> 4) - static_cast<int64_t>(lane) : INT64_MAX;
+      // inclusive running minimum (value, lane), the earlier lane kept on ties
+      int64_t mv = ak;
+      int mi = static_cast<int>(lane);
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const int64_t v2 = __shfl_up(mv, off, 64);
+        const int i2 = __shfl_up(mi, off, 64);
+        if (lane >= static_cast<uint32_t>(off) && v2 <= mv) {
+          mv = v2;
+          mi = i2;
+        }
+      }
+      // exclusive, with the carried-in pair (lane -1) first
+      int64_t ev = __shfl_up(mv, 1, 64);
+      int ei = __shfl_up(mi, 1, 64);
+      if (lane == 0 || !(ev < p)) {
+        ev = p;
+        ei = -1;
+      }
+      // the pair at slot `lane` and its key
+      Blob<KS> hk;
+#pragma unroll
+      for (int w = 0; w < KW; ++w) {
+        const uint32_t x = static_cast<uint32_t>(__shfl(static_cast<int>(rk.w[w]), ei < 0 ? 0 : ei, 64));
+        hk.w[w] = ei < 0 ? ck0.w[w] : x;
+      }
+      const uint32_t hvs_l = static_cast<uint32_t>(__shfl(static_cast<int>(rvs), ei < 0 ? 0 : ei, 64));
+      const uint32_t hvs = ei < 0 ? cvs0 : hvs_l;
+      const int64_t ppos = ev + static_cast<int64_t>(lane);  // position of that pair here
+      bool eq = occ;
+#pragma unroll
+      for (int w = 0; w < KW; ++w) eq = eq && rk.w[w] == hk.w[w];
+      const bool drop = ppos > static_cast<int64_t>(t.mask);
+      const bool stop = drop || !in || !occ || (ak == ev && eq);
+      const unsigned long long stops = __ballot(stop);
+      const uint32_t E = stops ? static_cast<uint32_t>(__ffsll(stops) - 1) : 64u;
+      // evictions before the end, and the end slot itself
+      if (lane < E && in && occ && ak < ev) {
+        vs[q] = hvs;
+        fl[q] = make_flag(kOccupied, static_cast<uint64_t>(ppos));
+#pragma unroll
+        for (int w = 0; w < KW; ++w) ky[q * KW + w] = hk.w[w];
+      }
+      bool end = false;
+      if (E < 64) {
+        end = true;
+        const uint32_t eflag = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(f), E));
+        const bool edrop = (__builtin_amdgcn_readlane(static_cast<int>(drop), E)) != 0;
+        if (edrop) {
+          // carried pair dropped: nothing written here
+        } else if (pos + E >= L) {
+          if (lane == 0) atomicOr(&ctr->status, kStatusBudget);
+        } else if ((eflag & 0xFu) != kOccupied) {  // EMPTY or TOMBSTONE: claim
+          if (lane == E) {
+            vs[q] = hvs;
+            fl[q] = make_flag(kOccupied, static_cast<uint64_t>(ppos));
+#pragma unroll
+            for (int w = 0; w < KW; ++w) ky[q * KW + w] = hk.w[w];
+          }
+          c1 = 1;
+        } else if (lane == E) {  // equal key: update its value
+          vs[q] = hvs;
+        }
+      } else {  // the pair holding the window's minimum is carried into the next window
+        int64_t tv = __shfl(mv, 63, 64);
+        int ti = __shfl(mi, 63, 64);
+        if (!(tv < p)) {
+          tv = p;
+          ti = -1;
+        }
+        if (ti >= 0) {
+#pragma unroll
+          for (int w = 0; w < KW; ++w) ck0.w[w] = static_cast<uint32_t>(__shfl(static_cast<int>(rk.w[w]), ti, 64));
+          cvs0 = static_cast<uint32_t>(__shfl(static_cast<int>(rvs), ti, 64));
+        }
+        p = tv + 64;
+      }
+      wave_sync();
+      if (end) break;
+      pos += 64;
+    }
+    claimed += c1;
+    if (lane == 0 && t.outcome) t.outcome[kb] = static_cast<uint8_t>(c1);
+  }
+  // Write-back, from the end of the arc down (see above).
+  for (int64_t base = static_cast<int64_t>((L - 1) & ~63u); base >= 0; base -= 64) {
+    const uint32_t q = static_cast<uint32_t>(base) + lane;
+    const bool in = q < L;
+    const uint32_t src = in ? vs[q] : q;
+    const bool moved = in && src != q;
+    Blob<VS> v;
+    if (moved) {
+      if (src & kArcBatch) v.load(in_vals + static_cast<uint64_t>(src & ~kArcBatch) * VS, VS);
+      else v.load(t.vals + ((r0 + src) & t.mask) * VS, VS);
+    }
+    wave_sync();  // every lane's source read before any write of this chunk
+    if (in) {
+      const uint64_t slot = (r0 + q) & t.mask;
+      t.flags[slot] = fl[q];
+      Blob<KS> k;
+#pragma unroll
+      for (int w = 0; w < KW; ++w) k.w[w] = ky[q * KW + w];
+      k.store(t.keys + slot * KS, KS);
+      if (moved) v.store(t.vals + slot * VS, VS);
+    }
+  }
+  return claimed;
+}
+
 // Leaders order and replay closures of up to kLeaderMax keys whose arc
 // (barrier to barrier) is at most `coop_arc` slots; other closures are
 // queued for k_ins_big.
@@ -841,7 +1044,8 @@ k_ins_leader(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __rest
 template <int KS, int VS, bool kPublish>
 __device__ uint32_t replay_arc(const Table& t, const uint8_t* __restrict__ in_keys,
                                const uint8_t* __restrict__ in_vals, uint64_t n, unsigned long long* Q, uint64_t s,
-                               uint32_t* lst, uint32_t lcap, uint32_t* carry, uint32_t* cnt, Counters* ctr) {
+                               uint32_t* lst, uint32_t lcap, uint32_t* carry, uint32_t* cnt, Counters* ctr,
+                               uint32_t* arc_lds = nullptr) {
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t K = KS ? KS : t.K;
   const uint32_t V = VS ? VS : t.V;
@@ -901,11 +1105,22 @@ __device__ uint32_t replay_arc(const Table& t, const uint8_t* __restrict__ in_ke
       }
     }
     if constexpr (KS > 0 && VS > 0 && KS % 4 == 0 && VS % 4 == 0) {
-      for (uint32_t j = 0; j < m; ++j) {
-        const uint64_t k = lst[j];
-        const uint32_t c1 = insert_one_wave<KS, VS>(t, in_keys + k * K, in_vals + dd_value_index(t, k) * V, carry, ctr);
-        claimed += c1;
-        if (lane == 0 && t.outcome) t.outcome[k] = static_cast<uint8_t>(c1);
+      // The whole arc (from the barrier before s to x) in LDS when it fits.
+      long long bb = 0;
+      if (arc_lds && !windowed && lane == 0) bb = static_cast<long long>(bm_prev_set(P, s, t.cap));
+      const uint64_t b = static_cast<uint64_t>(__shfl(bb, 0, 64));
+      const uint64_t L = (x - b - 1) & t.mask;
+      if (arc_lds && !windowed && b != s && L <= arc_lds_cap<KS>()) {
+        claimed += replay_arc_lds<KS, VS>(t, in_keys, in_vals, (b + 1) & t.mask, static_cast<uint32_t>(L), lst, m,
+                                          arc_lds, ctr);
+      } else {
+        for (uint32_t j = 0; j < m; ++j) {
+          const uint64_t k = lst[j];
+          const uint32_t c1 =
+              insert_one_wave<KS, VS>(t, in_keys + k * K, in_vals + dd_value_index(t, k) * V, carry, ctr);
+          claimed += c1;
+          if (lane == 0 && t.outcome) t.outcome[k] = static_cast<uint8_t>(c1);
+        }
       }
     } else {
       uint32_t c0 = 0;
@@ -958,10 +1173,11 @@ k_ins_big(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __restric
   __shared__ uint32_t lst[kBigMax];
   __shared__ uint32_t carry[2 * kMaxGenericSize / 4];
   __shared__ uint32_t cnt;
+  __shared__ uint32_t arc[(KS > 0 && VS > 0 && KS % 4 == 0 && VS % 4 == 0) ? kArcLdsWords : 1];
   const uint32_t nbig = __hip_atomic_load(&ctr->nbig, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   for (uint32_t c = blockIdx.x; c < nbig; c += gridDim.x) {
     const uint32_t claimed = replay_arc<KS, VS, false>(t, in_keys, in_vals, n, const_cast<unsigned long long*>(Q),
-                                                       big[c], lst, kBigMax, carry, &cnt, ctr);
+                                                       big[c], lst, kBigMax, carry, &cnt, ctr, arc);
     if (threadIdx.x == 0 && claimed) atomicAdd(&ctr->count, static_cast<unsigned long long>(claimed));
   }
 }
