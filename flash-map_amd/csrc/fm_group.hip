@@ -50,12 +50,16 @@ struct OpDesc {  // device-resident description of one op over the group
   const uint32_t* n_chunks;
 };
 
+// nbig[0]: closures queued for kg_ins_big, nbig[1]: its work ticket.
 __global__ void kg_zero(Counters* a, Counters* b, uint32_t n, uint32_t* nbig) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     a[i] = Counters{0, 0, 0};
     b[i] = Counters{0, 0, 0};
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) *nbig = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    nbig[0] = 0;
+    nbig[1] = 0;
+  }
 }
 
 __global__ void kg_copy_words(unsigned long long* __restrict__ dst, const unsigned long long* __restrict__ src,
@@ -159,14 +163,24 @@ __global__ void __launch_bounds__(64)
 kg_ins_big(const Table* __restrict__ tables, OpDesc op, const uint8_t* __restrict__ keys,
            const uint8_t* __restrict__ vals, const unsigned long long* __restrict__ Q,
            const unsigned long long* __restrict__ qoff, const unsigned long long* __restrict__ big,
-           const uint32_t* nbig_p, Counters* ctr) {
+           uint32_t* nbig_p, Counters* ctr) {
   __shared__ uint32_t lst[kBigMax];
   __shared__ uint32_t carry[2 * kMaxGenericSize / 4];
   __shared__ uint32_t cnt;
   __shared__ unsigned long long barrier_at, barrier_before;
   __shared__ uint32_t arc_lds[(KS > 0 && VS > 0 && KS % 4 == 0 && VS % 4 == 0) ? kArcLdsWords : 1];
-  const uint32_t nbig = *nbig_p;
-  for (uint32_t c = blockIdx.x; c < nbig; c += gridDim.x) {
+  __shared__ uint32_t cur;
+  const uint32_t nbig = nbig_p[0];
+  // Closures are handed out one at a time (their replay times differ by
+  // orders of magnitude, so a fixed assignment would leave workgroups idle).
+  // Workgroups beyond the queue's length leave at once.
+  if (blockIdx.x >= nbig) return;
+  for (;;) {
+    if (threadIdx.x == 0) cur = atomicAdd(nbig_p + 1, 1u);
+    __syncthreads();
+    const uint32_t c = cur;
+    __syncthreads();
+    if (c >= nbig) break;
     const uint32_t table = static_cast<uint32_t>(big[c] >> 32);
     const uint64_t s = big[c] & 0xFFFFFFFFull;
     const Table t = tables[table];
@@ -704,7 +718,7 @@ fm_status GroupImpl::init(uint32_t n, const uint64_t* caps, uint32_t K, uint32_t
       (e = hipMalloc(&owner_all_, total_slots_ * 4)) != hipSuccess ||
       (e = hipMalloc(&Q_, total_words_ * 8)) != hipSuccess || (e = hipMalloc(&d_tables_, n * sizeof(Table))) != hipSuccess ||
       (e = hipMalloc(&d_qoff_, n * 8)) != hipSuccess || (e = hipMalloc(&d_nch_, 3 * 4)) != hipSuccess ||
-      (e = hipMalloc(&nbig_, 4)) != hipSuccess || (e = hipMalloc(&ins_ctr_, n * sizeof(Counters))) != hipSuccess ||
+      (e = hipMalloc(&nbig_, 8)) != hipSuccess || (e = hipMalloc(&ins_ctr_, n * sizeof(Counters))) != hipSuccess ||
       (e = hipMalloc(&rem_ctr_, n * sizeof(Counters))) != hipSuccess ||
       (e = hipHostMalloc(&h_ins_ctr_, n * sizeof(Counters), hipHostMallocDefault)) != hipSuccess ||
       (e = hipHostMalloc(&h_rem_ctr_, n * sizeof(Counters), hipHostMallocDefault)) != hipSuccess) {

@@ -114,7 +114,7 @@ struct Counters {
   uint32_t nbig;  // closures queued for the wave replay
   unsigned long long count;
   uint32_t done;      // fused insert: workgroups finished
-  uint32_t nsmall;    // fused insert: closures queued for the register replay
+  uint32_t nsmall;    // fused insert: closures queued for the register replay; k_ins_big: work ticket
 };
 
 // Grid barrier words of the fused insert, each on a cache line of its own
@@ -1174,8 +1174,18 @@ k_ins_big(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __restric
   __shared__ uint32_t carry[2 * kMaxGenericSize / 4];
   __shared__ uint32_t cnt;
   __shared__ uint32_t arc[(KS > 0 && VS > 0 && KS % 4 == 0 && VS % 4 == 0) ? kArcLdsWords : 1];
+  __shared__ uint32_t cur;
   const uint32_t nbig = __hip_atomic_load(&ctr->nbig, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  for (uint32_t c = blockIdx.x; c < nbig; c += gridDim.x) {
+  // Closures are handed out one at a time (ctr->nsmall is the ticket; their
+  // replay times differ by orders of magnitude).  Workgroups beyond the
+  // queue's length leave at once (the others take every ticket below it).
+  if (blockIdx.x >= nbig) return;
+  for (;;) {
+    if (threadIdx.x == 0) cur = atomicAdd(&ctr->nsmall, 1u);
+    __syncthreads();
+    const uint32_t c = cur;
+    __syncthreads();
+    if (c >= nbig) break;
     const uint32_t claimed = replay_arc<KS, VS, false>(t, in_keys, in_vals, n, const_cast<unsigned long long*>(Q),
                                                        big[c], lst, kBigMax, carry, &cnt, ctr, arc);
     if (threadIdx.x == 0 && claimed) atomicAdd(&ctr->count, static_cast<unsigned long long>(claimed));
