@@ -507,3 +507,24 @@ def test_gpu_claim_bitmap_reuse_sequence():
     m.copy_from(m2)
     apply_and_compare(m, o, [("ins",) + batch(20_000), ("rem", pool[rng.integers(0, len(pool), 3_000)]),
                              ("ins",) + batch(20_000)])
+
+
+@pytest.mark.parametrize("K,V", [(8, 8), (32, 128)])
+def test_gpu_long_arcs_in_lds_and_hbm(K, V):
+    """Near-full tables leave few barriers, so a batch forms closures whose
+    arcs span thousands of slots: the ones that fit the LDS buffer (up to
+    7,168 slots at K=8, 2,867 at K=32) replay there, longer ones through HBM,
+    and arcs that wrap past the end of the table are among them."""
+    say(f"long arcs K={K} V={V}")
+    rng = np.random.default_rng(91 + K)
+    cap = 1 << 14
+    m, o = gpu_make(cap, K, V), OracleAdapter(cap, K, V)
+    keys = rng.integers(0, 256, (cap, K), dtype=np.uint8)
+    vals = rng.integers(0, 256, (cap, V), dtype=np.uint8)
+    ops = [("ins", keys[:cap - 900], vals[:cap - 900]),            # ~94% full
+           ("ins", keys[cap - 900:cap - 60], vals[cap - 900:cap - 60]),  # most barriers consumed
+           ("rem", keys[:2000:7]),                                  # tombstones inside the runs
+           ("ins", keys[:4000:5], vals[4000:8000:5]),               # updates and reinserts past tombstones
+           ("ins", np.concatenate([keys[cap - 60:], keys[10:40]]), np.concatenate([vals[:60], vals[:30]])),
+           ("get", keys)]
+    apply_and_compare(m, o, ops)
