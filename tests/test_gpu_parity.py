@@ -524,7 +524,8 @@ def test_gpu_long_arcs_in_lds_and_hbm(K, V):
     ops = [("ins", keys[:cap - 900], vals[:cap - 900]),            # ~94% full
            ("ins", keys[cap - 900:cap - 60], vals[cap - 900:cap - 60]),  # most barriers consumed
            ("rem", keys[:2000:7]),                                  # tombstones inside the runs
-           ("ins", keys[:4000:5], vals[4000:8000:5]),               # updates and reinserts past tombstones
+           ("ins", keys[:1500:5], vals[4000:5500:5]),               # updates and reinserts past tombstones
            ("ins", np.concatenate([keys[cap - 60:], keys[10:40]]), np.concatenate([vals[:60], vals[:30]])),
+           ("ins", keys[:4000:5], vals[4000:8000:5]),               # len + batch > capacity: TableFull
            ("get", keys)]
     apply_and_compare(m, o, ops)
