@@ -131,8 +131,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="timed length of the CPU-baseline sample")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--scale", type=float, default=1.0, help="shrink the workload (testing only)")
-    ap.add_argument("--config", choices=["c2", "c4", "c5"], default="c2",
-                    help="c2: account-state mix (default, the metric line); c4: 512-table fused vs plain; "
+    ap.add_argument("--config", choices=["c1", "c2", "c3", "c4", "c5"], default="c2",
+                    help="c2: account-state mix (default, the metric line); c1: plain u64 map, per-op roofline; "
+                         "c3: join-probe index (40M load, 100M probes); c4: 512-table fused vs plain; "
                          "c5: serving loop at several offered loads")
     ap.add_argument("--tables", type=int, default=512, help="c4: number of tables")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
@@ -143,6 +144,10 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path (the table would shard by "
               "slot range; see the sharding design). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
+    if args.config == "c1":
+        return run_c1(args)
+    if args.config == "c3":
+        return run_c3(args)
     if args.config == "c4":
         return run_c4(args)
     if args.config == "c5":
@@ -304,6 +309,361 @@ def main():
         "check": check,
     }
     print(json.dumps(out), flush=True)
+
+
+# ---------------------------------------------------------------------------
+# Shared by configs 1 and 3: a measured stream-copy peak, per-op device time
+# from the map's kernel events, and the figures derived from them.
+# ---------------------------------------------------------------------------
+def stream_copy_peak(m, nbytes=1 << 30, reps=5):
+    """Device-to-device copy of nbytes on the map's stream (the runtime's
+    copy kernel), timed with HIP events: best and median GB/s of read+write."""
+    try:
+        hip = ctypes.CDLL("libamdhip64.so")
+    except OSError:
+        hip = ctypes.CDLL("/opt/rocm/lib/libamdhip64.so")
+    vp = ctypes.c_void_p
+    hip.hipEventCreate.argtypes = [ctypes.POINTER(vp)]
+    hip.hipEventRecord.argtypes = [vp, vp]
+    hip.hipEventSynchronize.argtypes = [vp]
+    hip.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), vp, vp]
+    hip.hipEventDestroy.argtypes = [vp]
+    hip.hipMemcpyAsync.argtypes = [vp, vp, ctypes.c_size_t, ctypes.c_int, vp]
+    src, dst = m.alloc_device(nbytes), m.alloc_device(nbytes)
+    stream = vp(m.device_stream())
+    e0, e1 = vp(), vp()
+    assert hip.hipEventCreate(ctypes.byref(e0)) == 0 and hip.hipEventCreate(ctypes.byref(e1)) == 0
+    rates = []
+    for i in range(reps + 1):
+        assert hip.hipEventRecord(e0, stream) == 0
+        assert hip.hipMemcpyAsync(vp(dst.ptr), vp(src.ptr), nbytes, 3, stream) == 0  # device to device
+        assert hip.hipEventRecord(e1, stream) == 0
+        assert hip.hipEventSynchronize(e1) == 0
+        ms = ctypes.c_float()
+        assert hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1) == 0
+        if i:  # the first copy warms up
+            rates.append(2 * nbytes / (ms.value / 1e3) / 1e9)
+    hip.hipEventDestroy(e0)
+    hip.hipEventDestroy(e1)
+    src.free()
+    dst.free()
+    return {"bytes": nbytes, "best_gbs": round(max(rates), 1), "median_gbs": round(float(np.median(rates)), 1),
+            "note": "hipMemcpyAsync device-to-device on the map's stream, read+write bytes per HIP-event time"}
+
+
+def timed_op(m, fn):
+    """Runs fn() on map m with kernel events on; returns (result, device ms
+    summed over the op's kernels, wall ms around the call, {kernel: ms})."""
+    m.synchronize()
+    m.profile(True)
+    t0 = time.perf_counter()
+    r = fn()
+    m.synchronize()
+    wall = (time.perf_counter() - t0) * 1e3
+    ks = m.profile_read()
+    m.profile(False)
+    return r, sum(ms for _, ms in ks.values()), wall, {k: ms for k, (_, ms) in ks.items()}
+
+
+def op_figures(n, b_key, line_b_key, dev_ms, wall_ms, peak_gbs, copy_gbs):
+    """Median device/wall time of one op over the timed steps and the rates
+    derived from them."""
+    d, w = float(np.median(dev_ms)), float(np.median(wall_ms))
+    ach = n * b_key / (d / 1e3) / 1e9
+    return {"keys": n, "device_ms": round(d, 4), "wall_ms": round(w, 4),
+            "mkeys_s": round(n / (d / 1e3) / 1e6, 2), "wall_mkeys_s": round(n / (w / 1e3) / 1e6, 2),
+            "alg_bytes_per_key": round(b_key, 2), "achieved_gbs": round(ach, 1),
+            "frac": round(ach / peak_gbs, 5), "frac_of_copy": round(ach / copy_gbs, 5) if copy_gbs else None,
+            "line_bytes_per_key": round(line_b_key, 2),
+            "line_frac": round(n * line_b_key / (d / 1e3) / 1e9 / peak_gbs, 5)}
+
+
+def b_get(K, V, F, h):
+    return K + F + h * (K + V) + V + 1
+
+
+def b_ins(K, V, F):
+    return 2 * (K + V) + 2 * F
+
+
+def b_rem(K, F):
+    return 2 * K + 2 * F
+
+
+# Line-granular companions: 64 B per random field touch (get: flag, key, and
+# the value on a hit; insert: flag, key, value; remove: flag, key) plus the
+# streamed query/pair input and result output.
+def line_get(K, V, h):
+    return 64 * (2 + h) + K + V + 1
+
+
+def line_ins(K, V):
+    return 64 * 3 + K + V
+
+
+def line_rem(K):
+    return 64 * 2 + K
+
+
+def load_pmc(name):
+    path = os.path.join(ROOT, "profiles", name)
+    if os.path.exists(path):
+        with open(path) as f:
+            return json.load(f)
+    return None
+
+
+# ---------------------------------------------------------------------------
+# Config 1: FlashMap<u64,u64> with_capacity(2^20).  One step from the empty
+# table: bulk_insert 500k, bulk_get 500k (50% hits), bulk_remove 100k,
+# bulk_get of the same 500k queries again.  Each op is timed on its own.
+# ---------------------------------------------------------------------------
+def run_c1(args):
+    from flashmap_amd.workloads import config1
+
+    c = config1()
+    K = V = 8
+    F = 4
+    m = fm.FlashMap.builder(c.capacity, K, V).require_gpu().build()
+    assert m.backend() == fm.Backend.GPU
+    copy = stream_copy_peak(m)
+    log(f"c1: stream copy {copy}")
+    ik, iv = m.upload(c.insert_keys.view(np.uint8)), m.upload(c.insert_vals.view(np.uint8))
+    q, rk = m.upload(c.queries.view(np.uint8)), m.upload(c.remove_keys.view(np.uint8))
+    nq, ni, nr = len(c.queries), len(c.insert_keys), len(c.remove_keys)
+    out, found = m.alloc_device(nq * V), m.alloc_device(nq)
+    names = ["insert", "get", "remove", "get2"]
+    fns = {"insert": lambda: m.bulk_insert_device(ik, iv, ni),
+           "get": lambda: m.bulk_get_device(q, nq, out, found),
+           "remove": lambda: m.bulk_remove_device(rk, nr),
+           "get2": lambda: m.bulk_get_device(q, nq, out, found)}
+    dev = {k: [] for k in names}
+    wall = {k: [] for k in names}
+    kern = {k: {} for k in names}
+    results = []
+    for i in range(args.warmup + args.steps):
+        m.clear()
+        step = {}
+        for k in names:
+            r, d, w, ks = timed_op(m, fns[k])
+            if k.startswith("get"):
+                r = (hashlib.sha256(m.download(out).tobytes() + m.download(found).tobytes()).hexdigest(),
+                     int(m.download(found).astype(bool).sum())) if i == 0 else None
+            step[k] = r
+            if i >= args.warmup:
+                dev[k].append(d)
+                wall[k].append(w)
+                for kn, ms in ks.items():
+                    kern[k].setdefault(kn, []).append(ms)
+        results.append(step)
+    first = results[0]
+    hits = {"get": first["get"][1], "get2": first["get2"][1]}
+
+    check = None
+    if not args.no_check:
+        from oracle.binding import OracleMap
+        o = OracleMap(c.capacity, K, V, 0)
+        exp = {"insert": o.bulk_insert(c.insert_keys.view(np.uint8), c.insert_vals.view(np.uint8))}
+        for k in ("get", "remove", "get2"):
+            if k == "remove":
+                exp[k] = o.bulk_remove(c.remove_keys.view(np.uint8))
+            else:
+                v, f = o.bulk_get(c.queries.view(np.uint8))
+                exp[k] = (hashlib.sha256(v.tobytes() + f.astype(np.uint8).tobytes()).hexdigest(), int(f.sum()))
+        ok = all(first[k] == exp[k] for k in names) and len(m) == len(o)
+        check = {"bit_exact": bool(ok), "inserted": first["insert"], "removed": first["remove"],
+                 "hits": hits}
+        o.close()
+        if not ok:
+            log(f"PARITY FAILURE c1: gpu {first} oracle {exp}")
+            sys.exit(1)
+
+    # Host-buffer calls (PCIe-inclusive, the reference's gpu.rs path): wall time.
+    host = {k: [] for k in names}
+    hk, hv, hq, hr = c.insert_keys, c.insert_vals, c.queries, c.remove_keys
+    host_fns = {"insert": lambda: m.bulk_insert(hk, hv), "get": lambda: m.bulk_get(hq),
+                "remove": lambda: m.bulk_remove(hr), "get2": lambda: m.bulk_get(hq)}
+    for i in range(1 + min(args.steps, 5)):
+        m.clear()
+        for k in names:
+            t0 = time.perf_counter()
+            host_fns[k]()
+            if i:
+                host[k].append((time.perf_counter() - t0) * 1e3)
+
+    n = {"insert": ni, "get": nq, "remove": nr, "get2": nq}
+    h = {"get": hits["get"] / nq, "get2": hits["get2"] / nq}
+    bkey = {"insert": b_ins(K, V, F), "get": b_get(K, V, F, h["get"]), "remove": b_rem(K, F),
+            "get2": b_get(K, V, F, h["get2"])}
+    lkey = {"insert": line_ins(K, V), "get": line_get(K, V, h["get"]), "remove": line_rem(K),
+            "get2": line_get(K, V, h["get2"])}
+    ops = {}
+    pmc = load_pmc("plain_u64_1M_pmc.json") or {}
+    for k in names:
+        f = op_figures(n[k], bkey[k], lkey[k], dev[k], wall[k], HBM_PEAK_GBS, copy["best_gbs"])
+        f["host_api_ms"] = round(float(np.median(host[k])), 4)
+        f["kernels_us"] = {kn: round(float(np.median(v)) * 1e3, 2) for kn, v in sorted(kern[k].items())}
+        tr = pmc.get(k)
+        if tr:
+            f["hbm_measured_gbs"] = round(tr["hbm_bytes"] / (f["device_ms"] / 1e3) / 1e9, 1)
+            f["traffic_bytes"] = tr["hbm_bytes"]
+        ops[k] = f
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle.binding import OracleMap, RayonPortMap
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        cpu = {}
+        for kind, mk in (("port", lambda: RayonPortMap(c.capacity, K, V, 0, threads=threads)),
+                         ("sequential", lambda: OracleMap(c.capacity, K, V, 0))):
+            times = {k: [] for k in names}
+            for i in range(6):  # one warm-up, then the median of 5
+                p = mk()
+                cf = {"insert": lambda: p.bulk_insert(c.insert_keys.view(np.uint8), c.insert_vals.view(np.uint8)),
+                      "get": lambda: p.bulk_get(c.queries.view(np.uint8)),
+                      "remove": lambda: p.bulk_remove(c.remove_keys.view(np.uint8)),
+                      "get2": lambda: p.bulk_get(c.queries.view(np.uint8))}
+                for k in names:
+                    t0 = time.perf_counter()
+                    cf[k]()
+                    if i:
+                        times[k].append(time.perf_counter() - t0)
+                p.close()
+            cpu[kind] = {"cores": threads if kind == "port" else 1,
+                         "mkeys_s": {k: round(n[k] / float(np.median(times[k])) / 1e6, 2) for k in names},
+                         "sequence_mkeys_s": round(sum(n.values()) / sum(float(np.median(times[k])) for k in names)
+                                                   / 1e6, 2)}
+    total_dev = sum(ops[k]["device_ms"] for k in names)
+    value = sum(n.values()) / (total_dev / 1e3) / 1e6
+    dom = max(names, key=lambda k: ops[k]["device_ms"])
+    out_line = {
+        "metric": "bulk_get/insert Mkeys/s + % HBM peak (config 1 per-op)",
+        "value": round(value, 3), "unit": "Mkeys/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(total_dev, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u64", "data": "synthetic (SplitMix64 seed 1)",
+        "config": {"workload": "plain: FlashMap<u64,u64> with_capacity(2^20); from empty: bulk_insert 500k, "
+                               "bulk_get 500k (50% hits), bulk_remove 100k, bulk_get of the same queries",
+                   "keys_per_step": sum(n.values()), "parallelism": "single GPU",
+                   "timing": "per op: kernel events on the map's stream summed (device_ms) and wall clock "
+                             "around the synchronised call (wall_ms); medians over the timed steps"},
+        "roofline": {"bound": "hbm", "op": dom, "achieved": ops[dom]["achieved_gbs"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": ops[dom]["frac"], "traffic": ops[dom].get("traffic_bytes")},
+        "stream_copy": copy,
+        "ops": ops,
+        "cpu_baseline": cpu and {"value": cpu["port"]["sequence_mkeys_s"], "unit": "Mkeys/s",
+                                 "cores": cpu["port"]["cores"], "kind": "port",
+                                 "sample": f"the whole config-1 sequence, one warm-up then the median of 5 per op; "
+                                           f"host '{host_cpu_model()}'", "detail": cpu},
+        "check": check,
+    }
+    print(json.dumps(out_line), flush=True)
+
+
+# ---------------------------------------------------------------------------
+# Config 3: join-probe index, FlashMap<[u8;32],u64> with_capacity(50M)
+# (2^26 slots).  One step: bulk-load 40M keys -> row id into the empty
+# table, then bulk_get 100M probes (90% Zipf(1.0) over the loaded keys, 10%
+# misses).
+# ---------------------------------------------------------------------------
+def run_c3(args):
+    from flashmap_amd.workloads import config3
+
+    s = args.scale
+    log("c3: generating workload")
+    t0 = time.perf_counter()
+    c = config3(n_load=int(40_000_000 * s), n_probe=int(100_000_000 * s), capacity=int(50_000_000 * s))
+    log(f"c3: generated in {time.perf_counter() - t0:.1f} s")
+    K, V, F = 32, 8, 4
+    nl, npb = len(c.keys), len(c.probes)
+    m = fm.FlashMap.builder(c.capacity, K, V).require_gpu().build()
+    assert m.backend() == fm.Backend.GPU
+    copy = stream_copy_peak(m)
+    log(f"c3: stream copy {copy}")
+    dk, dv = m.upload(c.keys), m.upload(c.row_ids.view(np.uint8))
+    dp = m.upload(c.probes)
+    out, found = m.alloc_device(npb * V), m.alloc_device(npb)
+    hit = float(c.probe_hit.mean())
+    get_tune = None
+    dev = {"load": [], "probe": []}
+    wall = {"load": [], "probe": []}
+    kern = {"load": {}, "probe": {}}
+    check = None
+    for i in range(args.warmup + args.steps):
+        m.clear()
+        r, d, w, ks = timed_op(m, lambda: m.bulk_insert_device(dk, dv, nl))
+        assert r == nl, f"load inserted {r} of {nl}"
+        if i == 0 and not args.no_autotune:
+            get_tune = m.autotune_get(dp, min(npb, 10_000_000), reps=3)
+            m.set_tuning(get_group=get_tune["get_group"], get_block=get_tune["get_block"])
+            log(f"c3: get autotune {get_tune}")
+        _, d2, w2, ks2 = timed_op(m, lambda: m.bulk_get_device(dp, npb, out, found))
+        if i == 0 and not args.no_check:
+            # Property check: every inserted probe returns the row id its key
+            # was loaded with, every miss is not found, the map holds nl keys.
+            f = m.download(found).astype(bool)
+            v = m.download(out).view("<u8")
+            exp_hit = c.probe_row >= 0
+            ok = (len(m) == nl and np.array_equal(f, exp_hit) and
+                  np.array_equal(v[exp_hit], c.probe_row[exp_hit].astype(np.uint64)) and not v[~exp_hit].any())
+            check = {"rows_exact": bool(ok), "probes_checked": npb, "hits": int(f.sum())}
+            if not ok:
+                log("PARITY FAILURE c3")
+                sys.exit(1)
+        if i >= args.warmup:
+            for k, dd, ww, kk in (("load", d, w, ks), ("probe", d2, w2, ks2)):
+                dev[k].append(dd)
+                wall[k].append(ww)
+                for kn, ms in kk.items():
+                    kern[k].setdefault(kn, []).append(ms)
+    pmc = load_pmc("join_probe_50M_pmc.json") or {}
+    ops = {}
+    for k, n, bk, lk in (("load", nl, b_ins(K, V, F), line_ins(K, V)),
+                         ("probe", npb, b_get(K, V, F, hit), line_get(K, V, hit))):
+        f = op_figures(n, bk, lk, dev[k], wall[k], HBM_PEAK_GBS, copy["best_gbs"])
+        f["kernels_us"] = {kn: round(float(np.median(v)) * 1e3, 2) for kn, v in sorted(kern[k].items())}
+        tr = pmc.get(k)
+        if tr:
+            f["hbm_measured_gbs"] = round(tr["hbm_bytes"] / (f["device_ms"] / 1e3) / 1e9, 1)
+            f["traffic_bytes"] = tr["hbm_bytes"]
+        ops[k] = f
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle.binding import RayonPortMap
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        n_cpu_probe = min(npb, int(20_000_000 * max(s, 0.01)))
+        p = RayonPortMap(c.capacity, K, V, 0, threads=threads)
+        t0 = time.perf_counter()
+        p.bulk_insert(c.keys, c.row_ids.view(np.uint8))
+        tl = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        p.bulk_get(c.probes[:n_cpu_probe])
+        tp = time.perf_counter() - t0
+        p.close()
+        cpu = {"value": round(n_cpu_probe / tp / 1e6, 3), "unit": "Mkeys/s (probe)", "cores": threads,
+               "kind": "port", "load_mkeys_s": round(nl / tl / 1e6, 3),
+               "sample": f"load of all {nl} keys ({tl:.2f} s) then the first {n_cpu_probe} probes ({tp:.2f} s); "
+                         f"host '{host_cpu_model()}'"}
+    probe = ops["probe"]
+    out_line = {
+        "metric": "bulk_get Mkeys/s + achieved HBM GB/s (join-probe index)",
+        "value": probe["mkeys_s"], "unit": "Mkeys/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ops["load"]["device_ms"] + probe["device_ms"], 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u8", "data": "synthetic (SplitMix64 seed 3)",
+        "config": {"workload": "join-probe: FlashMap<[u8;32],u64> with_capacity(50M)=2^26 slots; bulk-load "
+                               f"{nl} keys -> row id, then {npb} bulk_get probes (90% Zipf(1.0) hits, 10% misses)",
+                   "parallelism": "single GPU",
+                   "timing": "per op: kernel events on the map's stream summed (device_ms) and wall clock around "
+                             "the synchronised call (wall_ms); medians over the timed steps"},
+        "roofline": {"bound": "hbm", "op": "probe", "achieved": probe["achieved_gbs"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": probe["frac"], "traffic": probe.get("traffic_bytes")},
+        "stream_copy": copy,
+        "ops": ops,
+        "get_autotune": get_tune,
+        "cpu_baseline": cpu,
+        "check": check,
+    }
+    print(json.dumps(out_line), flush=True)
 
 
 # ---------------------------------------------------------------------------
