@@ -42,8 +42,9 @@ struct Ops {
   hipError_t (*ins_replay)(const Table&, const uint8_t*, const uint8_t*, uint64_t, const uint32_t*, const uint32_t*,
                            Counters*, uint32_t, hipStream_t);
   hipError_t (*rem_probe)(const Table&, const uint8_t*, uint64_t, unsigned long long*, uint32_t, uint32_t, hipStream_t);
-  hipError_t (*rem_runs)(const Table&, const uint8_t*, uint64_t, uint32_t*, uint32_t*, uint32_t, hipStream_t);
-  hipError_t (*rem_replay)(const Table&, const uint8_t*, uint64_t, const uint32_t*, const uint32_t*, Counters*,
+  hipError_t (*rem_runs)(const Table&, const uint8_t*, uint64_t, uint32_t*, uint32_t*, const Counters*, uint32_t,
+                         hipStream_t);
+  hipError_t (*rem_replay)(const Table&, const uint8_t*, uint64_t, const uint32_t*, const uint32_t*, Counters*, bool,
                            uint32_t, hipStream_t);
   hipError_t (*ins_fused)(const Table&, const dev::FusedInsert&, uint32_t, hipStream_t);
   hipError_t (*ins_dedup)(const Table&, const uint8_t*, uint64_t, uint32_t, hipStream_t);
@@ -132,16 +133,17 @@ hipError_t launch_rem_probe(const Table& t, const uint8_t* q, uint64_t n, unsign
 }
 
 template <int KS>
-hipError_t launch_rem_runs(const Table& t, const uint8_t* q, uint64_t n, uint32_t* cid, uint32_t* idx, uint32_t block,
-                           hipStream_t s) {
-  hipLaunchKernelGGL((dev::k_rem_runs<KS>), dim3(grid_for(n, block)), dim3(block), 0, s, t, q, n, cid, idx);
+hipError_t launch_rem_runs(const Table& t, const uint8_t* q, uint64_t n, uint32_t* cid, uint32_t* idx,
+                           const Counters* gate, uint32_t block, hipStream_t s) {
+  hipLaunchKernelGGL((dev::k_rem_runs<KS>), dim3(grid_for(n, block)), dim3(block), 0, s, t, q, n, cid, idx, gate);
   return hipGetLastError();
 }
 
 template <int KS>
 hipError_t launch_rem_replay(const Table& t, const uint8_t* q, uint64_t n, const uint32_t* cid, const uint32_t* idx,
-                             Counters* c, uint32_t block, hipStream_t s) {
-  hipLaunchKernelGGL((dev::k_rem_replay<KS>), dim3(grid_for(n, block)), dim3(block), 0, s, t, q, n, cid, idx, c);
+                             Counters* c, bool gated, uint32_t block, hipStream_t s) {
+  hipLaunchKernelGGL((dev::k_rem_replay<KS>), dim3(grid_for(n, block)), dim3(block), 0, s, t, q, n, cid, idx, c,
+                     gated);
   return hipGetLastError();
 }
 
@@ -272,13 +274,12 @@ class GpuBackend final : public Backend {
     if ((st = use_device()) != FM_OK || (st = ensure_scratch(max_keys)) != FM_OK) return st;
     return ensure_dedup(max_keys);
   }
-  fm_status serve_remove(const void* d_keys, uint64_t n, uint8_t* d_removed, uint64_t* removed) override {
+  fm_status serve_remove(const void* d_keys, uint64_t n, uint8_t* d_removed) override {
     fm_status st;
     if ((st = use_device()) != FM_OK) return st;
-    if ((st = run_remove(static_cast<const uint8_t*>(d_keys), n, removed, d_removed)) != FM_OK) return st;
-    len_ -= std::min(len_, *removed);
-    return FM_OK;
+    return run_remove(static_cast<const uint8_t*>(d_keys), n, nullptr, d_removed, true);
   }
+  void serve_settle_remove(uint64_t removed) override { len_ -= std::min(len_, removed); }
   fm_status profile_enable(bool on) override;
   fm_status profile_read(fm_kernel_stat* out, uint32_t max, uint32_t* count) override;
   fm_status get_tuning(fm_tuning* out) const override {
@@ -336,7 +337,10 @@ class GpuBackend final : public Backend {
   fm_status run_get(const uint8_t* d_keys, uint64_t n, uint8_t* d_out, uint8_t* d_found);
   fm_status run_insert(const uint8_t* d_keys, const uint8_t* d_vals, uint64_t n,
                        uint8_t* outcome = nullptr);  // leaves count in ctr
-  fm_status run_remove(const uint8_t* d_keys, uint64_t n, uint64_t* removed, uint8_t* outcome = nullptr);
+  // async: no status readback and no count (serving reads the outcome
+  // flags); the conflict replay is gated on the device.
+  fm_status run_remove(const uint8_t* d_keys, uint64_t n, uint64_t* removed, uint8_t* outcome = nullptr,
+                       bool async = false);
 
   void release();
 
@@ -795,7 +799,8 @@ fm_status GpuBackend::run_insert(const uint8_t* d_keys, const uint8_t* d_vals, u
   return check_launch(hipGetLastError(), "k_dedup_reset");
 }
 
-fm_status GpuBackend::run_remove(const uint8_t* d_keys, uint64_t n, uint64_t* removed, uint8_t* outcome) {
+fm_status GpuBackend::run_remove(const uint8_t* d_keys, uint64_t n, uint64_t* removed, uint8_t* outcome,
+                                 bool async) {
   fm_status st;
   Table t = t_;
   t.outcome = outcome;
@@ -822,6 +827,25 @@ fm_status GpuBackend::run_remove(const uint8_t* d_keys, uint64_t n, uint64_t* re
     hipLaunchKernelGGL(dev::k_rem_apply, dim3(grid_for(n, blk)), dim3(blk), 0, stream_, t, ev_, action_, n, ctr_);
     if ((st = check_launch(hipGetLastError(), "k_rem_apply")) != FM_OK) return st;
   }
+  if (async) {
+    // No status readback: the replay steps are launched unconditionally and
+    // run only if k_rem_check flagged a conflict (k_rem_apply then left the
+    // table untouched).  Removed counts come from the outcome flags.
+    {
+      Prof p(this, kKidRemRuns);
+      if ((st = check_launch(ops_.rem_runs(t_, d_keys, n, cid_, idx_, ctr_, blk, stream_), "k_rem_runs")) != FM_OK)
+        return st;
+    }
+    {
+      Prof p(this, kKidRemSort);
+      size_t tmp = sort_tmp_bytes_;
+      hipError_t e = hipcub::DeviceRadixSort::SortPairs(sort_tmp_, tmp, cid_, cid_s_, idx_, idx_s_,
+                                                        static_cast<int>(n), 0, sort_end_bit_, stream_);
+      if (e != hipSuccess) return fail(FM_ERR_KERNEL_LAUNCH, hip_msg("radix sort", e));
+    }
+    Prof p(this, kKidRemReplay);
+    return check_launch(ops_.rem_replay(t, d_keys, n, cid_s_, idx_s_, ctr_, true, blk, stream_), "k_rem_replay");
+  }
   Counters c;
   if ((st = read_counters(&c)) != FM_OK) return st;
   if (c.status & dev::kStatusConflict) {
@@ -830,7 +854,7 @@ fm_status GpuBackend::run_remove(const uint8_t* d_keys, uint64_t n, uint64_t* re
     if ((st = zero_counters()) != FM_OK) return st;
     {
       Prof p(this, kKidRemRuns);
-      if ((st = check_launch(ops_.rem_runs(t_, d_keys, n, cid_, idx_, blk, stream_), "k_rem_runs")) != FM_OK)
+      if ((st = check_launch(ops_.rem_runs(t_, d_keys, n, cid_, idx_, nullptr, blk, stream_), "k_rem_runs")) != FM_OK)
         return st;
     }
     {
@@ -842,7 +866,7 @@ fm_status GpuBackend::run_remove(const uint8_t* d_keys, uint64_t n, uint64_t* re
     }
     {
       Prof p(this, kKidRemReplay);
-      if ((st = check_launch(ops_.rem_replay(t, d_keys, n, cid_s_, idx_s_, ctr_, blk, stream_), "k_rem_replay")) !=
+      if ((st = check_launch(ops_.rem_replay(t, d_keys, n, cid_s_, idx_s_, ctr_, false, blk, stream_), "k_rem_replay")) !=
           FM_OK)
         return st;
     }

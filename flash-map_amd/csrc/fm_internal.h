@@ -53,15 +53,18 @@ class Backend {
   virtual fm_status copy_from(Backend&, uint32_t /*parts*/) { return gpu_required(); }
 
   // Serving-loop hooks (GPU only).  Device buffers; work is ordered on the
-  // map's stream.  serve_insert raises len by n (an upper bound, so len never
-  // undercounts occupied slots) and serve_settle_insert lowers it by the
-  // keys that did not claim a slot once their outcome flags are known.
+  // map's stream and none of them waits for it.  serve_insert raises len by
+  // n (an upper bound, so len never undercounts occupied slots) and
+  // serve_settle_insert lowers it by the keys that did not claim a slot once
+  // their outcome flags are known; serve_remove leaves len alone and
+  // serve_settle_remove lowers it by the removed count read from the flags.
   virtual fm_status serve_get(const void*, uint64_t, void*, uint8_t*) { return gpu_required(); }
   virtual fm_status serve_insert(const void*, const void*, uint64_t, uint8_t*) { return gpu_required(); }
   virtual void serve_settle_insert(uint64_t, uint64_t) {}
   virtual void serve_mode(bool) {}  // inside fm_serve_log: inserts deduplicate hot keys
   virtual fm_status serve_reserve(uint64_t /*max_keys*/) { return FM_OK; }  // per-batch scratch up front
-  virtual fm_status serve_remove(const void*, uint64_t, uint8_t*, uint64_t*) { return gpu_required(); }
+  virtual fm_status serve_remove(const void*, uint64_t, uint8_t*) { return gpu_required(); }
+  virtual void serve_settle_remove(uint64_t) {}
   virtual fm_status profile_enable(bool) { return gpu_required(); }
   virtual fm_status profile_read(fm_kernel_stat*, uint32_t, uint32_t*) { return gpu_required(); }
   virtual fm_status get_tuning(fm_tuning*) const { return gpu_required(); }

@@ -2155,10 +2155,13 @@ k_rem_apply(Table t, const unsigned long long* __restrict__ ev, const uint8_t* _
 
 // Exact replay, step 1: run id = first EMPTY slot at/after home (removes
 // never cross or create EMPTY slots).  No EMPTY slot: one run.
+// gate (asynchronous removes): when set, the replay steps run only if the
+// check pass found a conflict, so the host need not read the status first.
 template <int KS>
 __global__ void __launch_bounds__(1024)
 k_rem_runs(Table t, const uint8_t* __restrict__ qkeys, uint64_t n, uint32_t* __restrict__ cid,
-           uint32_t* __restrict__ idx) {
+           uint32_t* __restrict__ idx, const Counters* gate) {
+  if (gate && !(gate->status & kStatusConflict)) return;
   const uint32_t K = KS ? KS : t.K;
   for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
        i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
@@ -2199,7 +2202,8 @@ __device__ inline uint32_t remove_one(const Table& t, const uint8_t* __restrict_
 template <int KS>
 __global__ void __launch_bounds__(1024)
 k_rem_replay(Table t, const uint8_t* __restrict__ qkeys, uint64_t n, const uint32_t* __restrict__ cid_s,
-             const uint32_t* __restrict__ idx_s, Counters* ctr) {
+             const uint32_t* __restrict__ idx_s, Counters* ctr, bool gated) {
+  if (gated && !(ctr->status & kStatusConflict)) return;
   const uint32_t K = KS ? KS : t.K;
   unsigned long long removed = 0;
   for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * blockDim.x;; base += static_cast<uint64_t>(gridDim.x) * blockDim.x) {

@@ -18,8 +18,11 @@
 // (read-only); adjacent inserts (resp. removes) applied as one batch in index
 // order equal applying them one after another.  TableFull is decided per
 // insert request with the table length before it: len is an upper bound
-// while inserts are in flight, and a request that might not fit waits for
-// them to settle so the exact length decides (rayon_cpu.rs:134-142).
+// while inserts or removes are in flight (an insert raises it by its batch
+// size, a remove lowers it only once its outcome flags are back), and a
+// request that might not fit waits for them to settle so the exact length
+// decides (rayon_cpu.rs:134-142).  No op waits for the device at dispatch:
+// a remove's conflict replay is gated on the device.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -28,6 +31,7 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <map>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -39,6 +43,23 @@ namespace fm {
 namespace {
 
 using Clock = std::chrono::steady_clock;
+
+// Allocations made by fm_host_alloc (base -> bytes).  A request buffer that
+// lies inside one of them is pinned: the DMA engines read or write it
+// directly, with no copy through the pinned staging ring.
+std::mutex g_pinned_mu;
+std::map<uintptr_t, uint64_t> g_pinned;
+
+bool in_pinned(const void* p, uint64_t bytes) {
+  if (bytes == 0) return true;  // nothing to move
+  if (!p) return false;
+  const uintptr_t a = reinterpret_cast<uintptr_t>(p);
+  std::lock_guard<std::mutex> l(g_pinned_mu);
+  auto it = g_pinned.upper_bound(a);
+  if (it == g_pinned.begin()) return false;
+  --it;
+  return a - it->first <= it->second && bytes <= it->second - (a - it->first);
+}
 
 inline uint64_t align256(uint64_t b) { return (b + 255) & ~uint64_t(255); }
 
@@ -78,9 +99,10 @@ class Ring {
   std::deque<uint64_t> live_;
 };
 
-// Host copy workers.  Request buffers are ordinary (pageable) memory: one
-// thread moves them into and out of the pinned staging ring at a few GB/s,
-// so large copies are cut into pieces that the workers and the caller share.
+// Host copy workers.  Request buffers in ordinary (pageable) memory go
+// through the pinned staging ring; one thread moves them at a few GB/s, so
+// large copies are cut into pieces that the workers and the caller share.
+// Buffers from fm_host_alloc skip this: the DMA engines move them directly.
 class CopyPool {
  public:
   struct Span {
@@ -169,12 +191,18 @@ struct Batch {
   uint64_t keys_at = 0, vals_at = 0, out_at = 0, found_at = 0, flags_at = 0;
   hipEvent_t h2d = nullptr, done = nullptr, d2h = nullptr;
   int32_t id = 0;
+  bool direct_out = false;  // every request's outputs are pinned: DMA straight into them
 };
 
 class Server {
  public:
   Server(fm_map* m, fm_serve_request* reqs, uint64_t n, const fm_serve_config& cfg)
-      : m_(m), reqs_(reqs), n_(n), cfg_(cfg), ring_(cfg.staging_bytes), copies_(kCopyWorkers) {}
+      : m_(m),
+        reqs_(reqs),
+        n_(n),
+        cfg_(cfg),
+        ring_(cfg.staging_bytes),
+        copies_(kCopyWorkers) {}
   ~Server();
   fm_status run(fm_serve_stats* stats);
 
@@ -209,10 +237,12 @@ class Server {
   uint64_t next_ = 0;       // next request to dispatch
   std::deque<Batch> inflight_;
   uint64_t inflight_inserts_ = 0;
+  uint64_t inflight_removes_ = 0;
   bool stop_ = false;
   fm_status async_err_ = FM_OK;
   uint64_t batches_ = 0, max_inflight_ = 0, staging_waits_ = 0;
-  static constexpr unsigned kCopyWorkers = 6;
+  std::vector<uint8_t> pin_in_, pin_out_;  // per request: inputs / outputs pinned
+  static constexpr unsigned kCopyWorkers = 10;
   CopyPool copies_;
 };
 
@@ -250,6 +280,14 @@ fm_status Server::setup() {
     r.status = FM_OK;
     r.batch = -1;
     r.admit_s = r.done_s = 0;
+  }
+  pin_in_.assign(n_, 0);
+  pin_out_.assign(n_, 0);
+  for (uint64_t i = 0; i < n_; ++i) {
+    const fm_serve_request& r = reqs_[i];
+    pin_in_[i] = in_pinned(r.keys, r.n * K_) && (r.op != FM_OP_INSERT || in_pinned(r.values, r.n * V_));
+    pin_out_[i] = r.op == FM_OP_GET ? (r.out_found && in_pinned(r.out_values, r.n * V_) && in_pinned(r.out_found, r.n))
+                                    : (r.out_flags && in_pinned(r.out_flags, r.n));
   }
   // Size the map's per-batch scratch for the largest batch now, not mid-run.
   uint64_t largest = 0;
@@ -303,8 +341,8 @@ fm_status Server::dispatch_one(uint64_t first) {
       if (op == FM_OP_INSERT && len_upper + b.n + n > cap_) {
         if (!b.reqs.empty()) break;  // decide this one after the batch
         // The table length before this request is needed exactly: wait for
-        // every in-flight insert to settle.
-        cv_.wait(l, [&] { return inflight_inserts_ == 0 || stop_; });
+        // every in-flight insert and remove to settle.
+        cv_.wait(l, [&] { return (inflight_inserts_ == 0 && inflight_removes_ == 0) || stop_; });
         {
           std::lock_guard<std::mutex> ml(m_->mu);
           len_upper = be.len();
@@ -328,6 +366,7 @@ fm_status Server::dispatch_one(uint64_t first) {
       b.off = off;
       next_ = b.reqs.back() + 1;
       if (op == FM_OP_INSERT) ++inflight_inserts_;
+      if (op == FM_OP_REMOVE) ++inflight_removes_;
       break;
     }
     ++staging_waits_;
@@ -344,13 +383,19 @@ fm_status Server::dispatch_one(uint64_t first) {
   (op == FM_OP_GET ? b.found_at : b.flags_at) = at;
   const double admit = now();
   uint64_t pos = 0;
+  bool any_pinned_in = false;
+  b.direct_out = true;
   std::vector<CopyPool::Span> spans;
   for (uint64_t i : b.reqs) {
     fm_serve_request& r = reqs_[i];
     r.admit_s = admit;
-    spans.push_back({h_stage_ + b.keys_at + pos * K_, static_cast<const uint8_t*>(r.keys), r.n * K_});
-    if (op == FM_OP_INSERT)
-      spans.push_back({h_stage_ + b.vals_at + pos * V_, static_cast<const uint8_t*>(r.values), r.n * V_});
+    any_pinned_in = any_pinned_in || pin_in_[i];
+    b.direct_out = b.direct_out && pin_out_[i];
+    if (!pin_in_[i]) {
+      spans.push_back({h_stage_ + b.keys_at + pos * K_, static_cast<const uint8_t*>(r.keys), r.n * K_});
+      if (op == FM_OP_INSERT)
+        spans.push_back({h_stage_ + b.vals_at + pos * V_, static_cast<const uint8_t*>(r.values), r.n * V_});
+    }
     pos += r.n;
   }
   copies_.run(spans);
@@ -360,13 +405,27 @@ fm_status Server::dispatch_one(uint64_t first) {
       (e = hipEventCreateWithFlags(&b.done, hipEventDisableTiming)) != hipSuccess ||
       (e = hipEventCreateWithFlags(&b.d2h, hipEventDisableTiming)) != hipSuccess)
     return fail(FM_ERR_KERNEL_LAUNCH, std::string("serve events: ") + hipGetErrorString(e));
-  const uint64_t in_bytes = op == FM_OP_INSERT ? (b.vals_at + b.n * V_ - b.keys_at) : b.n * K_;
-  if ((e = hipMemcpyAsync(d_stage_ + b.keys_at, h_stage_ + b.keys_at, in_bytes, hipMemcpyHostToDevice, h2d_)) !=
-          hipSuccess ||
-      (e = hipEventRecord(b.h2d, h2d_)) != hipSuccess || (e = hipStreamWaitEvent(compute_, b.h2d, 0)) != hipSuccess)
+  if (!any_pinned_in) {  // the whole input region from the staging ring in one copy
+    const uint64_t in_bytes = op == FM_OP_INSERT ? (b.vals_at + b.n * V_ - b.keys_at) : b.n * K_;
+    e = hipMemcpyAsync(d_stage_ + b.keys_at, h_stage_ + b.keys_at, in_bytes, hipMemcpyHostToDevice, h2d_);
+  } else {  // per request: straight from a pinned buffer, else from the staging ring
+    pos = 0;
+    for (uint64_t i : b.reqs) {
+      const fm_serve_request& r = reqs_[i];
+      const uint8_t* kin = pin_in_[i] ? static_cast<const uint8_t*>(r.keys) : h_stage_ + b.keys_at + pos * K_;
+      if (e == hipSuccess && r.n)
+        e = hipMemcpyAsync(d_stage_ + b.keys_at + pos * K_, kin, r.n * K_, hipMemcpyHostToDevice, h2d_);
+      if (e == hipSuccess && r.n && op == FM_OP_INSERT) {
+        const uint8_t* vin = pin_in_[i] ? static_cast<const uint8_t*>(r.values) : h_stage_ + b.vals_at + pos * V_;
+        e = hipMemcpyAsync(d_stage_ + b.vals_at + pos * V_, vin, r.n * V_, hipMemcpyHostToDevice, h2d_);
+      }
+      pos += r.n;
+    }
+  }
+  if (e != hipSuccess || (e = hipEventRecord(b.h2d, h2d_)) != hipSuccess ||
+      (e = hipStreamWaitEvent(compute_, b.h2d, 0)) != hipSuccess)
     return fail(FM_ERR_TRANSFER, std::string("serve H2D: ") + hipGetErrorString(e));
   fm_status st;
-  uint64_t removed = 0;
   {
     std::lock_guard<std::mutex> ml(m_->mu);
     if (op == FM_OP_GET)
@@ -374,15 +433,32 @@ fm_status Server::dispatch_one(uint64_t first) {
     else if (op == FM_OP_INSERT)
       st = be.serve_insert(d_stage_ + b.keys_at, d_stage_ + b.vals_at, b.n, d_stage_ + b.flags_at);
     else
-      st = be.serve_remove(d_stage_ + b.keys_at, b.n, d_stage_ + b.flags_at, &removed);
+      st = be.serve_remove(d_stage_ + b.keys_at, b.n, d_stage_ + b.flags_at);
   }
   if (st != FM_OK) return st;
   const uint64_t out_from = op == FM_OP_GET ? b.out_at : b.flags_at;
   const uint64_t out_bytes = op == FM_OP_GET ? (b.found_at + b.n - b.out_at) : b.n;
-  if ((e = hipEventRecord(b.done, compute_)) != hipSuccess || (e = hipStreamWaitEvent(d2h_, b.done, 0)) != hipSuccess ||
-      (e = hipMemcpyAsync(h_stage_ + out_from, d_stage_ + out_from, out_bytes, hipMemcpyDeviceToHost, d2h_)) !=
-          hipSuccess ||
-      (e = hipEventRecord(b.d2h, d2h_)) != hipSuccess)
+  if ((e = hipEventRecord(b.done, compute_)) == hipSuccess) e = hipStreamWaitEvent(d2h_, b.done, 0);
+  if (!b.direct_out) {  // into the staging ring; the completion thread scatters
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(h_stage_ + out_from, d_stage_ + out_from, out_bytes, hipMemcpyDeviceToHost, d2h_);
+  } else {  // straight into each request's pinned outputs
+    pos = 0;
+    for (uint64_t i : b.reqs) {
+      const fm_serve_request& r = reqs_[i];
+      if (e == hipSuccess && r.n) {
+        if (op == FM_OP_GET) {
+          e = hipMemcpyAsync(r.out_values, d_stage_ + b.out_at + pos * V_, r.n * V_, hipMemcpyDeviceToHost, d2h_);
+          if (e == hipSuccess)
+            e = hipMemcpyAsync(r.out_found, d_stage_ + b.found_at + pos, r.n, hipMemcpyDeviceToHost, d2h_);
+        } else {
+          e = hipMemcpyAsync(r.out_flags, d_stage_ + b.flags_at + pos, r.n, hipMemcpyDeviceToHost, d2h_);
+        }
+      }
+      pos += r.n;
+    }
+  }
+  if (e != hipSuccess || (e = hipEventRecord(b.d2h, d2h_)) != hipSuccess)
     return fail(FM_ERR_TRANSFER, std::string("serve D2H: ") + hipGetErrorString(e));
   std::lock_guard<std::mutex> l(mu_);
   inflight_.push_back(b);
@@ -401,9 +477,10 @@ void Server::completion_loop() {
       b = inflight_.front();
     }
     hipError_t e = hipEventSynchronize(b.d2h);
-    uint64_t pos = 0, claimed = 0;
+    uint64_t pos = 0, settled = 0;
     std::vector<CopyPool::Span> spans;
     for (uint64_t i : b.reqs) {
+      if (b.direct_out) break;  // the DMA wrote every request's outputs
       fm_serve_request& r = reqs_[i];
       if (b.op == FM_OP_GET) {
         spans.push_back({static_cast<uint8_t*>(r.out_values), h_stage_ + b.out_at + pos * V_, r.n * V_});
@@ -419,9 +496,10 @@ void Server::completion_loop() {
     for (uint64_t i : b.reqs) {
       fm_serve_request& r = reqs_[i];
       uint64_t count = 0;
-      const uint8_t* f = h_stage_ + (b.op == FM_OP_GET ? b.found_at : b.flags_at) + pos;
+      const uint8_t* f = b.direct_out ? (b.op == FM_OP_GET ? r.out_found : r.out_flags)
+                                      : h_stage_ + (b.op == FM_OP_GET ? b.found_at : b.flags_at) + pos;
       for (uint64_t k = 0; k < r.n; ++k) count += f[k];
-      if (b.op == FM_OP_INSERT) claimed += count;
+      settled += count;
       r.count = count;
       r.batch = b.id;
       r.done_s = done;
@@ -429,7 +507,10 @@ void Server::completion_loop() {
     }
     if (b.op == FM_OP_INSERT) {
       std::lock_guard<std::mutex> ml(m_->mu);
-      m_->be->serve_settle_insert(b.n, claimed);
+      m_->be->serve_settle_insert(b.n, settled);
+    } else if (b.op == FM_OP_REMOVE) {
+      std::lock_guard<std::mutex> ml(m_->mu);
+      m_->be->serve_settle_remove(settled);
     }
     (void)hipEventDestroy(b.h2d);
     (void)hipEventDestroy(b.done);
@@ -442,6 +523,7 @@ void Server::completion_loop() {
     inflight_.pop_front();
     ring_.free_front();
     if (b.op == FM_OP_INSERT) --inflight_inserts_;
+    if (b.op == FM_OP_REMOVE) --inflight_removes_;
     cv_.notify_all();
   }
 }
@@ -523,4 +605,42 @@ extern "C" fm_status fm_serve_log(fm_map* m, fm_serve_request* reqs, uint64_t n_
   }
   fm::Server s(m, reqs, n_reqs, *cfg);
   return s.run(stats);
+}
+
+extern "C" fm_status fm_host_alloc(uint64_t bytes, void** out) {
+  if (!out || bytes == 0) {
+    fm::set_error(FM_ERR_INVALID_ARGUMENT, "fm_host_alloc: bad arguments");
+    return FM_ERR_INVALID_ARGUMENT;
+  }
+  void* p = nullptr;
+  const hipError_t e = hipHostMalloc(&p, bytes, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    fm::set_error(FM_ERR_GPU_ALLOC, std::string("fm_host_alloc: ") + hipGetErrorString(e));
+    return FM_ERR_GPU_ALLOC;
+  }
+  {
+    std::lock_guard<std::mutex> l(fm::g_pinned_mu);
+    fm::g_pinned[reinterpret_cast<uintptr_t>(p)] = bytes;
+  }
+  *out = p;
+  return FM_OK;
+}
+
+extern "C" fm_status fm_host_free(void* p) {
+  if (!p) return FM_OK;
+  {
+    std::lock_guard<std::mutex> l(fm::g_pinned_mu);
+    auto it = fm::g_pinned.find(reinterpret_cast<uintptr_t>(p));
+    if (it == fm::g_pinned.end()) {
+      fm::set_error(FM_ERR_INVALID_ARGUMENT, "fm_host_free: not a pointer from fm_host_alloc");
+      return FM_ERR_INVALID_ARGUMENT;
+    }
+    fm::g_pinned.erase(it);
+  }
+  const hipError_t e = hipHostFree(p);
+  if (e != hipSuccess) {
+    fm::set_error(FM_ERR_GPU_ALLOC, std::string("fm_host_free: ") + hipGetErrorString(e));
+    return FM_ERR_GPU_ALLOC;
+  }
+  return FM_OK;
 }

@@ -174,6 +174,8 @@ def lib() -> ctypes.CDLL:
             "fm_copy_table": (ctypes.c_int, [vp, vp, u32]),
             "fm_serve_log": (ctypes.c_int, [vp, ctypes.POINTER(ServeRequest), u64, ctypes.POINTER(ServeConfig),
                                             ctypes.POINTER(ServeStats)]),
+            "fm_host_alloc": (ctypes.c_int, [u64, ctypes.POINTER(vp)]),
+            "fm_host_free": (ctypes.c_int, [vp]),
             "fm_group_create": (ctypes.c_int, [u32, pu64, u32, u32, u32, i32, ctypes.POINTER(vp)]),
             "fm_group_destroy": (None, [vp]),
             "fm_group_tables": (u32, [vp]),
@@ -479,11 +481,17 @@ class FlashMap:
 class ServeResult:
     """Outputs of one served request."""
 
-    def __init__(self, op, n, value_size):
+    def __init__(self, op, n, value_size, arena=None):
+        # Outputs come from a pinned arena (DMA writes them directly), or are
+        # allocated here with np.full, which writes every page now: np.zeros
+        # would map untouched pages that the server's result copy then faults
+        # in one by one inside the serving window.
         self.op = op
-        self.values = np.zeros((n, value_size), dtype=np.uint8) if op == OP_GET else None
-        self.found = np.zeros(n, dtype=np.uint8) if op == OP_GET else None
-        self.flags = np.zeros(n, dtype=np.uint8) if op != OP_GET else None
+        self._arena = arena
+        take = arena.take if arena is not None else (lambda shape: np.full(shape, 0, dtype=np.uint8))
+        self.values = take((n, value_size)) if op == OP_GET else None
+        self.found = take(n) if op == OP_GET else None
+        self.flags = take(n) if op != OP_GET else None
         self.count = 0
         self.status = OK
         self.batch = -1
@@ -494,20 +502,80 @@ class ServeResult:
         return (self.done_s - self.arrival_s) * 1e3
 
 
+class PinnedArena:
+    """One pinned host allocation (fm_host_alloc) carved into numpy arrays.
+    Serving requests whose buffers lie in it are moved by DMA, without the
+    host copy through the staging ring."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = max(int(nbytes), 256)
+        p = ctypes.c_void_p()
+        _check(lib().fm_host_alloc(self.nbytes, ctypes.byref(p)))
+        self.ptr = p.value
+        self.buf = np.frombuffer((ctypes.c_uint8 * self.nbytes).from_address(self.ptr), dtype=np.uint8)
+        self.used = 0
+
+    def take(self, shape, fill=0) -> np.ndarray:
+        size = int(np.prod(shape))
+        start = (self.used + 255) & ~255
+        if start + size > self.nbytes:
+            raise ValueError("pinned arena exhausted")
+        self.used = start + size
+        a = self.buf[start:start + size].reshape(shape)
+        if fill is not None:
+            a[...] = fill
+        return a
+
+    def copy_of(self, a: np.ndarray) -> np.ndarray:
+        out = self.take(a.shape, fill=None)
+        out[...] = a
+        return out
+
+    def close(self):
+        if self.ptr:
+            self.buf = None
+            lib().fm_host_free(ctypes.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def serve_log(fmap: "FlashMap", requests, staging_bytes: int = 256 << 20, max_batch_keys: int = 1 << 20,
-              realtime: bool = True):
+              realtime: bool = True, pinned=False):
     """Runs a request log through the continuous-batching server
     (fm_serve_log).  `requests`: objects with .op ("get"|"insert"|"remove"),
     .keys ((n, K) bytes), .vals ((n,) or (n, V); inserts) and .arrival_s.
+    pinned: False (request buffers in pageable memory, copied through the
+    staging ring), True (inputs copied into a pinned arena before serving
+    starts and outputs allocated there, so the server moves both by DMA),
+    "inputs" or "outputs" (only that side pinned).
     Returns ([ServeResult], ServeStats)."""
     ops = {"get": OP_GET, "insert": OP_INSERT, "remove": OP_REMOVE}
     n = len(requests)
     arr = (ServeRequest * max(n, 1))()
     keep, results = [], []
+    pin_in = pinned in (True, "inputs")
+    pin_out = pinned in (True, "outputs")
+    arena = None
+    if pin_in or pin_out:
+        need = 0
+        for r in requests:
+            nk = _as_bytes(r.keys, fmap.key_size, "keys")[1]
+            if pin_in:
+                need += nk * fmap.key_size + 256 + (nk * fmap.value_size + 256 if r.op == "insert" else 0)
+            if pin_out:
+                need += nk * (fmap.value_size + 1) + 512 if r.op == "get" else nk + 256
+        arena = PinnedArena(need)
     for i, r in enumerate(requests):
         op = ops[r.op]
         k, nk = _as_bytes(r.keys, fmap.key_size, "keys")
-        res = ServeResult(op, nk, fmap.value_size)
+        if pin_in:
+            k = arena.copy_of(k)
+        res = ServeResult(op, nk, fmap.value_size, arena if pin_out else None)
         res.arrival_s = float(r.arrival_s) if realtime else 0.0
         a = arr[i]
         a.op, a.n, a.keys, a.arrival_s = op, nk, k.ctypes.data if nk else None, float(r.arrival_s)
@@ -515,6 +583,8 @@ def serve_log(fmap: "FlashMap", requests, staging_bytes: int = 256 << 20, max_ba
         if op == OP_INSERT:
             v, nv = _as_bytes(r.vals, fmap.value_size, "values")
             assert nv == nk
+            if pin_in:
+                v = arena.copy_of(v)
             a.values = v.ctypes.data if nv else None
             keep.append(v)
         if op == OP_GET:
@@ -530,7 +600,7 @@ def serve_log(fmap: "FlashMap", requests, staging_bytes: int = 256 << 20, max_ba
         a = arr[i]
         res.count, res.status, res.batch = int(a.count), int(a.status), int(a.batch)
         res.admit_s, res.done_s = float(a.admit_s), float(a.done_s)
-    del keep
+    del keep  # a pinned input arena stays alive through the results that point into it
     return results, stats
 
 

@@ -280,6 +280,15 @@ typedef struct fm_serve_stats {
 fm_status fm_serve_log(fm_map* m, fm_serve_request* reqs, uint64_t n_reqs,
                        const fm_serve_config* cfg, fm_serve_stats* stats);
 
+/* Pinned host memory for serving requests (no reference counterpart).  A
+ * request buffer (keys, values, outputs) that lies inside one allocation
+ * from fm_host_alloc is moved by DMA directly between it and the device
+ * staging ring; other buffers go through the pinned staging ring by a host
+ * copy.  Results are the same either way.  fm_host_free takes exactly a
+ * pointer returned by fm_host_alloc. */
+fm_status fm_host_alloc(uint64_t bytes, void** out);
+fm_status fm_host_free(void* p);
+
 /* ---- table groups: many independent maps, fused launches ---------------
  * No reference counterpart (the reference launches once per call per map,
  * gpu.rs:141-167).  A group holds n independent tables with one key/value

@@ -78,16 +78,20 @@ def _check(results, expected, what):
             assert np.array_equal(res.values, ev), f"{what}: request {i} values"
 
 
-@pytest.mark.parametrize("realtime,rate,staging", [(False, 1.0, 64 << 20), (True, 20_000.0, 64 << 20),
-                                                   (False, 1.0, 1 << 20)],
-                         ids=["burst", "poisson20k", "tiny-staging"])
-def test_serve_log_matches_sequential_replay(realtime, rate, staging):
+@pytest.mark.parametrize("realtime,rate,staging,pinned", [
+    (False, 1.0, 64 << 20, False), (True, 20_000.0, 64 << 20, False), (False, 1.0, 1 << 20, False),
+    (False, 1.0, 64 << 20, True), (True, 20_000.0, 64 << 20, True), (False, 1.0, 1 << 20, "inputs"),
+    (False, 1.0, 64 << 20, "outputs")],
+    ids=["burst", "poisson20k", "tiny-staging", "burst-pinned", "poisson20k-pinned", "tiny-staging-pinned-inputs",
+         "burst-pinned-outputs"])
+def test_serve_log_matches_sequential_replay(realtime, rate, staging, pinned):
     cap = 1 << 16
     pre, pre_v, reqs = _log(5, cap, 30_000, 300, 4096, rate)
-    say(f"serve realtime={realtime} rate={rate} staging={staging}")
+    say(f"serve realtime={realtime} rate={rate} staging={staging} pinned={pinned}")
     m = fm.FlashMap.builder(cap, K, V).require_gpu().build()
     m.bulk_insert(pre, pre_v)
-    results, stats = fm.serve_log(m, reqs, staging_bytes=staging, max_batch_keys=1 << 14, realtime=realtime)
+    results, stats = fm.serve_log(m, reqs, staging_bytes=staging, max_batch_keys=1 << 14, realtime=realtime,
+                                  pinned=pinned)
     o, expected = _expected(cap, pre, pre_v, reqs)
     _check(results, expected, "served")
     assert len(m) == len(o)
@@ -97,17 +101,19 @@ def test_serve_log_matches_sequential_replay(realtime, rate, staging):
         assert stats.batches < len(reqs), "adjacent same-kind requests should coalesce"
 
 
-def test_serve_alone_equals_in_the_mix_and_table_full():
+@pytest.mark.parametrize("pinned", [False, True], ids=["pageable", "pinned"])
+def test_serve_alone_equals_in_the_mix_and_table_full(pinned):
     """Each request run alone through the bulk API equals its served result;
     the log drives the table to TableFull so that path is exercised too."""
     cap = 1 << 13
     pre, pre_v, reqs = _log(7, cap, 5_000, 200, 1024, 1.0, p_get=0.4, p_ins=0.5)
-    say("serve alone vs mix")
+    say(f"serve alone vs mix pinned={pinned}")
     served_map = fm.FlashMap.builder(cap, K, V).require_gpu().build()
     alone_map = fm.FlashMap.builder(cap, K, V).require_gpu().build()
     served_map.bulk_insert(pre, pre_v)
     alone_map.bulk_insert(pre, pre_v)
-    results, _ = fm.serve_log(served_map, reqs, staging_bytes=8 << 20, max_batch_keys=4096, realtime=False)
+    results, _ = fm.serve_log(served_map, reqs, staging_bytes=8 << 20, max_batch_keys=4096, realtime=False,
+                              pinned=pinned)
     saw_full = False
     for i, (r, res) in enumerate(zip(reqs, results)):
         if r.op == "get":
@@ -123,3 +129,19 @@ def test_serve_alone_equals_in_the_mix_and_table_full():
             assert res.count == alone_map.bulk_remove(r.keys), f"request {i}"
     assert saw_full, "log should reach TableFull"
     assert_same_slots(served_map, alone_map, "served vs alone")
+
+
+def test_host_alloc_and_free():
+    """fm_host_alloc hands out pinned memory the serving loop recognises;
+    fm_host_free accepts exactly those pointers, once."""
+    import ctypes
+    from flashmap_amd import lib
+    say("host alloc/free")
+    a = fm.PinnedArena(1 << 20)
+    x = a.take(1000, fill=7)
+    assert x.sum() == 7000
+    inner = ctypes.c_void_p(a.ptr + 256)
+    assert lib().fm_host_free(inner) == fm.INVALID_ARGUMENT
+    a.close()
+    assert lib().fm_host_free(ctypes.c_void_p(12345)) == fm.INVALID_ARGUMENT
+    assert lib().fm_host_free(ctypes.c_void_p(0)) == fm.OK
