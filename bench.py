@@ -137,7 +137,7 @@ def main():
                          "c5: serving loop at several offered loads")
     ap.add_argument("--tables", type=int, default=512, help="c4: number of tables")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
-    ap.add_argument("--requests", type=int, default=400, help="c5: requests per offered load")
+    ap.add_argument("--requests", type=int, default=1000, help="c5: requests per offered load")
     ap.add_argument("--rates", default="2000,10000,50000", help="c5: offered loads, requests/s")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
