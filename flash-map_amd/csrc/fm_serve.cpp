@@ -123,6 +123,7 @@ class CopyPool {
   }
   // Copies every span; returns when all bytes are in place.
   void run(const std::vector<Span>& spans) {
+    if (spans.empty()) return;  // nothing to copy: wake no worker
     constexpr uint64_t kPiece = 256 << 10;
     Call call;
     {
@@ -181,6 +182,41 @@ class CopyPool {
   std::deque<Job> jobs_;
   bool stop_ = false;
   std::vector<std::thread> threads_;
+};
+
+// Issues hipMemcpyAsync calls, merging a copy into the previous one when
+// both its source and its destination continue where the previous ended
+// (adjacent requests laid out back to back need one call, not one each:
+// a call costs ~20 us of host time).
+class CopyMerger {
+ public:
+  CopyMerger(hipMemcpyKind kind, hipStream_t stream) : kind_(kind), stream_(stream) {}
+  void add(void* dst, const void* src, uint64_t bytes) {
+    if (bytes == 0 || err_ != hipSuccess) return;
+    uint8_t* d = static_cast<uint8_t*>(dst);
+    const uint8_t* s = static_cast<const uint8_t*>(src);
+    if (n_ && d == dst_ + n_ && s == src_ + n_) {
+      n_ += bytes;
+      return;
+    }
+    (void)flush();
+    dst_ = d;
+    src_ = s;
+    n_ = bytes;
+  }
+  hipError_t flush() {
+    if (n_ && err_ == hipSuccess) err_ = hipMemcpyAsync(dst_, src_, n_, kind_, stream_);
+    n_ = 0;
+    return err_;
+  }
+
+ private:
+  hipMemcpyKind kind_;
+  hipStream_t stream_;
+  uint8_t* dst_ = nullptr;
+  const uint8_t* src_ = nullptr;
+  uint64_t n_ = 0;
+  hipError_t err_ = hipSuccess;
 };
 
 struct Batch {
@@ -242,12 +278,25 @@ class Server {
   fm_status async_err_ = FM_OK;
   uint64_t batches_ = 0, max_inflight_ = 0, staging_waits_ = 0;
   std::vector<uint8_t> pin_in_, pin_out_;  // per request: inputs / outputs pinned
+  std::vector<hipEvent_t> events_;          // free events (under mu_), reused across batches
+  hipError_t take_event(hipEvent_t* ev) {
+    {
+      std::lock_guard<std::mutex> l(mu_);
+      if (!events_.empty()) {
+        *ev = events_.back();
+        events_.pop_back();
+        return hipSuccess;
+      }
+    }
+    return hipEventCreateWithFlags(ev, hipEventDisableTiming);
+  }
   static constexpr unsigned kCopyWorkers = 10;
   CopyPool copies_;
 };
 
 Server::~Server() {
   if (compute_ || h2d_ || d2h_) (void)hipDeviceSynchronize();
+  for (hipEvent_t ev : events_) (void)hipEventDestroy(ev);
   if (h2d_) (void)hipStreamDestroy(h2d_);
   if (d2h_) (void)hipStreamDestroy(d2h_);
   if (d_stage_) (void)hipFree(d_stage_);
@@ -401,26 +450,25 @@ fm_status Server::dispatch_one(uint64_t first) {
   copies_.run(spans);
   b.id = static_cast<int32_t>(batches_++);
   hipError_t e;
-  if ((e = hipEventCreateWithFlags(&b.h2d, hipEventDisableTiming)) != hipSuccess ||
-      (e = hipEventCreateWithFlags(&b.done, hipEventDisableTiming)) != hipSuccess ||
-      (e = hipEventCreateWithFlags(&b.d2h, hipEventDisableTiming)) != hipSuccess)
+  if ((e = take_event(&b.h2d)) != hipSuccess || (e = take_event(&b.done)) != hipSuccess ||
+      (e = take_event(&b.d2h)) != hipSuccess)
     return fail(FM_ERR_KERNEL_LAUNCH, std::string("serve events: ") + hipGetErrorString(e));
   if (!any_pinned_in) {  // the whole input region from the staging ring in one copy
     const uint64_t in_bytes = op == FM_OP_INSERT ? (b.vals_at + b.n * V_ - b.keys_at) : b.n * K_;
     e = hipMemcpyAsync(d_stage_ + b.keys_at, h_stage_ + b.keys_at, in_bytes, hipMemcpyHostToDevice, h2d_);
   } else {  // per request: straight from a pinned buffer, else from the staging ring
+    CopyMerger keys(hipMemcpyHostToDevice, h2d_), vals(hipMemcpyHostToDevice, h2d_);
     pos = 0;
     for (uint64_t i : b.reqs) {
       const fm_serve_request& r = reqs_[i];
-      const uint8_t* kin = pin_in_[i] ? static_cast<const uint8_t*>(r.keys) : h_stage_ + b.keys_at + pos * K_;
-      if (e == hipSuccess && r.n)
-        e = hipMemcpyAsync(d_stage_ + b.keys_at + pos * K_, kin, r.n * K_, hipMemcpyHostToDevice, h2d_);
-      if (e == hipSuccess && r.n && op == FM_OP_INSERT) {
-        const uint8_t* vin = pin_in_[i] ? static_cast<const uint8_t*>(r.values) : h_stage_ + b.vals_at + pos * V_;
-        e = hipMemcpyAsync(d_stage_ + b.vals_at + pos * V_, vin, r.n * V_, hipMemcpyHostToDevice, h2d_);
-      }
+      keys.add(d_stage_ + b.keys_at + pos * K_,
+               pin_in_[i] ? static_cast<const uint8_t*>(r.keys) : h_stage_ + b.keys_at + pos * K_, r.n * K_);
+      if (op == FM_OP_INSERT)
+        vals.add(d_stage_ + b.vals_at + pos * V_,
+                 pin_in_[i] ? static_cast<const uint8_t*>(r.values) : h_stage_ + b.vals_at + pos * V_, r.n * V_);
       pos += r.n;
     }
+    if ((e = keys.flush()) == hipSuccess) e = vals.flush();
   }
   if (e != hipSuccess || (e = hipEventRecord(b.h2d, h2d_)) != hipSuccess ||
       (e = hipStreamWaitEvent(compute_, b.h2d, 0)) != hipSuccess)
@@ -442,21 +490,20 @@ fm_status Server::dispatch_one(uint64_t first) {
   if (!b.direct_out) {  // into the staging ring; the completion thread scatters
     if (e == hipSuccess)
       e = hipMemcpyAsync(h_stage_ + out_from, d_stage_ + out_from, out_bytes, hipMemcpyDeviceToHost, d2h_);
-  } else {  // straight into each request's pinned outputs
+  } else if (e == hipSuccess) {  // straight into each request's pinned outputs
+    CopyMerger vals(hipMemcpyDeviceToHost, d2h_), flags(hipMemcpyDeviceToHost, d2h_);
     pos = 0;
     for (uint64_t i : b.reqs) {
       const fm_serve_request& r = reqs_[i];
-      if (e == hipSuccess && r.n) {
-        if (op == FM_OP_GET) {
-          e = hipMemcpyAsync(r.out_values, d_stage_ + b.out_at + pos * V_, r.n * V_, hipMemcpyDeviceToHost, d2h_);
-          if (e == hipSuccess)
-            e = hipMemcpyAsync(r.out_found, d_stage_ + b.found_at + pos, r.n, hipMemcpyDeviceToHost, d2h_);
-        } else {
-          e = hipMemcpyAsync(r.out_flags, d_stage_ + b.flags_at + pos, r.n, hipMemcpyDeviceToHost, d2h_);
-        }
+      if (op == FM_OP_GET) {
+        vals.add(r.out_values, d_stage_ + b.out_at + pos * V_, r.n * V_);
+        flags.add(r.out_found, d_stage_ + b.found_at + pos, r.n);
+      } else {
+        flags.add(r.out_flags, d_stage_ + b.flags_at + pos, r.n);
       }
       pos += r.n;
     }
+    if ((e = vals.flush()) == hipSuccess) e = flags.flush();
   }
   if (e != hipSuccess || (e = hipEventRecord(b.d2h, d2h_)) != hipSuccess)
     return fail(FM_ERR_TRANSFER, std::string("serve D2H: ") + hipGetErrorString(e));
@@ -512,10 +559,10 @@ void Server::completion_loop() {
       std::lock_guard<std::mutex> ml(m_->mu);
       m_->be->serve_settle_remove(settled);
     }
-    (void)hipEventDestroy(b.h2d);
-    (void)hipEventDestroy(b.done);
-    (void)hipEventDestroy(b.d2h);
     std::lock_guard<std::mutex> l(mu_);
+    events_.push_back(b.h2d);
+    events_.push_back(b.done);
+    events_.push_back(b.d2h);
     if (e != hipSuccess && async_err_ == FM_OK) {
       async_err_ = FM_ERR_TRANSFER;
       set_error(FM_ERR_TRANSFER, std::string("serve completion: ") + hipGetErrorString(e));

@@ -481,17 +481,23 @@ class FlashMap:
 class ServeResult:
     """Outputs of one served request."""
 
-    def __init__(self, op, n, value_size, arena=None):
-        # Outputs come from a pinned arena (DMA writes them directly), or are
-        # allocated here with np.full, which writes every page now: np.zeros
-        # would map untouched pages that the server's result copy then faults
-        # in one by one inside the serving window.
+    def __init__(self, op, n, value_size, arenas=None):
+        # Outputs come from pinned arenas (values, found, flags; DMA writes
+        # them directly, packed back to back so that adjacent requests take
+        # one copy), or are allocated here with np.full, which writes every
+        # page now: np.zeros would map untouched pages that the server's
+        # result copy then faults in one by one inside the serving window.
         self.op = op
-        self._arena = arena
-        take = arena.take if arena is not None else (lambda shape: np.full(shape, 0, dtype=np.uint8))
-        self.values = take((n, value_size)) if op == OP_GET else None
-        self.found = take(n) if op == OP_GET else None
-        self.flags = take(n) if op != OP_GET else None
+        self._arenas = arenas
+        if arenas is not None:
+            vals_a, found_a, flags_a = arenas
+            self.values = vals_a.take((n, value_size), align=1) if op == OP_GET else None
+            self.found = found_a.take(n, align=1) if op == OP_GET else None
+            self.flags = flags_a.take(n, align=1) if op != OP_GET else None
+        else:
+            self.values = np.full((n, value_size), 0, dtype=np.uint8) if op == OP_GET else None
+            self.found = np.full(n, 0, dtype=np.uint8) if op == OP_GET else None
+            self.flags = np.full(n, 0, dtype=np.uint8) if op != OP_GET else None
         self.count = 0
         self.status = OK
         self.batch = -1
@@ -515,9 +521,9 @@ class PinnedArena:
         self.buf = np.frombuffer((ctypes.c_uint8 * self.nbytes).from_address(self.ptr), dtype=np.uint8)
         self.used = 0
 
-    def take(self, shape, fill=0) -> np.ndarray:
+    def take(self, shape, fill=0, align=256) -> np.ndarray:
         size = int(np.prod(shape))
-        start = (self.used + 255) & ~255
+        start = (self.used + align - 1) // align * align
         if start + size > self.nbytes:
             raise ValueError("pinned arena exhausted")
         self.used = start + size
@@ -526,8 +532,8 @@ class PinnedArena:
             a[...] = fill
         return a
 
-    def copy_of(self, a: np.ndarray) -> np.ndarray:
-        out = self.take(a.shape, fill=None)
+    def copy_of(self, a: np.ndarray, align=256) -> np.ndarray:
+        out = self.take(a.shape, fill=None, align=align)
         out[...] = a
         return out
 
@@ -560,22 +566,22 @@ def serve_log(fmap: "FlashMap", requests, staging_bytes: int = 256 << 20, max_ba
     keep, results = [], []
     pin_in = pinned in (True, "inputs")
     pin_out = pinned in (True, "outputs")
-    arena = None
-    if pin_in or pin_out:
-        need = 0
-        for r in requests:
-            nk = _as_bytes(r.keys, fmap.key_size, "keys")[1]
-            if pin_in:
-                need += nk * fmap.key_size + 256 + (nk * fmap.value_size + 256 if r.op == "insert" else 0)
-            if pin_out:
-                need += nk * (fmap.value_size + 1) + 512 if r.op == "get" else nk + 256
-        arena = PinnedArena(need)
+    K, V = fmap.key_size, fmap.value_size
+    sizes = [_as_bytes(r.keys, K, "keys")[1] for r in requests]
+
+    def field(pred, per):  # one pinned arena per field, requests packed in arrival order
+        return PinnedArena(sum(nk * per for r, nk in zip(requests, sizes) if pred(r)))
+
+    key_a = field(lambda r: True, K) if pin_in else None
+    val_a = field(lambda r: r.op == "insert", V) if pin_in else None
+    outs = (field(lambda r: r.op == "get", V), field(lambda r: r.op == "get", 1),
+            field(lambda r: r.op != "get", 1)) if pin_out else None
     for i, r in enumerate(requests):
         op = ops[r.op]
-        k, nk = _as_bytes(r.keys, fmap.key_size, "keys")
+        k, nk = _as_bytes(r.keys, K, "keys")
         if pin_in:
-            k = arena.copy_of(k)
-        res = ServeResult(op, nk, fmap.value_size, arena if pin_out else None)
+            k = key_a.copy_of(k, align=1)
+        res = ServeResult(op, nk, V, outs)
         res.arrival_s = float(r.arrival_s) if realtime else 0.0
         a = arr[i]
         a.op, a.n, a.keys, a.arrival_s = op, nk, k.ctypes.data if nk else None, float(r.arrival_s)
@@ -584,7 +590,7 @@ def serve_log(fmap: "FlashMap", requests, staging_bytes: int = 256 << 20, max_ba
             v, nv = _as_bytes(r.vals, fmap.value_size, "values")
             assert nv == nk
             if pin_in:
-                v = arena.copy_of(v)
+                v = val_a.copy_of(v, align=1)
             a.values = v.ctypes.data if nv else None
             keep.append(v)
         if op == OP_GET:
@@ -600,7 +606,7 @@ def serve_log(fmap: "FlashMap", requests, staging_bytes: int = 256 << 20, max_ba
         a = arr[i]
         res.count, res.status, res.batch = int(a.count), int(a.status), int(a.batch)
         res.admit_s, res.done_s = float(a.admit_s), float(a.done_s)
-    del keep  # a pinned input arena stays alive through the results that point into it
+    del keep  # pinned input arenas go with this frame; output arenas live on in the results
     return results, stats
 
 
