@@ -1,0 +1,53 @@
+"""bench.py end to end: the refusal of several GPUs (CPU), and the config 1
+and reduced config 3 lines (GPU): one JSON line each, exact, with the
+roofline and per-op fields filled."""
+import json
+import os
+import subprocess
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def run_bench(*args, timeout=150):
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), *args], capture_output=True, text=True,
+                       timeout=timeout, cwd=ROOT)
+    return r
+
+
+def json_line(r):
+    assert r.returncode == 0, r.stderr[-2000:]
+    lines = [l for l in r.stdout.splitlines() if l.startswith("{")]
+    assert len(lines) == 1, r.stdout[-2000:]
+    return json.loads(lines[0])
+
+
+def test_bench_refuses_several_gpus():
+    r = run_bench("--gpus", "2", timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
+
+
+@pytest.mark.gpu
+def test_bench_config1_line():
+    print("bench c1", flush=True)
+    d = json_line(run_bench("--config", "c1", "--steps", "2", "--warmup", "1", "--no-cpu-baseline"))
+    assert d["check"]["bit_exact"] is True
+    assert d["check"]["inserted"] == 500_000 and d["check"]["removed"] == 100_000
+    assert d["value"] > 0 and d["n_gpus"] == 1
+    assert set(d["ops"]) == {"insert", "get", "remove", "get2"}
+    for op in d["ops"].values():
+        assert op["device_ms"] > 0 and 0 < op["frac"] < 1
+    assert d["roofline"]["peak"] == 8000.0 and d["stream_copy"]["best_gbs"] > 0
+
+
+@pytest.mark.gpu
+def test_bench_config3_reduced_line():
+    print("bench c3 reduced", flush=True)
+    d = json_line(run_bench("--config", "c3", "--scale", "0.02", "--steps", "1", "--warmup", "0",
+                            "--no-cpu-baseline", "--no-autotune"))
+    assert d["check"]["rows_exact"] is True
+    assert d["check"]["probes_checked"] == 2_000_000
+    assert d["value"] > 0 and 0 < d["roofline"]["frac"] < 1
