@@ -50,15 +50,109 @@ struct OpDesc {  // device-resident description of one op over the group
   const uint32_t* n_chunks;
 };
 
-// nbig[0]: closures queued for kg_ins_big, nbig[1]: its work ticket.
+// nbig[0]: closures queued for kg_ins_big, nbig[1]: its work ticket,
+// nbig[2 + b]: queued closures in size class b, nbig[2 + kBigClasses + b]:
+// the order pass's fill cursor of class b.
+constexpr uint32_t kBigClasses = 32;
+constexpr uint32_t kNbigWords = 2 + 2 * kBigClasses;
+
 __global__ void kg_zero(Counters* a, Counters* b, uint32_t n, uint32_t* nbig) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     a[i] = Counters{0, 0, 0};
     b[i] = Counters{0, 0, 0};
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    nbig[0] = 0;
-    nbig[1] = 0;
+  if (blockIdx.x == 0 && threadIdx.x < kNbigWords) nbig[threadIdx.x] = 0;  // blockDim >= kNbigWords
+}
+
+// Longest replays first.  The big queue fills in whatever order the leader
+// lanes reach it, so a closure of a thousand members can start after most
+// of the others have finished and alone keep kg_ins_big running.  These two
+// passes sort the queue by member count in powers of two (largest class
+// first; any order inside a class).  Closures own disjoint arcs, so the
+// order of their replays does not change the result.
+// Adds, for every active thread of the block, 1 to ctr[c] for its class c,
+// with one global atomic per class present in the block (the classes are
+// few, so per-thread or per-wave atomics would queue on a handful of
+// addresses).  Returns the counter's old value plus the thread's rank among
+// the block's threads of its class.  Every thread of the block calls it.
+__device__ inline uint32_t block_class_add(uint32_t* ctr, uint32_t c, bool active, uint32_t* hist, uint32_t* gbase) {
+  if (threadIdx.x < kBigClasses) hist[threadIdx.x] = 0;
+  __syncthreads();
+  const uint32_t rank = active ? atomicAdd(&hist[c], 1u) : 0u;
+  __syncthreads();
+  if (threadIdx.x < kBigClasses && hist[threadIdx.x]) gbase[threadIdx.x] = atomicAdd(&ctr[threadIdx.x], hist[threadIdx.x]);
+  __syncthreads();
+  const uint32_t r = active ? gbase[c] + rank : 0u;
+  __syncthreads();  // hist and gbase are reused by the next call
+  return r;
+}
+
+// Pass 1: a closure's members are the dry-claimed slots (P & ~Q) of its
+// arc, which runs from its first member s up to the next slot still free in
+// Q.  Only the first kClassWords bitmap words from s are read (all at once):
+// a closure whose arc runs past them is put in the top class.
+constexpr uint32_t kClassWords = 16;
+
+__global__ void __launch_bounds__(256)
+kg_big_class(const Table* __restrict__ tables, const unsigned long long* __restrict__ Q,
+             const unsigned long long* __restrict__ qoff, const unsigned long long* __restrict__ big, uint32_t* nbig,
+             uint8_t* __restrict__ cls) {
+  __shared__ uint32_t hist[kBigClasses], gbase[kBigClasses];
+  const uint32_t nb = nbig[0];
+  for (uint32_t base = blockIdx.x * blockDim.x; base < nb; base += gridDim.x * blockDim.x) {  // block-uniform
+    const uint32_t c = base + threadIdx.x;
+    const bool active = c < nb;
+    uint32_t b = 0;
+    if (active) {
+      const uint32_t table = static_cast<uint32_t>(big[c] >> 32);
+      const uint64_t s = big[c] & 0xFFFFFFFFull;
+      const Table t = tables[table];
+      const unsigned long long* Qt = Q + qoff[table];
+      const uint64_t words = (t.cap + 63) >> 6;
+      unsigned long long qw[kClassWords], mw[kClassWords];
+#pragma unroll
+      for (uint32_t j = 0; j < kClassWords; ++j) {
+        const uint64_t w = ((s >> 6) + j) % words;
+        qw[j] = Qt[w];
+        mw[j] = t.freebits[w] & ~qw[j];
+      }
+      // Bits at or after s in the first word; the barrier is the first Q bit.
+      // Tables of fewer than kClassWords words repeat words here, but the
+      // barrier comes within the first lap, so nothing is counted twice.
+      uint64_t members = 0;
+      bool found = false;
+#pragma unroll
+      for (uint32_t j = 0; j < kClassWords; ++j) {  // no early exit: constant indices keep qw/mw in registers
+        const unsigned long long from = j == 0 ? (~0ull << (s & 63)) : ~0ull;
+        const unsigned long long q = found ? 0ull : (qw[j] & from);
+        const unsigned long long upto = q ? ((q & (0ull - q)) - 1ull) : ~0ull;  // bits below the barrier
+        members += found ? 0u : static_cast<uint32_t>(__popcll(mw[j] & from & upto));
+        found = found || q != 0;
+      }
+      const uint32_t k = members ? 63u - static_cast<uint32_t>(__clzll(members)) : 0u;
+      b = found ? (k < kBigClasses - 1 ? k : kBigClasses - 2) : kBigClasses - 1;
+      cls[c] = static_cast<uint8_t>(b);
+    }
+    (void)block_class_add(nbig + 2, b, active, hist, gbase);
+  }
+}
+
+// Pass 2: scatter the queue into `ordered`, largest class first.
+__global__ void __launch_bounds__(256)
+kg_big_order(const unsigned long long* __restrict__ big, const uint8_t* __restrict__ cls, uint32_t* nbig,
+             unsigned long long* __restrict__ ordered) {
+  __shared__ uint32_t hist[kBigClasses], gbase[kBigClasses];
+  const uint32_t nb = nbig[0];
+  for (uint32_t base = blockIdx.x * blockDim.x; base < nb; base += gridDim.x * blockDim.x) {  // block-uniform
+    const uint32_t c = base + threadIdx.x;
+    const bool active = c < nb;
+    const uint32_t b = active ? cls[c] : 0u;
+    const uint32_t at = block_class_add(nbig + 2 + kBigClasses, b, active, hist, gbase);
+    if (active) {
+      uint32_t before = 0;
+      for (uint32_t h = b + 1; h < kBigClasses; ++h) before += nbig[2 + h];
+      ordered[before + at] = big[c];
+    }
   }
 }
 
@@ -458,7 +552,7 @@ inline uint32_t gridn(uint64_t threads, uint32_t block) {
 struct GroupOps {
   void (*insert)(const Table*, OpDesc, const TableOp*, uint32_t, const uint8_t*, const uint8_t*,
                  unsigned long long*, const unsigned long long*, const unsigned long long*, uint64_t, uint32_t*,
-                 unsigned long long*, uint32_t*, Counters*, uint32_t, uint32_t, uint32_t, hipStream_t);
+                 unsigned long long*, uint64_t, uint32_t*, Counters*, uint32_t, uint32_t, uint32_t, hipStream_t);
   void (*get)(const Table*, OpDesc, const uint8_t*, uint8_t*, uint8_t*, uint32_t, uint32_t, uint32_t, hipStream_t);
   void (*remove)(const Table*, OpDesc, const TableOp*, uint32_t, const uint8_t*, unsigned long long*, uint8_t*,
                  Counters*, uint32_t, uint32_t, uint32_t, hipStream_t);
@@ -468,12 +562,18 @@ template <int KS, int VS>
 void g_insert(const Table* tables, OpDesc op, const TableOp* tops, uint32_t n_tables, const uint8_t* keys,
               const uint8_t* vals, unsigned long long* Q, const unsigned long long* freebits_all,
               const unsigned long long* qoff, uint64_t total_words, uint32_t* dslot, unsigned long long* big,
-              uint32_t* nbig, Counters* ctr, uint32_t block, uint32_t grid, uint32_t coop_arc, hipStream_t s) {
+              uint64_t big_cap, uint32_t* nbig, Counters* ctr, uint32_t block, uint32_t grid, uint32_t coop_arc,
+              hipStream_t s) {
+  // big: [queue | ordered queue] of big_cap entries each, then big_cap class bytes
+  unsigned long long* ordered = big + big_cap;
+  uint8_t* cls = reinterpret_cast<uint8_t*>(big + 2 * big_cap);
   hipLaunchKernelGGL(dev::kg_copy_words, dim3(gridn(total_words, 256)), dim3(256), 0, s, Q, freebits_all, total_words);
   hipLaunchKernelGGL((dev::kg_ins_dry<KS>), dim3(grid), dim3(block), 0, s, tables, op, keys, Q, qoff, dslot, ctr);
   hipLaunchKernelGGL((dev::kg_ins_leader<KS, VS>), dim3(grid), dim3(block), dev::kLeaderMax * block * 4, s, tables, op,
                      keys, vals, Q, qoff, dslot, big, nbig, ctr, coop_arc);
-  hipLaunchKernelGGL((dev::kg_ins_big<KS, VS>), dim3(1024), dim3(64), 0, s, tables, op, keys, vals, Q, qoff, big,
+  hipLaunchKernelGGL(dev::kg_big_class, dim3(256), dim3(256), 0, s, tables, Q, qoff, big, nbig, cls);
+  hipLaunchKernelGGL(dev::kg_big_order, dim3(256), dim3(256), 0, s, big, cls, nbig, ordered);
+  hipLaunchKernelGGL((dev::kg_ins_big<KS, VS>), dim3(1024), dim3(64), 0, s, tables, op, keys, vals, Q, qoff, ordered,
                      nbig, ctr);
   hipLaunchKernelGGL((dev::kg_ins_serial<KS, VS>), dim3(gridn(n_tables, 64)), dim3(64), 0, s, tables, tops, n_tables,
                      keys, vals, ctr);
@@ -718,7 +818,7 @@ fm_status GroupImpl::init(uint32_t n, const uint64_t* caps, uint32_t K, uint32_t
       (e = hipMalloc(&owner_all_, total_slots_ * 4)) != hipSuccess ||
       (e = hipMalloc(&Q_, total_words_ * 8)) != hipSuccess || (e = hipMalloc(&d_tables_, n * sizeof(Table))) != hipSuccess ||
       (e = hipMalloc(&d_qoff_, n * 8)) != hipSuccess || (e = hipMalloc(&d_nch_, 3 * 4)) != hipSuccess ||
-      (e = hipMalloc(&nbig_, 8)) != hipSuccess || (e = hipMalloc(&ins_ctr_, n * sizeof(Counters))) != hipSuccess ||
+      (e = hipMalloc(&nbig_, dev::kNbigWords * 4)) != hipSuccess || (e = hipMalloc(&ins_ctr_, n * sizeof(Counters))) != hipSuccess ||
       (e = hipMalloc(&rem_ctr_, n * sizeof(Counters))) != hipSuccess ||
       (e = hipHostMalloc(&h_ins_ctr_, n * sizeof(Counters), hipHostMallocDefault)) != hipSuccess ||
       (e = hipHostMalloc(&h_rem_ctr_, n * sizeof(Counters), hipHostMallocDefault)) != hipSuccess) {
@@ -804,7 +904,7 @@ fm_status GroupImpl::ensure_keys(uint64_t total) {
   const uint64_t want = std::max<uint64_t>(total, 4096);
   hipError_t e;
   if ((e = hipMalloc(&dslot_, want * 4)) != hipSuccess || (e = hipMalloc(&ev_, want * 8)) != hipSuccess ||
-      (e = hipMalloc(&action_, want)) != hipSuccess || (e = hipMalloc(&big_, want * 8)) != hipSuccess)
+      (e = hipMalloc(&action_, want)) != hipSuccess || (e = hipMalloc(&big_, want * 17)) != hipSuccess)
     return fail(FM_ERR_GPU_ALLOC, hmsg("hipMalloc (group scratch)", e));
   key_cap_ = want;
   return FM_OK;
@@ -840,8 +940,8 @@ fm_status GroupImpl::stage(int which, const HostOp& op, uint64_t total) {
 
 fm_status GroupImpl::enqueue_insert(const uint8_t* keys, const uint8_t* vals) {
   OpDesc op{d_tops_[0], d_chunks_[0], d_nch_ + 0};
-  ops_.insert(d_tables_, op, d_tops_[0], n_, keys, vals, Q_, free_all_, d_qoff_, total_words_, dslot_, big_, nbig_,
-              ins_ctr_, tune_.block, tune_.grid, tune_.coop_arc, stream_);
+  ops_.insert(d_tables_, op, d_tops_[0], n_, keys, vals, Q_, free_all_, d_qoff_, total_words_, dslot_, big_, key_cap_,
+              nbig_, ins_ctr_, tune_.block, tune_.grid, tune_.coop_arc, stream_);
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? FM_OK : fail(FM_ERR_KERNEL_LAUNCH, hmsg("group insert", e));
 }
