@@ -761,9 +761,17 @@ fm_status GpuBackend::run_insert(const uint8_t* d_keys, const uint8_t* d_vals, u
         return st;
     }
     Prof p(this, kKidInsBits);
-    // 1024-thread workgroups: one completion ticket per 1024 keys
-    hipLaunchKernelGGL(dev::k_ins_bits, dim3(grid_for(n, 1024)), dim3(1024), 0, stream_, t, dslot_, n, ctr_, Q_,
-                       h_ctr_);
+    if (!dedup && n * 16 >= t_.cap) {
+      // Most bitmap words change: recompute them all from the flags (one
+      // coalesced pass) instead of one atomic per key; at most 1024
+      // workgroups, so at most 1024 completion tickets.
+      const uint32_t g = std::min<uint32_t>(grid_for(t_.cap, 1024), 1024);
+      hipLaunchKernelGGL(dev::k_ins_bits_rebuild, dim3(g), dim3(1024), 0, stream_, t, ctr_, Q_, h_ctr_);
+    } else {
+      // 1024-thread workgroups: one completion ticket per 1024 keys
+      hipLaunchKernelGGL(dev::k_ins_bits, dim3(grid_for(n, 1024)), dim3(1024), 0, stream_, t, dslot_, n, ctr_, Q_,
+                         h_ctr_);
+    }
     if ((st = check_launch(hipGetLastError(), "k_ins_bits")) != FM_OK) return st;
     q_pending_ = true;     // Q_ == freebits once the status shows no failed dry pass
     ctr_zero_ = true;      // k_ins_bits zeroed the counters ...

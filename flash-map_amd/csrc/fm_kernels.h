@@ -1237,6 +1237,44 @@ k_ins_bits(Table t, const uint32_t* __restrict__ dslot, uint64_t n, Counters* ct
   }
 }
 
+// k_ins_bits for a batch that touches most bitmap words (the caller uses it
+// when n * 16 >= cap and no deduplication is on): every word is recomputed
+// from the flags, one wave per 64 slots, instead of one atomic per key.
+// After the replay the flags are the truth, so freebits (not OCCUPIED),
+// emptybits (EMPTY) and Q (= freebits) come out exactly as k_ins_bits
+// leaves them, and every owner[] entry is reset.  A failed dry pass left
+// the flags untouched, so the words come out unchanged.  The last
+// workgroup hands the counters to host memory as k_ins_bits does.
+static __global__ void __launch_bounds__(1024)
+k_ins_bits_rebuild(Table t, Counters* ctr, unsigned long long* Q, Counters* host_out) {
+  const uint64_t words = (t.cap + 63) >> 6;
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint64_t wave = (static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+  const uint64_t waves = (static_cast<uint64_t>(gridDim.x) * blockDim.x) >> 6;
+  for (uint64_t w = wave; w < words; w += waves) {  // wave-uniform
+    const uint64_t i = (w << 6) + lane;
+    const uint32_t st = i < t.cap ? (t.flags[i] & 0xFu) : kOccupied;
+    const unsigned long long fb = __ballot(st != kOccupied);
+    const unsigned long long eb = __ballot(st == kEmpty);
+    if (lane == 0) {
+      t.freebits[w] = fb;
+      t.emptybits[w] = eb;
+      if (Q) Q[w] = fb;
+    }
+    if (i < t.cap) t.owner[i] = kNoOwner;
+  }
+  if (!host_out) return;
+  __syncthreads();
+  if (threadIdx.x == 0 && atomicAdd(&ctr->done, 1u) == gridDim.x - 1) {
+    Counters out{};
+    out.status = ld_agent(&ctr->status);
+    out.nbig = ld_agent(&ctr->nbig);
+    out.count = __hip_atomic_load(&ctr->count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    *host_out = out;
+    *ctr = Counters{};
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Fused insert: the leader path in one cooperative launch.  Three phases over
 // the same key-to-lane mapping, separated by grid-wide barriers:
