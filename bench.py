@@ -791,6 +791,36 @@ def run_c4(args):
             log(f"c4: table {t} differs from the oracle")
         o.close()
     fused = float(np.median(fused_ms))
+    cpu = None
+    if not args.no_cpu_baseline:
+        # Rayon restatement on every 8th table: each preloaded untimed, then
+        # its insert -> get -> remove timed (one table's map and thread pool
+        # at a time).
+        from oracle.binding import RayonPortMap, TableFull as OTF
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        cpu_keys, cpu_s = 0, 0.0
+        picked = list(range(0, n, 8))
+        for t in picked:
+            w = tabs[t]
+            p = RayonPortMap(w.capacity, 8, V, 0, threads=threads)
+            if len(w.preload_keys):
+                p.bulk_insert(rows64(w.preload_keys), w.preload_vals)
+            t0 = time.perf_counter()
+            if len(w.ins_keys):
+                try:
+                    p.bulk_insert(rows64(w.ins_keys), w.ins_vals)
+                except OTF:
+                    pass
+            if len(w.get_keys):
+                p.bulk_get(rows64(w.get_keys))
+            if len(w.rem_keys):
+                p.bulk_remove(rows64(w.rem_keys))
+            cpu_s += time.perf_counter() - t0
+            cpu_keys += len(w.ins_keys) + len(w.get_keys) + len(w.rem_keys)
+            p.close()
+        cpu = {"value": round(cpu_keys / cpu_s / 1e6, 3), "unit": "Mkeys/s", "cores": threads, "kind": "port",
+               "sample": f"step of {len(picked)} of the {n} tables (every 8th): {cpu_keys} key-ops in "
+                         f"{cpu_s:.2f} s, each table preloaded untimed; host '{host_cpu_model()}'"}
     out_line = {
         "metric": "batched multi-table step Mkeys/s (fused hipGraph vs plain per-table launches)",
         "value": round(keys_per_step / fused / 1e3, 3), "unit": "Mkeys/s", "n_gpus": 1,
@@ -804,6 +834,7 @@ def run_c4(args):
                    "table_full_batches": int(sum(1 for s in f_status if s))},
         "tuning": {k: getattr(work.tuning(), k) for k in ("group", "block", "grid", "coop_arc")},
         "autotune": tune,
+        "cpu_baseline": cpu,
         "check": {"fused_equals_plain": exact, "oracle_sample_tables": len(sample), "oracle_exact": bool(oracle_ok)},
     }
     print(json.dumps(out_line), flush=True)
@@ -872,13 +903,41 @@ def run_c5(args):
                       "p99_ms": round(st.p99_ms, 3), "mean_ms": round(st.mean_ms, 3), "batches": int(st.batches),
                       "max_inflight_batches": int(st.max_inflight_batches), "staging_waits": int(st.staging_waits),
                       "check": check})
+    cpu = None
+    if not args.no_cpu_baseline:
+        # Rayon restatement applying the first load's requests back to back in
+        # arrival order (no arrival gaps: its throughput ceiling).
+        from oracle.binding import RayonPortMap, TableFull as OTF
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        reqs = config5_requests(pk, float(args.rates.split(",")[0]), args.requests)
+        p = RayonPortMap(cap, 32, 8, 0, threads=threads)
+        p.bulk_insert(pk, pv)
+        t0 = time.perf_counter()
+        keys = 0
+        for r in reqs:
+            keys += len(r.keys)
+            if r.op == "get":
+                p.bulk_get(r.keys)
+            elif r.op == "insert":
+                try:
+                    p.bulk_insert(r.keys, r.vals)
+                except OTF:
+                    pass
+            else:
+                p.bulk_remove(r.keys)
+        dt = time.perf_counter() - t0
+        p.close()
+        cpu = {"value": round(len(reqs) / dt, 1), "unit": "req/s", "mkeys_s": round(keys / dt / 1e6, 2),
+               "cores": threads, "kind": "port",
+               "sample": f"the {len(reqs)} requests of the first offered load applied back to back in arrival "
+                         f"order ({keys} keys, {dt:.2f} s) on the 20M-key preload; host '{host_cpu_model()}'"}
     print(json.dumps({"metric": "serving p50/p99 latency and throughput at offered loads", "n_gpus": 1,
                       "config": {"workload": "FlashMap<[u8;32],u64> with_capacity(24M)=2^25, preload 20M; requests "
                                              "70% get / 20% insert / 10% remove, log-uniform [64,262144] keys, gets "
                                              "80% hits Zipf(0.9)", "staging_bytes": 1 << 30,
                                  "request_buffers": "loads: pinned (fm_host_alloc, DMA both ways); "
                                                     "loads_pageable: pageable (host copy through the staging ring)"},
-                      "loads": lines, "loads_pageable": pageable}), flush=True)
+                      "loads": lines, "loads_pageable": pageable, "cpu_baseline": cpu}), flush=True)
     if any(l["check"] and l["check"]["mismatches"] for l in lines):
         sys.exit(1)
 
