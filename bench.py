@@ -437,6 +437,15 @@ def run_c1(args):
            "get": lambda: m.bulk_get_device(q, nq, out, found),
            "remove": lambda: m.bulk_remove_device(rk, nr),
            "get2": lambda: m.bulk_get_device(q, nq, out, found)}
+    get_tune = None
+    if not args.no_autotune:
+        # get launch shape for this table and query mix, tuned on the
+        # populated table (gets change nothing); every step then clears it
+        m.clear()
+        m.bulk_insert_device(ik, iv, ni)
+        get_tune = m.autotune_get(q, nq, reps=5)
+        m.set_tuning(get_group=get_tune["get_group"], get_block=get_tune["get_block"])
+        log(f"c1: get autotune {get_tune}")
     dev = {k: [] for k in names}
     wall = {k: [] for k in names}
     kern = {k: {} for k in names}
@@ -550,6 +559,7 @@ def run_c1(args):
                      "unit": "GB/s", "frac": ops[dom]["frac"], "traffic": ops[dom].get("traffic_bytes")},
         "stream_copy": copy,
         "ops": ops,
+        "get_autotune": get_tune,
         "cpu_baseline": cpu and {"value": cpu["port"]["sequence_mkeys_s"], "unit": "Mkeys/s",
                                  "cores": cpu["port"]["cores"], "kind": "port",
                                  "sample": f"the whole config-1 sequence, one warm-up then the median of 5 per op; "
