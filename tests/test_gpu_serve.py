@@ -145,3 +145,25 @@ def test_host_alloc_and_free():
     a.close()
     assert lib().fm_host_free(ctypes.c_void_p(12345)) == fm.INVALID_ARGUMENT
     assert lib().fm_host_free(ctypes.c_void_p(0)) == fm.OK
+
+
+@pytest.mark.parametrize("pinned", [False, True], ids=["pageable", "pinned"])
+def test_serve_empty_requests_in_the_mix(pinned):
+    """Requests with no keys (of every kind) sit between ordinary ones: they
+    complete with count 0 and leave the neighbours' results exact."""
+    cap = 1 << 14
+    pre, pre_v, reqs = _log(11, cap, 6_000, 60, 2048, 1.0)
+    for i in range(0, len(reqs), 7):
+        r = reqs[i]
+        r.keys = r.keys[:0]
+        if r.op == "insert":
+            r.vals = r.vals[:0]
+    say(f"serve with empty requests pinned={pinned}")
+    m = fm.FlashMap.builder(cap, K, V).require_gpu().build()
+    m.bulk_insert(pre, pre_v)
+    results, _ = fm.serve_log(m, reqs, staging_bytes=8 << 20, max_batch_keys=4096, realtime=False, pinned=pinned)
+    o, expected = _expected(cap, pre, pre_v, reqs)
+    _check(results, expected, "served")
+    assert all(results[i].count == 0 for i in range(0, len(reqs), 7))
+    assert len(m) == len(o)
+    assert_same_slots(m, o, "after serving")
