@@ -529,3 +529,27 @@ def test_gpu_long_arcs_in_lds_and_hbm(K, V):
            ("ins", keys[:4000:5], vals[4000:8000:5]),               # len + batch > capacity: TableFull
            ("get", keys)]
     apply_and_compare(m, o, ops)
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_gpu_bitmap_rebuild_and_per_key_batches_alternate(seed):
+    """Insert batches alternate across the size where the bits pass switches
+    from per-key atomics to rebuilding every bitmap word (n * 16 >= cap),
+    with removes (tombstones) between them: each path must leave the claim
+    bitmap the other expects.  The whole slot array is compared after every
+    op."""
+    say(f"bits rebuild alternation seed={seed}")
+    rng = np.random.default_rng(300 + seed)
+    cap, K, V = 1 << 12, 8, 8
+    m, o = gpu_make(cap, K, V), OracleAdapter(cap, K, V)
+    universe = rng.integers(0, 1 << 62, 6000).astype(np.uint64).view(np.uint8).reshape(-1, 8)
+    ops = []
+    for step in range(16):
+        n = int(rng.choice([40, 200, 255, 256, 300, 700]))  # 256 = cap / 16
+        idx = rng.integers(0, len(universe), n)
+        ops.append(("ins", universe[idx], rng.integers(0, 256, (n, V), dtype=np.uint8)))
+        if step % 2:
+            ops.append(("rem", universe[rng.integers(0, len(universe), int(rng.integers(20, 400)))]))
+        if step % 5 == 4:
+            ops.append(("get", universe[rng.integers(0, len(universe), 500)]))
+    apply_and_compare(m, o, ops)
