@@ -92,7 +92,7 @@ template <int KS, int VS>
 hipError_t launch_ins_leader(const Table& t, const uint8_t* k, const uint8_t* v, uint64_t n,
                              const unsigned long long* Q, const uint32_t* dslot, uint32_t* big, Counters* c,
                              uint32_t coop_arc, hipStream_t s) {
-  hipLaunchKernelGGL((dev::k_ins_leader<KS, VS>), dim3(grid_for(n, 256)), dim3(256), 0, s, t, k, v, n, Q, dslot, big,
+  hipLaunchKernelGGL((dev::k_ins_leader<KS, VS>), dim3(grid_for(n, 64)), dim3(64), 0, s, t, k, v, n, Q, dslot, big,
                      c, coop_arc);
   return hipGetLastError();
 }

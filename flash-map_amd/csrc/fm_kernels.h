@@ -971,11 +971,11 @@ __device__ uint32_t replay_arc_lds(const Table& t, const uint8_t* __restrict__ i
 // (barrier to barrier) is at most `coop_arc` slots; other closures are
 // queued for k_ins_big.
 template <int KS, int VS>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(64)
 k_ins_leader(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __restrict__ in_vals, uint64_t n,
              const unsigned long long* __restrict__ Q, const uint32_t* __restrict__ dslot, uint32_t* __restrict__ big,
              Counters* ctr, uint32_t coop_arc) {
-  __shared__ uint32_t lst[kLeaderMax * 256];  // column per lane: lst[j * 256 + lane]
+  __shared__ uint32_t lst[kLeaderMax * 64];  // column per lane: lst[j * 64 + lane]
   const uint32_t K = KS ? KS : t.K;
   const uint32_t V = VS ? VS : t.V;
   const unsigned long long* P = t.freebits;
@@ -1002,7 +1002,7 @@ k_ins_leader(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __rest
             m = kLeaderMax + 1;
             break;
           }
-          my[m * 256] = t.owner[x];
+          my[m * 64] = t.owner[x];
           ++m;
           pos = (x + 1 == t.cap) ? 0 : x + 1;
         }
@@ -1010,16 +1010,16 @@ k_ins_leader(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __rest
           big[atomicAdd(&ctr->nbig, 1u)] = static_cast<uint32_t>(s);
         } else {
           for (uint32_t a = 1; a < m; ++a) {  // insertion sort by batch index
-            const uint32_t v = my[a * 256];
+            const uint32_t v = my[a * 64];
             uint32_t c = a;
-            while (c > 0 && my[(c - 1) * 256] > v) {
-              my[c * 256] = my[(c - 1) * 256];
+            while (c > 0 && my[(c - 1) * 64] > v) {
+              my[c * 64] = my[(c - 1) * 64];
               --c;
             }
-            my[c * 256] = v;
+            my[c * 64] = v;
           }
           for (uint32_t j = 0; j < m; ++j) {
-            const uint64_t k = my[j * 256];
+            const uint64_t k = my[j * 64];
             const uint32_t c1 =
                 insert_one<KS, VS, false>(t, in_keys + k * K, in_vals + dd_value_index(t, k) * V, K, V, ctr);
             claimed += c1;
