@@ -174,6 +174,7 @@ def main():
     step = Step(A, cfg)
 
     check = None
+    seq_mkeys_s = None
     if not args.no_check:
         log("checking one step against the oracle")
         A.copy_from(snap)
@@ -183,6 +184,8 @@ def main():
         exp = oracle_step(cfg)
         got_sha = hashlib.sha256(vals.tobytes() + found.tobytes()).hexdigest()
         ok = new == exp["new"] and removed == exp["removed"] and got_sha == exp["get_sha"] and len(A) == exp["len"]
+        seq_mkeys_s = (sum(len(k) for k, _ in cfg.batches) + len(cfg.get_keys) + len(cfg.remove_keys)) / \
+            exp["seconds"] / 1e6
         check = {"bit_exact": bool(ok), "hits": exp["hits"], "new_per_batch": exp["new"][:3],
                  "removed": removed}
         if not ok:
@@ -275,6 +278,8 @@ def main():
         cpu = {"value": round(rate, 3), "unit": "Mkeys/s", "cores": threads, "kind": "port",
                "sample": f"{reps} steps of {step.keys_per_step} key-ops, each from the 8M preloaded state "
                          f"(restored untimed between steps); {dt:.2f} s timed; host '{host_cpu_model()}'"}
+        if seq_mkeys_s is not None:  # the checker's one step of the sequential oracle, 1 thread
+            cpu["sequential_1_thread_mkeys_s"] = round(seq_mkeys_s, 3)
 
     out = {
         "metric": "bulk_get/insert Mkeys/s + % HBM peak; batched vs plain; serving p99 latency",
