@@ -553,3 +553,29 @@ def test_gpu_bitmap_rebuild_and_per_key_batches_alternate(seed):
         if step % 5 == 4:
             ops.append(("get", universe[rng.integers(0, len(universe), 500)]))
     apply_and_compare(m, o, ops)
+
+
+@pytest.mark.parametrize("K,V", [(256, 256), (1, 1), (255, 3), (4, 256)])
+def test_gpu_size_limits(K, V):
+    """Key and value sizes at the GPU limits (1 and 256 bytes; 255 and 3
+    take the byte-wise generic kernels) against the oracle, murmur3 and
+    identity hashing."""
+    for mode in (0, 1):
+        say(f"size limits K={K} V={V} mode={mode}")
+        rng = np.random.default_rng(K * 1000 + V + mode)
+        cap = 256
+        ops = random_history(rng, cap, K, V, 16, cap // 2, cap * 2 if K > 1 else 256)
+        apply_and_compare(gpu_make(cap, K, V, mode), OracleAdapter(cap, K, V, mode), ops)
+
+
+def test_gpu_builder_size_above_limit_falls_back():
+    """A 257-byte key is beyond the GPU kernels: the builder falls back to
+    the CPU arm (lib.rs:414-449), and require_gpu reports the GPU as
+    unavailable instead."""
+    say("builder above the size limit")
+    m = fm.FlashMap.builder(64, 257, 8).quiet().build()
+    assert m.backend() == fm.Backend.CPU
+    k = np.arange(257 * 3, dtype=np.uint64).astype(np.uint8).reshape(3, 257)
+    assert m.bulk_insert(k, np.zeros((3, 8), np.uint8)) == 3
+    with pytest.raises(fm.FlashMapError):
+        fm.FlashMap.builder(64, 257, 8).require_gpu().build()
