@@ -579,3 +579,28 @@ def test_gpu_builder_size_above_limit_falls_back():
     assert m.bulk_insert(k, np.zeros((3, 8), np.uint8)) == 3
     with pytest.raises(fm.FlashMapError):
         fm.FlashMap.builder(64, 257, 8).require_gpu().build()
+
+
+def test_gpu_max_capacity_wraps_at_2_31():
+    """The largest GPU table (2^31 slots; 16 GiB each of keys and values,
+    so every slot offset needs 64 bits).  Identity-hashed keys crowded at the
+    last homes of the table run past slot 2^31 - 1 and wrap to slot 0; gets,
+    updates and removes there must match the oracle.  One slot more is
+    refused when the GPU is required."""
+    say("max capacity 2^31")
+    cap = 1 << 31
+    m = gpu_make(cap, 8, 8)
+    assert m.capacity() == cap
+    o = OracleAdapter(cap, 8, 8)
+    rng = np.random.default_rng(31)
+    homes = (cap - 1 - rng.integers(0, 40, 300)).astype(np.uint64)
+    lows = rng.integers(0, 20, 100).astype(np.uint64)
+    ids = np.concatenate([homes, lows]) + (np.arange(400, dtype=np.uint64) << np.uint64(31))  # same homes, distinct keys
+    keys = ids[rng.permutation(ids.size)].view(np.uint8).reshape(-1, 8)
+    vals = rng.integers(0, 256, (keys.shape[0], 8), dtype=np.uint8)
+    ops = [("ins", keys, vals), ("get", keys), ("ins", keys[::3], vals[::-3][: len(keys[::3])]),
+           ("rem", keys[::5]), ("get", keys), ("ins", keys[::5], vals[::5]), ("get", keys)]
+    apply_and_compare(m, o, ops, check_slots=False)
+    m.close()
+    with pytest.raises(fm.FlashMapError):
+        fm.FlashMap.builder(cap + 1, 8, 8).require_gpu().build()
