@@ -10,11 +10,13 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <memory>
 #include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
 
+#include "fm_copy.h"
 #include "fm_internal.h"
 #include "fm_kernels.h"
 
@@ -428,6 +430,19 @@ class GpuBackend final : public Backend {
   uint64_t st_out_bytes_ = 0;
   uint8_t* st_found_ = nullptr;
   uint64_t st_found_bytes_ = 0;
+  // Host buffers of large calls: pageable memory goes through pinned
+  // buffers with the copy pool (buffers from fm_host_alloc are moved by DMA
+  // directly); small calls use the runtime's pageable copies.
+  static constexpr uint64_t kPinnedMin = 1ull << 20;   // bytes per call, in + out
+  static constexpr uint64_t kPinnedMax = 1ull << 30;   // largest pinned buffer kept
+  static constexpr unsigned kCopyWorkers = 8;
+  std::unique_ptr<CopyPool> copies_;
+  uint8_t* h_in_ = nullptr;
+  uint64_t h_in_bytes_ = 0;
+  uint8_t* h_out_ = nullptr;
+  uint64_t h_out_bytes_ = 0;
+  // Grows the pinned buffers; false when a call should keep the pageable path.
+  bool ensure_host_staging(uint64_t in, uint64_t out);
 };
 
 template <typename T>
@@ -469,6 +484,12 @@ void GpuBackend::release() {
   dfree(st_vals_);
   dfree(st_out_);
   dfree(st_found_);
+  if (stream_) (void)hipStreamSynchronize(stream_);
+  if (h_in_) (void)hipHostFree(h_in_);
+  if (h_out_) (void)hipHostFree(h_out_);
+  h_in_ = h_out_ = nullptr;
+  h_in_bytes_ = h_out_bytes_ = 0;
+  copies_.reset();
   pending_.clear();
   for (hipEvent_t e : ev_pool_) (void)hipEventDestroy(e);
   ev_pool_.clear();
@@ -618,6 +639,27 @@ fm_status GpuBackend::ensure_staging(uint64_t bk, uint64_t bv, uint64_t bo, uint
       (e = grow(st_out_, st_out_bytes_, bo)) != hipSuccess || (e = grow(st_found_, st_found_bytes_, bf)) != hipSuccess)
     return fail(FM_ERR_GPU_ALLOC, hip_msg("hipMalloc (staging)", e));
   return FM_OK;
+}
+
+bool GpuBackend::ensure_host_staging(uint64_t in, uint64_t out) {
+  if (in > kPinnedMax || out > kPinnedMax) return false;
+  auto grow = [&](uint8_t*& p, uint64_t& have, uint64_t need) -> bool {
+    if (need <= have) return true;
+    (void)hipStreamSynchronize(stream_);
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    have = 0;
+    const uint64_t want = std::min<uint64_t>(kPinnedMax, std::max<uint64_t>(need, 4u << 20));
+    if (hipHostMalloc(reinterpret_cast<void**>(&p), want, hipHostMallocDefault) != hipSuccess) {
+      p = nullptr;
+      return false;
+    }
+    have = want;
+    return true;
+  };
+  if (!grow(h_in_, h_in_bytes_, in) || !grow(h_out_, h_out_bytes_, out)) return false;
+  if (!copies_) copies_ = std::make_unique<CopyPool>(kCopyWorkers);
+  return true;
 }
 
 // Zeroes the counters for an op that reads them back with read_counters.
@@ -971,13 +1013,26 @@ fm_status GpuBackend::bulk_get(const void* keys, uint64_t n, void* out_values, u
   fm_status st;
   if ((st = use_device()) != FM_OK) return st;
   if ((st = ensure_staging(n * t_.K, 0, n * t_.V, n)) != FM_OK) return st;
-  hipError_t e = hipMemcpyAsync(st_keys_, keys, n * t_.K, hipMemcpyHostToDevice, stream_);
+  const uint64_t kb = n * t_.K, vb = n * t_.V;
+  const bool large = kb + vb + n >= kPinnedMin;
+  const bool stage_in = large && !host_pinned(keys, kb) && ensure_host_staging(kb, vb + n);
+  const bool stage_out =
+      large && !(host_pinned(out_values, vb) && host_pinned(out_found, n)) && ensure_host_staging(kb, vb + n);
+  const void* src = keys;
+  if (stage_in) {
+    copies_->run({{h_in_, static_cast<const uint8_t*>(keys), kb}});
+    src = h_in_;
+  }
+  hipError_t e = hipMemcpyAsync(st_keys_, src, kb, hipMemcpyHostToDevice, stream_);
   if (e != hipSuccess) return fail(FM_ERR_TRANSFER, hip_msg("H2D keys", e));
   if ((st = run_get(st_keys_, n, st_out_, st_found_)) != FM_OK) return st;
-  if ((e = hipMemcpyAsync(out_values, st_out_, n * t_.V, hipMemcpyDeviceToHost, stream_)) != hipSuccess ||
-      (e = hipMemcpyAsync(out_found, st_found_, n, hipMemcpyDeviceToHost, stream_)) != hipSuccess ||
+  uint8_t* ov = stage_out ? h_out_ : static_cast<uint8_t*>(out_values);
+  uint8_t* of = stage_out ? h_out_ + vb : out_found;
+  if ((e = hipMemcpyAsync(ov, st_out_, vb, hipMemcpyDeviceToHost, stream_)) != hipSuccess ||
+      (e = hipMemcpyAsync(of, st_found_, n, hipMemcpyDeviceToHost, stream_)) != hipSuccess ||
       (e = hipStreamSynchronize(stream_)) != hipSuccess)
     return fail(FM_ERR_TRANSFER, hip_msg("D2H results", e));
+  if (stage_out) copies_->run({{static_cast<uint8_t*>(out_values), h_out_, vb}, {out_found, h_out_ + vb, n}});
   return FM_OK;
 }
 
@@ -993,9 +1048,17 @@ fm_status GpuBackend::bulk_insert(const void* keys, const void* values, uint64_t
   fm_status st;
   if ((st = use_device()) != FM_OK) return st;
   if ((st = ensure_staging(n * t_.K, n * t_.V, 0, 0)) != FM_OK) return st;
+  const uint64_t kb = n * t_.K, vb = n * t_.V;
+  const void* ks = keys;
+  const void* vs = values;
+  if (kb + vb >= kPinnedMin && !(host_pinned(keys, kb) && host_pinned(values, vb)) && ensure_host_staging(kb + vb, 0)) {
+    copies_->run({{h_in_, static_cast<const uint8_t*>(keys), kb}, {h_in_ + kb, static_cast<const uint8_t*>(values), vb}});
+    ks = h_in_;
+    vs = h_in_ + kb;
+  }
   hipError_t e;
-  if ((e = hipMemcpyAsync(st_keys_, keys, n * t_.K, hipMemcpyHostToDevice, stream_)) != hipSuccess ||
-      (e = hipMemcpyAsync(st_vals_, values, n * t_.V, hipMemcpyHostToDevice, stream_)) != hipSuccess)
+  if ((e = hipMemcpyAsync(st_keys_, ks, kb, hipMemcpyHostToDevice, stream_)) != hipSuccess ||
+      (e = hipMemcpyAsync(st_vals_, vs, vb, hipMemcpyHostToDevice, stream_)) != hipSuccess)
     return fail(FM_ERR_TRANSFER, hip_msg("H2D pairs", e));
   if ((st = run_insert(st_keys_, st_vals_, n)) != FM_OK) return st;
   Counters c;
@@ -1015,7 +1078,13 @@ fm_status GpuBackend::bulk_remove(const void* keys, uint64_t n, uint64_t* out_re
   fm_status st;
   if ((st = use_device()) != FM_OK) return st;
   if ((st = ensure_staging(n * t_.K, 0, 0, 0)) != FM_OK) return st;
-  hipError_t e = hipMemcpyAsync(st_keys_, keys, n * t_.K, hipMemcpyHostToDevice, stream_);
+  const uint64_t kb = n * t_.K;
+  const void* src = keys;
+  if (kb >= kPinnedMin && !host_pinned(keys, kb) && ensure_host_staging(kb, 0)) {
+    copies_->run({{h_in_, static_cast<const uint8_t*>(keys), kb}});
+    src = h_in_;
+  }
+  hipError_t e = hipMemcpyAsync(st_keys_, src, kb, hipMemcpyHostToDevice, stream_);
   if (e != hipSuccess) return fail(FM_ERR_TRANSFER, hip_msg("H2D keys", e));
   uint64_t removed = 0;
   if ((st = run_remove(st_keys_, n, &removed)) != FM_OK) return st;

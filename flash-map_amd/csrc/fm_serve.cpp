@@ -37,20 +37,18 @@
 #include <thread>
 #include <vector>
 
+#include "fm_copy.h"
 #include "fm_internal.h"
 
 namespace fm {
-namespace {
-
-using Clock = std::chrono::steady_clock;
 
 // Allocations made by fm_host_alloc (base -> bytes).  A request buffer that
 // lies inside one of them is pinned: the DMA engines read or write it
 // directly, with no copy through the pinned staging ring.
-std::mutex g_pinned_mu;
-std::map<uintptr_t, uint64_t> g_pinned;
+static std::mutex g_pinned_mu;
+static std::map<uintptr_t, uint64_t> g_pinned;
 
-bool in_pinned(const void* p, uint64_t bytes) {
+bool host_pinned(const void* p, uint64_t bytes) {
   if (bytes == 0) return true;  // nothing to move
   if (!p) return false;
   const uintptr_t a = reinterpret_cast<uintptr_t>(p);
@@ -60,6 +58,11 @@ bool in_pinned(const void* p, uint64_t bytes) {
   --it;
   return a - it->first <= it->second && bytes <= it->second - (a - it->first);
 }
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
 
 inline uint64_t align256(uint64_t b) { return (b + 255) & ~uint64_t(255); }
 
@@ -97,91 +100,6 @@ class Ring {
  private:
   uint64_t size_, head_ = 0, tail_ = 0;
   std::deque<uint64_t> live_;
-};
-
-// Host copy workers.  Request buffers in ordinary (pageable) memory go
-// through the pinned staging ring; one thread moves them at a few GB/s, so
-// large copies are cut into pieces that the workers and the caller share.
-// Buffers from fm_host_alloc skip this: the DMA engines move them directly.
-class CopyPool {
- public:
-  struct Span {
-    uint8_t* dst;
-    const uint8_t* src;
-    uint64_t bytes;
-  };
-  explicit CopyPool(unsigned workers) {
-    for (unsigned i = 0; i < workers; ++i) threads_.emplace_back([this] { loop(); });
-  }
-  ~CopyPool() {
-    {
-      std::lock_guard<std::mutex> l(mu_);
-      stop_ = true;
-    }
-    cv_.notify_all();
-    for (std::thread& t : threads_) t.join();
-  }
-  // Copies every span; returns when all bytes are in place.
-  void run(const std::vector<Span>& spans) {
-    if (spans.empty()) return;  // nothing to copy: wake no worker
-    constexpr uint64_t kPiece = 256 << 10;
-    Call call;
-    {
-      std::lock_guard<std::mutex> l(mu_);
-      for (const Span& sp : spans)
-        for (uint64_t o = 0; o < sp.bytes; o += kPiece) {
-          jobs_.push_back(Job{sp.dst + o, sp.src + o, std::min(kPiece, sp.bytes - o), &call});
-          ++call.left;
-        }
-    }
-    cv_.notify_all();
-    for (;;) {  // the caller copies too
-      Job j;
-      {
-        std::lock_guard<std::mutex> l(mu_);
-        if (jobs_.empty()) break;
-        j = jobs_.front();
-        jobs_.pop_front();
-      }
-      do_job(j);
-    }
-    std::unique_lock<std::mutex> l(mu_);
-    done_cv_.wait(l, [&] { return call.left == 0; });
-  }
-
- private:
-  struct Call {
-    uint64_t left = 0;  // pieces not yet copied (under mu_)
-  };
-  struct Job {
-    uint8_t* dst = nullptr;
-    const uint8_t* src = nullptr;
-    uint64_t bytes = 0;
-    Call* call = nullptr;
-  };
-  void do_job(const Job& j) {
-    std::memcpy(j.dst, j.src, j.bytes);
-    std::lock_guard<std::mutex> l(mu_);
-    if (--j.call->left == 0) done_cv_.notify_all();
-  }
-  void loop() {
-    for (;;) {
-      Job j;
-      {
-        std::unique_lock<std::mutex> l(mu_);
-        cv_.wait(l, [&] { return stop_ || !jobs_.empty(); });
-        if (jobs_.empty()) return;
-        j = jobs_.front();
-        jobs_.pop_front();
-      }
-      do_job(j);
-    }
-  }
-  std::mutex mu_;
-  std::condition_variable cv_, done_cv_;
-  std::deque<Job> jobs_;
-  bool stop_ = false;
-  std::vector<std::thread> threads_;
 };
 
 // Issues hipMemcpyAsync calls, merging a copy into the previous one when
@@ -334,9 +252,9 @@ fm_status Server::setup() {
   pin_out_.assign(n_, 0);
   for (uint64_t i = 0; i < n_; ++i) {
     const fm_serve_request& r = reqs_[i];
-    pin_in_[i] = in_pinned(r.keys, r.n * K_) && (r.op != FM_OP_INSERT || in_pinned(r.values, r.n * V_));
-    pin_out_[i] = r.op == FM_OP_GET ? (r.out_found && in_pinned(r.out_values, r.n * V_) && in_pinned(r.out_found, r.n))
-                                    : (r.out_flags && in_pinned(r.out_flags, r.n));
+    pin_in_[i] = host_pinned(r.keys, r.n * K_) && (r.op != FM_OP_INSERT || host_pinned(r.values, r.n * V_));
+    pin_out_[i] = r.op == FM_OP_GET ? (r.out_found && host_pinned(r.out_values, r.n * V_) && host_pinned(r.out_found, r.n))
+                                    : (r.out_flags && host_pinned(r.out_flags, r.n));
   }
   // Size the map's per-batch scratch for the largest batch now, not mid-run.
   uint64_t largest = 0;
