@@ -280,12 +280,13 @@ typedef struct fm_serve_stats {
 fm_status fm_serve_log(fm_map* m, fm_serve_request* reqs, uint64_t n_reqs,
                        const fm_serve_config* cfg, fm_serve_stats* stats);
 
-/* Pinned host memory for serving requests (no reference counterpart).  A
- * request buffer (keys, values, outputs) that lies inside one allocation
- * from fm_host_alloc is moved by DMA directly between it and the device
- * staging ring; other buffers go through the pinned staging ring by a host
- * copy.  Results are the same either way.  fm_host_free takes exactly a
- * pointer returned by fm_host_alloc. */
+/* Pinned host memory (no reference counterpart).  A host buffer (keys,
+ * values, outputs) that lies inside one allocation from fm_host_alloc is
+ * moved by DMA directly: by the serving loop between a request and the
+ * device staging ring, and by the host-buffer calls (fm_bulk_*) of 1 MiB or
+ * more between the caller and the device.  Other buffers of that size go
+ * through pinned staging by a host copy.  Results are the same either way.
+ * fm_host_free takes exactly a pointer returned by fm_host_alloc. */
 fm_status fm_host_alloc(uint64_t bytes, void** out);
 fm_status fm_host_free(void* p);
 
