@@ -167,3 +167,42 @@ def test_serve_empty_requests_in_the_mix(pinned):
     assert all(results[i].count == 0 for i in range(0, len(reqs), 7))
     assert len(m) == len(o)
     assert_same_slots(m, o, "after serving")
+
+
+def test_host_buffer_calls_with_pinned_buffers():
+    """fm_bulk_insert / fm_bulk_get / fm_bulk_remove with host buffers of
+    more than 1 MiB that come from fm_host_alloc (moved by DMA directly) and
+    from ordinary memory (copied through pinned staging): both equal the
+    oracle."""
+    import ctypes
+    from flashmap_amd import lib
+    say("host-buffer calls with pinned buffers")
+    cap, n = 1 << 19, 200_000
+    rng = np.random.default_rng(77)
+    keys = rng.integers(0, 1 << 62, n).astype(np.uint64).view(np.uint8).reshape(-1, 8)
+    vals = rng.integers(0, 256, (n, 8), dtype=np.uint8)
+    q = np.concatenate([keys[::2], rng.integers(0, 1 << 62, n // 2).astype(np.uint64).view(np.uint8).reshape(-1, 8)])
+    arena = fm.PinnedArena(6 * n * 8 + n + (1 << 16))
+    pk, pv, pq = arena.copy_of(keys), arena.copy_of(vals), arena.copy_of(q)
+    pout, pfound = arena.take((len(q), 8)), arena.take(len(q))
+    o = OracleAdapter(cap, 8, 8)
+    m = fm.FlashMap.builder(cap, 8, 8).require_gpu().build()
+    vp = ctypes.c_void_p
+    new = ctypes.c_uint64(0)
+    assert lib().fm_bulk_insert(m._h, vp(pk.ctypes.data), vp(pv.ctypes.data), n, ctypes.byref(new)) == fm.OK
+    assert new.value == o.bulk_insert(keys, vals)
+    assert lib().fm_bulk_get(m._h, vp(pq.ctypes.data), len(q), vp(pout.ctypes.data), vp(pfound.ctypes.data)) == fm.OK
+    ev, ef = o.bulk_get(q)
+    assert np.array_equal(pout, ev) and np.array_equal(pfound.astype(bool), ef)
+    rem = ctypes.c_uint64(0)
+    rk = np.ascontiguousarray(keys[: 3 * n // 4])  # pageable, staged
+    assert lib().fm_bulk_remove(m._h, vp(rk.ctypes.data), len(rk), ctypes.byref(rem)) == fm.OK
+    assert rem.value == o.bulk_remove(rk)
+    rk2 = arena.copy_of(np.concatenate([keys[n // 2:], keys[n // 2:]]))  # pinned, with duplicates
+    assert lib().fm_bulk_remove(m._h, vp(rk2.ctypes.data), len(rk2), ctypes.byref(rem)) == fm.OK
+    assert rem.value == o.bulk_remove(rk2)
+    v2, f2 = m.bulk_get(q)  # pageable buffers, staged
+    ev, ef = o.bulk_get(q)
+    assert np.array_equal(v2, ev) and np.array_equal(f2, ef)
+    assert len(m) == len(o)
+    arena.close()
