@@ -9,6 +9,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <chrono>
 #include <cstdio>
 #include <memory>
 #include <cstdlib>
@@ -305,6 +306,11 @@ class GpuBackend final : public Backend {
   fm_status ensure_scratch(uint64_t n);
   fm_status ensure_staging(uint64_t bytes_keys, uint64_t bytes_vals, uint64_t bytes_out, uint64_t bytes_found);
   fm_status read_counters(Counters* out);
+  bool host_seq_arrived(uint32_t seq) const;
+  uint32_t next_ctr_seq() {  // never 0, the value a copied-back op leaves in h_ctr_->done
+    ctr_seq_ = ctr_seq_ == UINT32_MAX ? 1 : ctr_seq_ + 1;
+    return ctr_seq_;
+  }
   fm_status zero_counters();
   fm_status check_device_ptr(const void* p, uint32_t align, const char* what);
   fm_status check_insert_status(const Counters& c) {
@@ -370,6 +376,8 @@ class GpuBackend final : public Backend {
   bool q_pending_ = false;     // the last led insert restored Q_; valid once its status is clean
   bool ctr_zero_ = false;      // ctr_ holds zeros
   bool ctr_in_host_ = false;   // the last op left its final counters in h_ctr_
+  uint32_t ctr_seq_ = 0;       // sequence number of the last counted insert's hand-over
+  uint32_t poll_seq_ = 0;      // nonzero: h_ctr_->done takes this value when the counters are there
 
   // scratch (grown geometrically, reused)
   uint64_t scratch_n_ = 0;
@@ -555,6 +563,7 @@ fm_status GpuBackend::init(const fm_builder& b, std::string* err) {
     *err = hip_msg("hipMalloc (table)", e);
     return FM_ERR_GPU_ALLOC;
   }
+  std::memset(h_ctr_, 0, sizeof(Counters));  // done == 0 matches no sequence number
   ops_ = select_ops(t_.K, t_.V);
   if (ops_.ins_fused_blocks(dev_, &fused_blocks_) != hipSuccess) fused_blocks_ = 0;  // multi-kernel path instead
   if (const char* env = std::getenv("FM_FUSED_SPIN")) fused_spin_ = static_cast<uint32_t>(std::strtoul(env, nullptr, 10));
@@ -667,14 +676,44 @@ fm_status GpuBackend::zero_counters() {
   hipError_t e = ctr_zero_ ? hipSuccess : hipMemsetAsync(ctr_, 0, sizeof(Counters), stream_);
   ctr_zero_ = false;  // the op about to run accumulates into them
   ctr_in_host_ = false;
+  poll_seq_ = 0;
   return e == hipSuccess ? FM_OK : fail(FM_ERR_KERNEL_LAUNCH, hip_msg("hipMemsetAsync", e));
 }
 
+// Waits for h_ctr_->done == seq by polling host memory, for at most
+// kPollLimit; false (the caller then synchronises the stream) if it did not
+// arrive.  A sleeping stream wait wakes the host well after the kernel's
+// last store; a poll sees the hand-over as it lands.
+bool GpuBackend::host_seq_arrived(uint32_t seq) const {
+  constexpr auto kPollLimit = std::chrono::milliseconds(20);
+  const auto t0 = std::chrono::steady_clock::now();
+  for (uint32_t i = 0;; ++i) {
+    if (__atomic_load_n(&h_ctr_->done, __ATOMIC_ACQUIRE) == seq) return true;
+    if ((i & 255u) == 255u && std::chrono::steady_clock::now() - t0 > kPollLimit) return false;
+#if defined(__x86_64__)
+    __builtin_ia32_pause();
+#endif
+  }
+}
+
 fm_status GpuBackend::read_counters(Counters* out) {
+  if (ctr_in_host_ && poll_seq_ && host_seq_arrived(poll_seq_)) {
+    // The last kernel handed the counters over: every table write of the op
+    // precedes that release, and later ops are ordered behind it on stream_.
+    ctr_in_host_ = false;
+    poll_seq_ = 0;
+    const volatile Counters* h = h_ctr_;
+    out->status = h->status;
+    out->nbig = h->nbig;
+    out->count = h->count;
+    out->done = out->nsmall = 0;
+    return FM_OK;
+  }
   hipError_t e = hipSuccess;
   if (!ctr_in_host_) e = hipMemcpyAsync(h_ctr_, ctr_, sizeof(Counters), hipMemcpyDeviceToHost, stream_);
   if (e == hipSuccess) e = hipStreamSynchronize(stream_);
   ctr_in_host_ = false;
+  poll_seq_ = 0;
   if (e != hipSuccess) return fail(FM_ERR_TRANSFER, hip_msg("counter readback", e));
   *out = *h_ctr_;
   return FM_OK;
@@ -766,6 +805,7 @@ fm_status GpuBackend::run_insert(const uint8_t* d_keys, const uint8_t* d_vals, u
     q_pending_ = true;  // Q_ == freebits once the status shows no failed dry pass or stall
     ctr_zero_ = true;
     ctr_in_host_ = true;
+    poll_seq_ = 0;
     return FM_OK;
   }
   if ((st = zero_counters()) != FM_OK) return st;
@@ -803,18 +843,20 @@ fm_status GpuBackend::run_insert(const uint8_t* d_keys, const uint8_t* d_vals, u
         return st;
     }
     Prof p(this, kKidInsBits);
+    const uint32_t seq = next_ctr_seq();
     if (!dedup && n * 16 >= t_.cap) {
       // Most bitmap words change: recompute them all from the flags (one
       // coalesced pass) instead of one atomic per key; at most 1024
       // workgroups, so at most 1024 completion tickets.
       const uint32_t g = std::min<uint32_t>(grid_for(t_.cap, 1024), 1024);
-      hipLaunchKernelGGL(dev::k_ins_bits_rebuild, dim3(g), dim3(1024), 0, stream_, t, ctr_, Q_, h_ctr_);
+      hipLaunchKernelGGL(dev::k_ins_bits_rebuild, dim3(g), dim3(1024), 0, stream_, t, ctr_, Q_, h_ctr_, seq);
     } else {
       // 1024-thread workgroups: one completion ticket per 1024 keys
       hipLaunchKernelGGL(dev::k_ins_bits, dim3(grid_for(n, 1024)), dim3(1024), 0, stream_, t, dslot_, n, ctr_, Q_,
-                         h_ctr_);
+                         h_ctr_, seq);
     }
     if ((st = check_launch(hipGetLastError(), "k_ins_bits")) != FM_OK) return st;
+    poll_seq_ = seq;
     q_pending_ = true;     // Q_ == freebits once the status shows no failed dry pass
     ctr_zero_ = true;      // k_ins_bits zeroed the counters ...
     ctr_in_host_ = true;   // ... after writing them to h_ctr_
@@ -874,8 +916,16 @@ fm_status GpuBackend::run_remove(const uint8_t* d_keys, uint64_t n, uint64_t* re
   }
   {
     Prof p(this, kKidRemApply);
-    hipLaunchKernelGGL(dev::k_rem_apply, dim3(grid_for(n, blk)), dim3(blk), 0, stream_, t, ev_, action_, n, ctr_);
+    // A counted remove reads the counters next: the last workgroup hands
+    // them to h_ctr_ (zeroing ctr_) and the host polls for the hand-over.
+    const uint32_t seq = async ? 0 : next_ctr_seq();
+    hipLaunchKernelGGL(dev::k_rem_apply, dim3(grid_for(n, blk)), dim3(blk), 0, stream_, t, ev_, action_, n, ctr_,
+                       async ? nullptr : h_ctr_, seq);
     if ((st = check_launch(hipGetLastError(), "k_rem_apply")) != FM_OK) return st;
+    if (!async) {
+      ctr_zero_ = ctr_in_host_ = true;
+      poll_seq_ = seq;
+    }
   }
   if (async) {
     // No status readback: the replay steps are launched unconditionally and

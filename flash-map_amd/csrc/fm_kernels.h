@@ -361,18 +361,23 @@ __device__ __forceinline__ void wave_add(unsigned long long* dst, uint32_t v) {
 // Adds per-thread counts into *dst with one atomic per workgroup: atomics
 // on one address serialise (about 14 ns each on MI355X), so counters are
 // reduced in the workgroup first.  Called by every thread of the workgroup.
-__device__ inline void block_add(unsigned long long* dst, unsigned long long v) {
+// Workgroup sum of v, valid in thread 0.
+__device__ inline unsigned long long block_sum(unsigned long long v) {
   __shared__ unsigned long long part[16];
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
   __syncthreads();  // part[] may still be read by a previous call
   if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned long long sum = 0;
+  unsigned long long sum = 0;
+  if (threadIdx.x == 0)
     for (uint32_t w = 0; w < (blockDim.x + 63) / 64; ++w) sum += part[w];
-    if (sum) atomicAdd(dst, sum);
-  }
+  return sum;
+}
+
+__device__ inline void block_add(unsigned long long* dst, unsigned long long v) {
+  const unsigned long long sum = block_sum(v);
+  if (threadIdx.x == 0 && sum) atomicAdd(dst, sum);
 }
 
 // ---------------------------------------------------------------------------
@@ -1192,15 +1197,31 @@ k_ins_big(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __restric
   }
 }
 
+// The last workgroup of an insert writes the op's final counters to
+// host_out (pinned host memory) and zeroes them for the next op.  host_out->
+// done then takes the op's sequence number with a system-scope release, so a
+// host that polls it reads the other fields complete without waiting for the
+// kernel to end.  The release writes back this XCD's L2 once per op.
+__device__ inline void hand_to_host(Counters* ctr, Counters* host_out, uint32_t seq, unsigned long long count) {
+  volatile Counters* h = host_out;
+  h->status = ld_agent(&ctr->status);
+  h->nbig = ld_agent(&ctr->nbig);
+  h->count = count;
+  h->nsmall = 0;
+  *ctr = Counters{};
+  __hip_atomic_store(&host_out->done, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 // Publishes the batch's real claims (a subset of its dry claims) in both
 // bitmaps and resets owner[] for the next op.  Q (optional): the dry pass's
 // claim bitmap; a dry claim that ended unused (the key updated an existing
 // slot) gets its Q bit back, which leaves Q equal to the new freebits, so
 // the next insert needs no copy.  host_out (optional): the last workgroup
-// writes the op's final counters there and zeroes them for the next op.
+// writes the op's final counters there and zeroes them for the next op
+// (hand_to_host).
 static __global__ void __launch_bounds__(1024)
 k_ins_bits(Table t, const uint32_t* __restrict__ dslot, uint64_t n, Counters* ctr, unsigned long long* Q,
-           Counters* host_out) {
+           Counters* host_out, uint32_t seq) {
   const bool failed = (ctr->status & kStatusDryFail) != 0;
   for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
        i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
@@ -1221,20 +1242,14 @@ k_ins_bits(Table t, const uint32_t* __restrict__ dslot, uint64_t n, Counters* ct
     }
     t.owner[s] = kNoOwner;
   }
-  // No fence: this kernel changes none of the published fields, every
-  // workgroup's read of ctr->status precedes its ticket, and the host reads
-  // host_out after the kernel has completed.  (An agent-scope fence writes
-  // back the L2 on this part, which costs more than the whole kernel.)
+  // No fence: this kernel changes none of the published fields and every
+  // workgroup's read of ctr->status precedes its ticket.  (An agent-scope
+  // fence writes back the L2 on this part, which costs more than the whole
+  // kernel; only the last workgroup's hand-over releases, once.)
   if (!host_out) return;
   __syncthreads();
-  if (threadIdx.x == 0 && atomicAdd(&ctr->done, 1u) == gridDim.x - 1) {
-    Counters out{};
-    out.status = ld_agent(&ctr->status);
-    out.nbig = ld_agent(&ctr->nbig);
-    out.count = __hip_atomic_load(&ctr->count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    *host_out = out;
-    *ctr = Counters{};
-  }
+  if (threadIdx.x == 0 && atomicAdd(&ctr->done, 1u) == gridDim.x - 1)
+    hand_to_host(ctr, host_out, seq, __hip_atomic_load(&ctr->count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
 }
 
 // k_ins_bits for a batch that touches most bitmap words (the caller uses it
@@ -1246,7 +1261,7 @@ k_ins_bits(Table t, const uint32_t* __restrict__ dslot, uint64_t n, Counters* ct
 // the flags untouched, so the words come out unchanged.  The last
 // workgroup hands the counters to host memory as k_ins_bits does.
 static __global__ void __launch_bounds__(1024)
-k_ins_bits_rebuild(Table t, Counters* ctr, unsigned long long* Q, Counters* host_out) {
+k_ins_bits_rebuild(Table t, Counters* ctr, unsigned long long* Q, Counters* host_out, uint32_t seq) {
   const uint64_t words = (t.cap + 63) >> 6;
   const uint32_t lane = threadIdx.x & 63u;
   const uint64_t wave = (static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
@@ -1265,14 +1280,8 @@ k_ins_bits_rebuild(Table t, Counters* ctr, unsigned long long* Q, Counters* host
   }
   if (!host_out) return;
   __syncthreads();
-  if (threadIdx.x == 0 && atomicAdd(&ctr->done, 1u) == gridDim.x - 1) {
-    Counters out{};
-    out.status = ld_agent(&ctr->status);
-    out.nbig = ld_agent(&ctr->nbig);
-    out.count = __hip_atomic_load(&ctr->count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    *host_out = out;
-    *ctr = Counters{};
-  }
+  if (threadIdx.x == 0 && atomicAdd(&ctr->done, 1u) == gridDim.x - 1)
+    hand_to_host(ctr, host_out, seq, __hip_atomic_load(&ctr->count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
 }
 
 // ---------------------------------------------------------------------------
@@ -2166,9 +2175,16 @@ k_rem_check(Table t, const unsigned long long* __restrict__ ev, uint64_t n, uint
 }
 
 // Pass 3: apply (only when the batch had no conflict) and reset owner[].
+// host_out (optional): the last workgroup hands the final counters to host
+// memory (hand_to_host).  Each workgroup then adds its count and its
+// completion ticket in one atomic on ctr->count (ticket above bit
+// kRemTicketShift), so the last one holds every other's count without a
+// fence.  Without host_out the counters stay on the device (an asynchronous
+// remove's gated replay reads the status after this pass).
+constexpr int kRemTicketShift = 40;  // counts < 2^40 keys, tickets < 2^24 workgroups
 static __global__ void __launch_bounds__(1024)
 k_rem_apply(Table t, const unsigned long long* __restrict__ ev, const uint8_t* __restrict__ action, uint64_t n,
-            Counters* ctr) {
+            Counters* ctr, Counters* host_out, uint32_t seq) {
   const bool conflict = (ctr->status & kStatusConflict) != 0;
   unsigned long long total = 0;
   for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * blockDim.x;; base += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
@@ -2188,7 +2204,16 @@ k_rem_apply(Table t, const unsigned long long* __restrict__ ev, const uint8_t* _
     if (t.outcome && i < n && !conflict) t.outcome[i] = static_cast<uint8_t>(removed);
     total += removed;
   }
-  block_add(&ctr->count, total);
+  if (!host_out) {
+    block_add(&ctr->count, total);
+    return;
+  }
+  const unsigned long long sum = block_sum(total);
+  if (threadIdx.x == 0) {
+    constexpr unsigned long long kMask = (1ull << kRemTicketShift) - 1;
+    const unsigned long long old = atomicAdd(&ctr->count, (1ull << kRemTicketShift) + sum);
+    if ((old >> kRemTicketShift) == gridDim.x - 1) hand_to_host(ctr, host_out, seq, (old & kMask) + sum);
+  }
 }
 
 // Exact replay, step 1: run id = first EMPTY slot at/after home (removes
