@@ -515,18 +515,26 @@ __device__ inline uint32_t insert_walk(const Table& t, Blob<KS> ck, Blob<VS> cv,
     }
     const uint64_t idx = (home + p) & t.mask;
     const uint32_t f = t.flags[idx];
+    // Static key sizes: the slot's key is loaded with its flag word (one
+    // round trip for the compare or the eviction instead of two).
+    Blob<KS> rk;
+    if constexpr (KS > 0) rk.load(t.keys + idx * K, K);
     const uint32_t st = f & 0xFu;
     const uint64_t d = f >> 4;
     if (st == kOccupied) {
       // equal keys share a home, hence sit at displacement p == d
-      if (d == p && ck.eq(t.keys + idx * K, K)) {
+      bool same;
+      if constexpr (KS > 0) same = d == p && ck.eqb(rk);
+      else same = d == p && ck.eq(t.keys + idx * K, K);
+      if (same) {
         cv.store(t.vals + idx * V, V);  // update in place, not new
         return 0;
       }
       if (p > d) {  // Robin Hood eviction: swap and carry the resident on
         Blob<KS> ek;
         Blob<VS> ev;
-        ek.load(t.keys + idx * K, K);
+        if constexpr (KS > 0) ek = rk;
+        else ek.load(t.keys + idx * K, K);
         ev.load(t.vals + idx * V, V);
         ck.store(t.keys + idx * K, K);
         cv.store(t.vals + idx * V, V);
