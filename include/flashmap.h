@@ -127,7 +127,10 @@ fm_status fm_last_table_full(uint64_t* occupied, uint64_t* capacity,
  * GPU backend only; the CPU backend returns FM_ERR_GPU_REQUIRED
  * (lib.rs:235).  Device buffers come from fm_alloc_device / fm_upload (or
  * any device allocation on the map's device, 16-byte aligned).  Work is
- * ordered on the map's stream; calls that return a count synchronise. */
+ * ordered on the map's stream.  Calls that return a count return once the
+ * op is done: the caller's buffers are read and every table write is made
+ * (the op's last kernel hands the count to host memory and may still be
+ * exiting; fm_synchronize also waits for that). */
 
 /* FlashMap::device (lib.rs:218-224): the hipStream_t of the map. */
 void* fm_device_stream(fm_map* m);
