@@ -343,8 +343,13 @@ class GpuBackend final : public Backend {
 
   // Device pipelines (inputs on the device, ordered on stream_).
   fm_status run_get(const uint8_t* d_keys, uint64_t n, uint8_t* d_out, uint8_t* d_found);
-  fm_status run_insert(const uint8_t* d_keys, const uint8_t* d_vals, uint64_t n,
-                       uint8_t* outcome = nullptr);  // leaves count in ctr
+  // leaves count in ctr.  defer_big (counted calls, which then read the
+  // counters with read_insert_counters): the led path launches no k_ins_big;
+  // its settle pass reports queued closures and the host runs them then.
+  fm_status run_insert(const uint8_t* d_keys, const uint8_t* d_vals, uint64_t n, uint8_t* outcome = nullptr,
+                       bool defer_big = false);
+  fm_status launch_bits(const Table& t, uint64_t n, bool dedup, bool defer_big);
+  fm_status read_insert_counters(Counters* out);
   // async: no status readback and no count (serving reads the outcome
   // flags); the conflict replay is gated on the device.
   fm_status run_remove(const uint8_t* d_keys, uint64_t n, uint64_t* removed, uint8_t* outcome = nullptr,
@@ -378,6 +383,14 @@ class GpuBackend final : public Backend {
   bool ctr_in_host_ = false;   // the last op left its final counters in h_ctr_
   uint32_t ctr_seq_ = 0;       // sequence number of the last counted insert's hand-over
   uint32_t poll_seq_ = 0;      // nonzero: h_ctr_->done takes this value when the counters are there
+  struct PendingBig {  // the last deferred led insert, for its k_ins_big and settle
+    Table t;
+    const uint8_t* keys;
+    const uint8_t* vals;
+    uint64_t n;
+    bool dedup;
+  };
+  PendingBig pend_{};
 
   // scratch (grown geometrically, reused)
   uint64_t scratch_n_ = 0;
@@ -680,6 +693,49 @@ fm_status GpuBackend::zero_counters() {
   return e == hipSuccess ? FM_OK : fail(FM_ERR_KERNEL_LAUNCH, hip_msg("hipMemsetAsync", e));
 }
 
+// The settle pass of a led insert: publishes the real claims, restores Q_
+// and hands the counters to h_ctr_ (see k_ins_bits).
+fm_status GpuBackend::launch_bits(const Table& t, uint64_t n, bool dedup, bool defer_big) {
+  Prof p(this, kKidInsBits);
+  const uint32_t seq = next_ctr_seq();
+  if (!dedup && n * 16 >= t_.cap) {
+    // Most bitmap words change: recompute them all from the flags (one
+    // coalesced pass) instead of one atomic per key; at most 1024
+    // workgroups, so at most 1024 completion tickets.
+    const uint32_t g = std::min<uint32_t>(grid_for(t_.cap, 1024), 1024);
+    hipLaunchKernelGGL(dev::k_ins_bits_rebuild, dim3(g), dim3(1024), 0, stream_, t, ctr_, Q_, h_ctr_, seq, defer_big);
+  } else {
+    // 1024-thread workgroups: one completion ticket per 1024 keys
+    hipLaunchKernelGGL(dev::k_ins_bits, dim3(grid_for(n, 1024)), dim3(1024), 0, stream_, t, dslot_, n, ctr_, Q_,
+                       h_ctr_, seq, defer_big);
+  }
+  fm_status st;
+  if ((st = check_launch(hipGetLastError(), "k_ins_bits")) != FM_OK) return st;
+  poll_seq_ = seq;
+  q_pending_ = true;     // Q_ == freebits once the status shows no failed dry pass
+  ctr_zero_ = true;      // k_ins_bits zeroed the counters (a deferred pass: once the settle after k_ins_big ran) ...
+  ctr_in_host_ = true;   // ... after writing them to h_ctr_
+  return FM_OK;
+}
+
+// read_counters for a counted insert launched with defer_big: when the
+// settle pass reports closures queued for k_ins_big, runs k_ins_big and the
+// settle now (the device counters still hold the count and the queue) and
+// reads the final counters.
+fm_status GpuBackend::read_insert_counters(Counters* out) {
+  fm_status st;
+  if ((st = read_counters(out)) != FM_OK) return st;
+  if (!(out->status & dev::kStatusBigPending)) return FM_OK;
+  {
+    Prof p(this, kKidInsBig);
+    if ((st = check_launch(ops_.ins_big(pend_.t, pend_.keys, pend_.vals, pend_.n, Q_, big_, ctr_, stream_),
+                           "k_ins_big")) != FM_OK)
+      return st;
+  }
+  if ((st = launch_bits(pend_.t, pend_.n, pend_.dedup, false)) != FM_OK) return st;
+  return read_counters(out);
+}
+
 // Waits for h_ctr_->done == seq by polling host memory, for at most
 // kPollLimit; false (the caller then synchronises the stream) if it did not
 // arrive.  A sleeping stream wait wakes the host well after the kernel's
@@ -763,7 +819,8 @@ fm_status GpuBackend::ensure_dedup(uint64_t n) {
   return FM_OK;
 }
 
-fm_status GpuBackend::run_insert(const uint8_t* d_keys, const uint8_t* d_vals, uint64_t n, uint8_t* outcome) {
+fm_status GpuBackend::run_insert(const uint8_t* d_keys, const uint8_t* d_vals, uint64_t n, uint8_t* outcome,
+                                 bool defer_big) {
   fm_status st;
   Table t = t_;
   t.outcome = outcome;
@@ -837,30 +894,14 @@ fm_status GpuBackend::run_insert(const uint8_t* d_keys, const uint8_t* d_vals, u
                              "k_ins_leader")) != FM_OK)
         return st;
     }
-    {
+    if (defer_big) {
+      pend_ = PendingBig{t, d_keys, d_vals, n, dedup};
+    } else {
       Prof p(this, kKidInsBig);
       if ((st = check_launch(ops_.ins_big(t, d_keys, d_vals, n, Q_, big_, ctr_, stream_), "k_ins_big")) != FM_OK)
         return st;
     }
-    Prof p(this, kKidInsBits);
-    const uint32_t seq = next_ctr_seq();
-    if (!dedup && n * 16 >= t_.cap) {
-      // Most bitmap words change: recompute them all from the flags (one
-      // coalesced pass) instead of one atomic per key; at most 1024
-      // workgroups, so at most 1024 completion tickets.
-      const uint32_t g = std::min<uint32_t>(grid_for(t_.cap, 1024), 1024);
-      hipLaunchKernelGGL(dev::k_ins_bits_rebuild, dim3(g), dim3(1024), 0, stream_, t, ctr_, Q_, h_ctr_, seq);
-    } else {
-      // 1024-thread workgroups: one completion ticket per 1024 keys
-      hipLaunchKernelGGL(dev::k_ins_bits, dim3(grid_for(n, 1024)), dim3(1024), 0, stream_, t, dslot_, n, ctr_, Q_,
-                         h_ctr_, seq);
-    }
-    if ((st = check_launch(hipGetLastError(), "k_ins_bits")) != FM_OK) return st;
-    poll_seq_ = seq;
-    q_pending_ = true;     // Q_ == freebits once the status shows no failed dry pass
-    ctr_zero_ = true;      // k_ins_bits zeroed the counters ...
-    ctr_in_host_ = true;   // ... after writing them to h_ctr_
-    return FM_OK;
+    return launch_bits(t, n, dedup, defer_big);
   }
   {
     Prof p(this, kKidInsDry);
@@ -1110,9 +1151,9 @@ fm_status GpuBackend::bulk_insert(const void* keys, const void* values, uint64_t
   if ((e = hipMemcpyAsync(st_keys_, ks, kb, hipMemcpyHostToDevice, stream_)) != hipSuccess ||
       (e = hipMemcpyAsync(st_vals_, vs, vb, hipMemcpyHostToDevice, stream_)) != hipSuccess)
     return fail(FM_ERR_TRANSFER, hip_msg("H2D pairs", e));
-  if ((st = run_insert(st_keys_, st_vals_, n)) != FM_OK) return st;
+  if ((st = run_insert(st_keys_, st_vals_, n, nullptr, true)) != FM_OK) return st;
   Counters c;
-  if ((st = read_counters(&c)) != FM_OK) return st;
+  if ((st = read_insert_counters(&c)) != FM_OK) return st;
   if ((st = check_insert_status(c)) != FM_OK) return st;
   q_synced_ = q_pending_;  // a clean led insert left Q_ equal to freebits
   len_ += c.count;
@@ -1257,14 +1298,15 @@ fm_status GpuBackend::bulk_insert_device(const void* d_keys, const void* d_vals,
   if ((st = check_device_ptr(d_keys, vec_align(t_.K), "keys")) != FM_OK ||
       (st = check_device_ptr(d_vals, vec_align(t_.V), "values")) != FM_OK)
     return st;
-  if ((st = run_insert(static_cast<const uint8_t*>(d_keys), static_cast<const uint8_t*>(d_vals), n)) != FM_OK)
+  if ((st = run_insert(static_cast<const uint8_t*>(d_keys), static_cast<const uint8_t*>(d_vals), n, nullptr,
+                       counted)) != FM_OK)
     return st;
   if (!counted) {  // gpu.rs:670: assumes every key new
     len_ += n;
     return FM_OK;
   }
   Counters c;
-  if ((st = read_counters(&c)) != FM_OK) return st;
+  if ((st = read_insert_counters(&c)) != FM_OK) return st;
   if ((st = check_insert_status(c)) != FM_OK) return st;
   q_synced_ = q_pending_;  // a clean led insert left Q_ equal to freebits
   len_ += c.count;

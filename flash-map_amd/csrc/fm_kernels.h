@@ -53,6 +53,7 @@ constexpr uint32_t kStatusDryFail = 1u;   // dry pass found no free slot
 constexpr uint32_t kStatusConflict = 2u;  // remove batch needs exact replay
 constexpr uint32_t kStatusBudget = 4u;    // an insert walk exceeded cap+2 steps
 constexpr uint32_t kStatusStall = 8u;     // fused insert: a grid barrier timed out
+constexpr uint32_t kStatusBigPending = 16u;  // host copy only: k_ins_big and the settle still to run
 
 // Event kinds of a probe (get / remove).
 constexpr uint32_t kEvNone = 0u;      // probed all cap slots
@@ -1220,18 +1221,36 @@ __device__ inline void hand_to_host(Counters* ctr, Counters* host_out, uint32_t 
   __hip_atomic_store(&host_out->done, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
+// The hand-over of a settle pass that found closures queued for k_ins_big
+// (not yet run): the host copy's status carries kStatusBigPending, the
+// device counters stay as they are (the count so far, the queue length)
+// except the completion tickets, which the settle after k_ins_big reuses.
+__device__ inline void hand_pending(Counters* ctr, Counters* host_out, uint32_t seq) {
+  volatile Counters* h = host_out;
+  h->status = ld_agent(&ctr->status) | kStatusBigPending;
+  h->nbig = ld_agent(&ctr->nbig);
+  h->count = __hip_atomic_load(&ctr->count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  h->nsmall = 0;
+  ctr->done = 0;
+  __hip_atomic_store(&host_out->done, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 // Publishes the batch's real claims (a subset of its dry claims) in both
 // bitmaps and resets owner[] for the next op.  Q (optional): the dry pass's
 // claim bitmap; a dry claim that ended unused (the key updated an existing
 // slot) gets its Q bit back, which leaves Q equal to the new freebits, so
 // the next insert needs no copy.  host_out (optional): the last workgroup
 // writes the op's final counters there and zeroes them for the next op
-// (hand_to_host).
+// (hand_to_host).  defer_big: k_ins_big has not run; if the leader queued
+// closures for it, their slots are not final, so the pass settles nothing
+// (deduplication state included) and hands over kStatusBigPending instead.
 static __global__ void __launch_bounds__(1024)
 k_ins_bits(Table t, const uint32_t* __restrict__ dslot, uint64_t n, Counters* ctr, unsigned long long* Q,
-           Counters* host_out, uint32_t seq) {
+           Counters* host_out, uint32_t seq, bool defer_big) {
   const bool failed = (ctr->status & kStatusDryFail) != 0;
-  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+  const bool pending = defer_big && ld_agent(&ctr->nbig) != 0;  // the same in every workgroup
+  const uint64_t end = pending ? 0 : n;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < end;
        i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
     if (t.dd_first) {
       const uint32_t b = t.dd_bucket[i] & ~kDupKey;
@@ -1256,8 +1275,10 @@ k_ins_bits(Table t, const uint32_t* __restrict__ dslot, uint64_t n, Counters* ct
   // kernel; only the last workgroup's hand-over releases, once.)
   if (!host_out) return;
   __syncthreads();
-  if (threadIdx.x == 0 && atomicAdd(&ctr->done, 1u) == gridDim.x - 1)
-    hand_to_host(ctr, host_out, seq, __hip_atomic_load(&ctr->count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+  if (threadIdx.x == 0 && atomicAdd(&ctr->done, 1u) == gridDim.x - 1) {
+    if (pending) hand_pending(ctr, host_out, seq);
+    else hand_to_host(ctr, host_out, seq, __hip_atomic_load(&ctr->count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+  }
 }
 
 // k_ins_bits for a batch that touches most bitmap words (the caller uses it
@@ -1269,8 +1290,10 @@ k_ins_bits(Table t, const uint32_t* __restrict__ dslot, uint64_t n, Counters* ct
 // the flags untouched, so the words come out unchanged.  The last
 // workgroup hands the counters to host memory as k_ins_bits does.
 static __global__ void __launch_bounds__(1024)
-k_ins_bits_rebuild(Table t, Counters* ctr, unsigned long long* Q, Counters* host_out, uint32_t seq) {
-  const uint64_t words = (t.cap + 63) >> 6;
+k_ins_bits_rebuild(Table t, Counters* ctr, unsigned long long* Q, Counters* host_out, uint32_t seq,
+                   bool defer_big) {
+  const bool pending = defer_big && ld_agent(&ctr->nbig) != 0;  // as in k_ins_bits
+  const uint64_t words = pending ? 0 : (t.cap + 63) >> 6;
   const uint32_t lane = threadIdx.x & 63u;
   const uint64_t wave = (static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
   const uint64_t waves = (static_cast<uint64_t>(gridDim.x) * blockDim.x) >> 6;
@@ -1288,8 +1311,10 @@ k_ins_bits_rebuild(Table t, Counters* ctr, unsigned long long* Q, Counters* host
   }
   if (!host_out) return;
   __syncthreads();
-  if (threadIdx.x == 0 && atomicAdd(&ctr->done, 1u) == gridDim.x - 1)
-    hand_to_host(ctr, host_out, seq, __hip_atomic_load(&ctr->count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+  if (threadIdx.x == 0 && atomicAdd(&ctr->done, 1u) == gridDim.x - 1) {
+    if (pending) hand_pending(ctr, host_out, seq);
+    else hand_to_host(ctr, host_out, seq, __hip_atomic_load(&ctr->count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+  }
 }
 
 // ---------------------------------------------------------------------------
