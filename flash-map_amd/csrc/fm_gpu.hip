@@ -894,14 +894,18 @@ fm_status GpuBackend::run_insert(const uint8_t* d_keys, const uint8_t* d_vals, u
                              "k_ins_leader")) != FM_OK)
         return st;
     }
-    if (defer_big) {
+    // Only sparse batches (settled per key) defer: a batch that covers a
+    // large share of the table (n * 16 >= cap) nearly always queues big
+    // closures, and deferring would only add a round trip.
+    const bool defer = defer_big && (dedup || n * 16 < t_.cap);
+    if (defer) {
       pend_ = PendingBig{t, d_keys, d_vals, n, dedup};
     } else {
       Prof p(this, kKidInsBig);
       if ((st = check_launch(ops_.ins_big(t, d_keys, d_vals, n, Q_, big_, ctr_, stream_), "k_ins_big")) != FM_OK)
         return st;
     }
-    return launch_bits(t, n, dedup, defer_big);
+    return launch_bits(t, n, dedup, defer);
   }
   {
     Prof p(this, kKidInsDry);
