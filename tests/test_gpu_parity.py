@@ -604,3 +604,29 @@ def test_gpu_max_capacity_wraps_at_2_31():
     m.close()
     with pytest.raises(fm.FlashMapError):
         fm.FlashMap.builder(cap + 1, 8, 8).require_gpu().build()
+
+
+@pytest.mark.parametrize("K,V", [(8, 8), (32, 128)])
+def test_gpu_sparse_batches_that_queue_big_closures(K, V):
+    """Counted inserts of sparse batches (n * 16 < cap) launch no k_ins_big up
+    front: when the leader queues closures, the settle pass reports them and
+    the host runs k_ins_big and the settle again.  In a near-full table most
+    such batches queue closures and the smallest queue none; batches of
+    updates (with repeated keys) sit between them.  The whole slot array is
+    compared with the oracle after every op."""
+    say(f"sparse batches with queued closures K={K} V={V}")
+    rng = np.random.default_rng(700 + K)
+    cap = 1 << 13
+    m, o = gpu_make(cap, K, V), OracleAdapter(cap, K, V)
+    keys = rng.integers(0, 256, (cap, K), dtype=np.uint8)
+    vals = rng.integers(0, 256, (cap, V), dtype=np.uint8)
+    lo = cap - 900
+    ops = [("ins", keys[:lo], vals[:lo])]  # dense: k_ins_big launched directly
+    for n in [8, 300, 1, 200, 50, 40]:  # every n * 16 < cap
+        ops.append(("ins", keys[lo:lo + n], vals[lo:lo + n]))
+        lo += n
+        ops.append(("ins", keys[rng.integers(0, lo, n)], vals[rng.integers(0, cap, n)]))
+    ops.append(("rem", keys[:3000:3]))
+    ops.append(("ins", keys[:400:2], vals[:200]))
+    ops.append(("get", keys))
+    apply_and_compare(m, o, ops)
