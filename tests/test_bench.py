@@ -51,3 +51,24 @@ def test_bench_config3_reduced_line():
     assert d["check"]["rows_exact"] is True
     assert d["check"]["probes_checked"] == 2_000_000
     assert d["value"] > 0 and 0 < d["roofline"]["frac"] < 1
+
+
+@pytest.mark.gpu
+def test_bench_default_line_has_every_section():
+    """The default run (reduced): the account-state line with a section per
+    other configuration and the host-buffer calls, every check passing."""
+    print("bench default reduced", flush=True)
+    d = json_line(run_bench("--scale", "0.02", "--steps", "2", "--warmup", "1", "--tables", "24", "--requests",
+                            "60", "--rates", "2000,10000", "--cpu-seconds", "0.5", "--no-autotune", timeout=300))
+    assert d["check"]["bit_exact"] is True
+    for name in ("c1", "c3", "c4", "c5", "host_api"):
+        assert d[name]["ok"] is True, (name, d[name])
+    assert d["c1"]["check"]["bit_exact"] is True
+    assert d["c3"]["check"]["rows_exact"] is True
+    assert d["c4"]["check"]["fused_equals_plain"] and d["c4"]["check"]["oracle_exact"]
+    assert d["c4"]["check"]["oracle_tables"] == 24
+    assert all(l["mismatches"] == 0 and l["checked"] == 60 for l in d["c5"]["loads"])
+    assert d["host_api"]["cap_2M"]["pageable_exact"] and d["host_api"]["cap_2M"]["pinned_exact"]
+    assert d["targets"]["all_checks_pass"] is True
+    assert d["cpu_baseline"]["cores"] == len(os.sched_getaffinity(0))
+    assert len(json.dumps(d, separators=(",", ":"))) < 8000

@@ -4,7 +4,9 @@ Checks, each chosen so that a plausible restatement mistake fails one:
   * the reference's own known-answer tests (src/hash.rs:92-117);
   * MurmurHash3's fmix64 against the published SMHasher verification value
     of MurmurHash3_x64_128 (0x6384BA69), then the reference's murmur3 for
-    8-byte keys expressed through that verified fmix64;
+    keys of 0 to 47 bytes (chunk chaining, the tail branch, the length fold)
+    assembled from that verified fmix64, with planted restatement mistakes
+    that the expected values must each reject;
   * C vs an independent pure-Python restatement (oracle/pyoracle.py) on
     hashes, random collision-heavy histories and exhaustive short histories;
   * closed-form invariants (displacement, len == #OCCUPIED, dict model on
@@ -113,6 +115,92 @@ def test_murmur3_structure_through_verified_fmix64():
         assert hash_bytes(k.to_bytes(8, "little"), 1) == _fmix64(h ^ 8)
     # 0-byte key: no chunk, no tail (rem == 0 and len % 8 == 0)
     assert hash_bytes(b"", 1) == _fmix64(0x9E3779B97F4A7C15 ^ 0)
+
+
+_C2 = 0xC4CEB9FE1A85EC53  # fmix64's second multiplier (verified with it above)
+_C2_INV = pow(_C2, -1, 1 << 64)
+
+
+def _xs33(x):
+    return x ^ (x >> 33)
+
+
+def _chunk_mix(k):
+    # k*c1, xs33, *c2, xs33 == fmix64 after undoing its leading xs33 (an
+    # involution on 64 bits)
+    return _fmix64(_xs33(k))
+
+
+def _tail_mix(r):
+    # xs33(r*c1), the first half of fmix64: fmix64(xs33(r)) = xs33(c2 * xs33(c1*r)),
+    # so xs33(c1*r) = c2^-1 * xs33(fmix64(xs33(r))) mod 2^64
+    return (_C2_INV * _xs33(_fmix64(_xs33(r)))) & M64
+
+
+def _murmur_via_fmix64(key: bytes, chunk_mul=5, chunk_add=0x52DCE729, tail_when_zero=True, tail_order="little",
+                       chain=True, fold_len=True, tail_mix="half"):
+    """The reference composition (hash.rs:43-77) assembled from the verified
+    fmix64; the keyword arguments plant restatement mistakes."""
+    h = 0x9E3779B97F4A7C15
+    n = len(key)
+    for c in range(n // 8):
+        k = int.from_bytes(key[8 * c:8 * c + 8], "little")
+        if chain:
+            h = ((h ^ _chunk_mix(k)) * chunk_mul + chunk_add) & M64
+        else:  # mistake: each chunk restarts from the seed
+            h = ((0x9E3779B97F4A7C15 ^ _chunk_mix(k)) * chunk_mul + chunk_add) & M64
+    tail = key[8 * (n // 8):]
+    r = int.from_bytes(tail, tail_order) if tail else 0
+    if (r != 0 or n % 8 != 0) if tail_when_zero else r != 0:
+        if tail_mix == "half":
+            h ^= _tail_mix(r)
+        elif tail_mix == "full":  # mistake: the whole fmix64 on the tail
+            h ^= _fmix64(r)
+    return _fmix64(h ^ n) if fold_len else _fmix64(h)
+
+
+MURMUR_LENGTHS = (3, 5, 12, 16, 24, 32, 33, 40, 47)
+
+
+def _murmur_keys():
+    rng = np.random.default_rng(29)
+    keys = []
+    for n in MURMUR_LENGTHS:
+        keys += [bytes(rng.integers(0, 256, n, dtype=np.uint8)) for _ in range(40)]
+        keys.append(bytes(n))  # all-zero key
+        keys.append(bytes(rng.integers(0, 256, n - n % 8, dtype=np.uint8)) + bytes(n % 8))  # zero tail only
+    return keys
+
+
+def test_murmur3_tail_and_chunk_chain_through_verified_fmix64():
+    # Keys of 3..47 bytes: chunk chaining (2..5 chunks), the tail branch
+    # (K % 8 != 0, hash.rs:61-68) including all-zero tails, and the length
+    # fold; expected values come from fmix64 (SMHasher-pinned), the xs33
+    # involution and the inverse of fmix64's constant, not from the oracle.
+    for key in _murmur_keys():
+        exp = _murmur_via_fmix64(key)
+        assert hash_bytes(key, 1) == exp, (len(key), key.hex())
+        assert pyoracle.murmur3_hash(key) == exp
+
+
+@pytest.mark.parametrize("mistake", [dict(chunk_mul=3), dict(chunk_add=0x52DCE728), dict(tail_order="big"),
+                                     dict(chain=False), dict(fold_len=False), dict(tail_mix="none"),
+                                     dict(tail_mix="full")])
+def test_murmur3_expectation_rejects_planted_mistakes(mistake):
+    # Each plausible restatement mistake of hash.rs:43-77, planted in the
+    # fmix64-based composition, disagrees with the oracle on some key of the
+    # set above, so that set pins the tail branch and the chunk chain.
+    keys = _murmur_keys()
+    assert any(_murmur_via_fmix64(k, **mistake) != hash_bytes(k, 1) for k in keys)
+
+
+def test_murmur3_zero_tail_condition_is_unobservable():
+    # The tail branch's condition (rem != 0 || len % 8 != 0, hash.rs:64) cannot
+    # change a result: an all-zero tail mixes to 0 (0 * c1 = 0, xs33(0) = 0),
+    # so skipping the branch for it gives the same hash.
+    assert _tail_mix(0) == 0
+    for k in _murmur_keys():
+        assert _murmur_via_fmix64(k, tail_when_zero=False) == hash_bytes(k, 1)
 
 
 def test_hash_c_vs_python_restatement():
