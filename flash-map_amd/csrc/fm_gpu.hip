@@ -6,7 +6,6 @@
 // are reused, so steady-state calls allocate nothing.  All work for a map is
 // ordered on one stream (default priority).
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <chrono>
@@ -20,6 +19,7 @@
 #include "fm_copy.h"
 #include "fm_internal.h"
 #include "fm_kernels.h"
+#include "fm_sort.h"
 
 namespace fm {
 namespace {
@@ -630,10 +630,7 @@ fm_status GpuBackend::ensure_scratch(uint64_t n) {
       (e = hipMalloc(&idx_s_, want * 4)) != hipSuccess || (e = hipMalloc(&ev_, want * 8)) != hipSuccess ||
       (e = hipMalloc(&action_, want)) != hipSuccess)
     return fail(FM_ERR_GPU_ALLOC, hip_msg("hipMalloc (scratch)", e));
-  size_t bytes = 0;
-  e = hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, cid_, cid_s_, idx_, idx_s_, static_cast<int>(want), 0,
-                                         sort_end_bit_, stream_);
-  if (e != hipSuccess) return fail(FM_ERR_KERNEL_LAUNCH, hip_msg("radix sort sizing", e));
+  const uint64_t bytes = sort_scratch_bytes(want);
   if ((e = hipMalloc(&sort_tmp_, bytes + 256)) != hipSuccess)
     return fail(FM_ERR_GPU_ALLOC, hip_msg("hipMalloc (sort scratch)", e));
   sort_tmp_bytes_ = bytes + 256;
@@ -920,9 +917,7 @@ fm_status GpuBackend::run_insert(const uint8_t* d_keys, const uint8_t* d_vals, u
   }
   {
     Prof p(this, kKidInsSort);
-    size_t tmp = sort_tmp_bytes_;
-    hipError_t e = hipcub::DeviceRadixSort::SortPairs(sort_tmp_, tmp, cid_, cid_s_, idx_, idx_s_, static_cast<int>(n),
-                                                      0, sort_end_bit_, stream_);
+    hipError_t e = sort_pairs(cid_, cid_s_, idx_, idx_s_, n, sort_end_bit_, sort_tmp_, nullptr, 0, stream_);
     if (e != hipSuccess) return fail(FM_ERR_KERNEL_LAUNCH, hip_msg("radix sort", e));
   }
   {
@@ -983,9 +978,10 @@ fm_status GpuBackend::run_remove(const uint8_t* d_keys, uint64_t n, uint64_t* re
     }
     {
       Prof p(this, kKidRemSort);
-      size_t tmp = sort_tmp_bytes_;
-      hipError_t e = hipcub::DeviceRadixSort::SortPairs(sort_tmp_, tmp, cid_, cid_s_, idx_, idx_s_,
-                                                        static_cast<int>(n), 0, sort_end_bit_, stream_);
+      // gated like the runs and the replay: every sort kernel exits at once
+      // unless k_rem_check flagged a conflict
+      hipError_t e = sort_pairs(cid_, cid_s_, idx_, idx_s_, n, sort_end_bit_, sort_tmp_, &ctr_->status,
+                                dev::kStatusConflict, stream_);
       if (e != hipSuccess) return fail(FM_ERR_KERNEL_LAUNCH, hip_msg("radix sort", e));
     }
     Prof p(this, kKidRemReplay);
@@ -1004,9 +1000,7 @@ fm_status GpuBackend::run_remove(const uint8_t* d_keys, uint64_t n, uint64_t* re
     }
     {
       Prof p(this, kKidRemSort);
-      size_t tmp = sort_tmp_bytes_;
-      hipError_t e = hipcub::DeviceRadixSort::SortPairs(sort_tmp_, tmp, cid_, cid_s_, idx_, idx_s_,
-                                                        static_cast<int>(n), 0, sort_end_bit_, stream_);
+      hipError_t e = sort_pairs(cid_, cid_s_, idx_, idx_s_, n, sort_end_bit_, sort_tmp_, nullptr, 0, stream_);
       if (e != hipSuccess) return fail(FM_ERR_KERNEL_LAUNCH, hip_msg("radix sort", e));
     }
     {
