@@ -265,6 +265,12 @@ fm_status fm_bulk_insert_device(fm_map* m, const void* d_keys, const void* d_val
   return m->be->bulk_insert_device(d_keys, d_values, count, out_new, true);
 }
 
+fm_status fm_bulk_insert_device_noalloc(fm_map* m, const void* d_keys, const void* d_values, uint64_t count,
+                                        uint64_t* out_new) {
+  FM_LOCK(m);
+  return m->be->bulk_insert_device_noalloc(d_keys, d_values, count, out_new);
+}
+
 fm_status fm_bulk_insert_device_uncounted(fm_map* m, const void* d_keys, const void* d_values, uint64_t count) {
   FM_LOCK(m);
   return m->be->bulk_insert_device(d_keys, d_values, count, nullptr, false);

@@ -11,6 +11,7 @@
 #include <chrono>
 #include <cstdio>
 #include <memory>
+#include <mutex>
 #include <cstdlib>
 #include <cstring>
 #include <string>
@@ -29,6 +30,47 @@ using dev::Table;
 
 constexpr uint64_t kMaxGpuCapacity = 1ull << 31;  // u32 slot ids in scratch
 constexpr uint64_t kQKeepMax = 1u << 15;          // removes up to this size keep Q_ == freebits
+
+// ---------------------------------------------------------------------------
+// Scratch of the insert that allocates nothing (fm_bulk_insert_device_noalloc):
+// one copy of `scratch` per GPU (32 MiB, held until the process ends),
+// shared by every such call on that GPU.  In host code the array's name is
+// only a host-side stand-in; each device's own address comes from
+// hipGetSymbolAddress, taken once per device at map creation with that
+// device current, and every kernel argument uses that address.  Nothing in
+// it needs to start at zero: every kernel writes an element before any
+// kernel of the same call reads it.  Layout for up to kNoAllocMaxKeys keys
+// (the led path and the sort-grouped path of one call use one or the other):
+//   led:  dslot [0, 4 MiB), big [4, 8 MiB)
+//   sort: home [0, 4), cid [4, 8), idx [8, 12), cid_s [12, 16),
+//         idx_s [16, 20), sort scratch [20, 32 MiB)
+// ---------------------------------------------------------------------------
+constexpr uint64_t kNoAllocMaxKeys = 1ull << 20;
+constexpr uint64_t kNoAllocArray = 4 * kNoAllocMaxKeys;
+constexpr uint64_t kNoAllocBytes = 32ull << 20;
+static_assert(5 * kNoAllocArray + sort_scratch_bytes(kNoAllocMaxKeys) <= kNoAllocBytes, "sort-path layout");
+static __device__ __align__(256) uint8_t scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint8_t* g_scratch_ptr[kMaxDevices] = {};
+
+// Records this device's address of `scratch` (the device is current).
+hipError_t scratch_setup(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> l(g_scratch_mu);
+  if (g_scratch_ptr[dev]) return hipSuccess;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e == hipSuccess) g_scratch_ptr[dev] = static_cast<uint8_t*>(p);
+  return e;
+}
+
+uint8_t* scratch_ptr(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> l(g_scratch_mu);
+  return g_scratch_ptr[dev];
+}
 
 // ---------------------------------------------------------------------------
 // Per-(K,V) launchers.  Specialised sizes get register-resident keys and
@@ -252,6 +294,8 @@ class GpuBackend final : public Backend {
   fm_status bulk_get_device(const void* d_keys, uint64_t n, void* d_out, uint8_t* d_found) override;
   fm_status bulk_insert_device(const void* d_keys, const void* d_vals, uint64_t n, uint64_t* out_new,
                                bool counted) override;
+  fm_status bulk_insert_device_noalloc(const void* d_keys, const void* d_vals, uint64_t n,
+                                       uint64_t* out_new) override;
   fm_status bulk_remove_device(const void* d_keys, uint64_t n, uint64_t* out_removed) override;
   fm_status recount(uint64_t* out) override;
   fm_status synchronize() override;
@@ -346,9 +390,22 @@ class GpuBackend final : public Backend {
   // leaves count in ctr.  defer_big (counted calls, which then read the
   // counters with read_insert_counters): the led path launches no k_ins_big;
   // its settle pass reports queued closures and the host runs them then.
+  // Per-call buffers of an insert: the map's growable scratch, or the
+  // per-GPU `scratch` array for the insert that allocates nothing.
+  struct InsertScratch {
+    uint32_t* home;
+    uint32_t* dslot;
+    uint32_t* big;
+    uint32_t* cid;
+    uint32_t* idx;
+    uint32_t* cid_s;
+    uint32_t* idx_s;
+    void* sort_tmp;
+  };
+  // fixed: use these buffers (no allocation, no deduplication, no fused launch).
   fm_status run_insert(const uint8_t* d_keys, const uint8_t* d_vals, uint64_t n, uint8_t* outcome = nullptr,
-                       bool defer_big = false);
-  fm_status launch_bits(const Table& t, uint64_t n, bool dedup, bool defer_big);
+                       bool defer_big = false, const InsertScratch* fixed = nullptr);
+  fm_status launch_bits(const Table& t, uint64_t n, bool dedup, bool defer_big, const uint32_t* dslot);
   fm_status read_insert_counters(Counters* out);
   // async: no status readback and no count (serving reads the outcome
   // flags); the conflict replay is gated on the device.
@@ -389,6 +446,8 @@ class GpuBackend final : public Backend {
     const uint8_t* vals;
     uint64_t n;
     bool dedup;
+    const uint32_t* dslot;
+    uint32_t* big;
   };
   PendingBig pend_{};
 
@@ -424,10 +483,10 @@ class GpuBackend final : public Backend {
     GpuBackend* g;
     int kid;
     Prof(GpuBackend* g_, int k) : g(g_), kid(k) {
-      if (g->prof_) g->prof_mark(kid, true);
+      if (g->prof_ && !g->noalloc_) g->prof_mark(kid, true);
     }
     ~Prof() {
-      if (g->prof_) g->prof_mark(kid, false);
+      if (g->prof_ && !g->noalloc_) g->prof_mark(kid, false);
     }
   };
   static const char* kid_name(int k);
@@ -435,6 +494,8 @@ class GpuBackend final : public Backend {
   void prof_mark(int kid, bool begin);
   void harvest();
   bool prof_ = false;
+  bool noalloc_ = false;  // inside the insert that allocates nothing: no timing events either
+  hipError_t scratch_err_ = hipSuccess;  // hipGetSymbolAddress of `scratch` at map creation
   hipEvent_t prof_open_ = nullptr;
   std::vector<PendingTiming> pending_;
   std::vector<hipEvent_t> ev_pool_;
@@ -569,6 +630,7 @@ fm_status GpuBackend::init(const fm_builder& b, std::string* err) {
       (e = hipMalloc(&t_.freebits, words * 8)) != hipSuccess ||
       (e = hipMalloc(&t_.emptybits, words * 8)) != hipSuccess ||
       (e = hipMalloc(&t_.owner, cap * sizeof(uint32_t))) != hipSuccess ||
+      (e = hipMalloc(&Q_, words * 8)) != hipSuccess ||
       (e = hipMalloc(&ctr_, sizeof(Counters))) != hipSuccess ||
       (e = hipMalloc(&bar_, sizeof(dev::GridBarrier))) != hipSuccess ||
       (e = hipMemset(bar_, 0, sizeof(dev::GridBarrier))) != hipSuccess ||
@@ -577,6 +639,7 @@ fm_status GpuBackend::init(const fm_builder& b, std::string* err) {
     return FM_ERR_GPU_ALLOC;
   }
   std::memset(h_ctr_, 0, sizeof(Counters));  // done == 0 matches no sequence number
+  scratch_err_ = scratch_setup(dev_);  // checked by the insert that allocates nothing
   ops_ = select_ops(t_.K, t_.V);
   if (ops_.ins_fused_blocks(dev_, &fused_blocks_) != hipSuccess) fused_blocks_ = 0;  // multi-kernel path instead
   if (const char* env = std::getenv("FM_FUSED_SPIN")) fused_spin_ = static_cast<uint32_t>(std::strtoul(env, nullptr, 10));
@@ -634,10 +697,6 @@ fm_status GpuBackend::ensure_scratch(uint64_t n) {
   if ((e = hipMalloc(&sort_tmp_, bytes + 256)) != hipSuccess)
     return fail(FM_ERR_GPU_ALLOC, hip_msg("hipMalloc (sort scratch)", e));
   sort_tmp_bytes_ = bytes + 256;
-  if (!Q_) {
-    const uint64_t words = (t_.cap + 63) / 64;
-    if ((e = hipMalloc(&Q_, words * 8)) != hipSuccess) return fail(FM_ERR_GPU_ALLOC, hip_msg("hipMalloc (Q)", e));
-  }
   scratch_n_ = want;
   return FM_OK;
 }
@@ -692,7 +751,7 @@ fm_status GpuBackend::zero_counters() {
 
 // The settle pass of a led insert: publishes the real claims, restores Q_
 // and hands the counters to h_ctr_ (see k_ins_bits).
-fm_status GpuBackend::launch_bits(const Table& t, uint64_t n, bool dedup, bool defer_big) {
+fm_status GpuBackend::launch_bits(const Table& t, uint64_t n, bool dedup, bool defer_big, const uint32_t* dslot) {
   Prof p(this, kKidInsBits);
   const uint32_t seq = next_ctr_seq();
   if (!dedup && n * 16 >= t_.cap) {
@@ -703,7 +762,7 @@ fm_status GpuBackend::launch_bits(const Table& t, uint64_t n, bool dedup, bool d
     hipLaunchKernelGGL(dev::k_ins_bits_rebuild, dim3(g), dim3(1024), 0, stream_, t, ctr_, Q_, h_ctr_, seq, defer_big);
   } else {
     // 1024-thread workgroups: one completion ticket per 1024 keys
-    hipLaunchKernelGGL(dev::k_ins_bits, dim3(grid_for(n, 1024)), dim3(1024), 0, stream_, t, dslot_, n, ctr_, Q_,
+    hipLaunchKernelGGL(dev::k_ins_bits, dim3(grid_for(n, 1024)), dim3(1024), 0, stream_, t, dslot, n, ctr_, Q_,
                        h_ctr_, seq, defer_big);
   }
   fm_status st;
@@ -725,11 +784,11 @@ fm_status GpuBackend::read_insert_counters(Counters* out) {
   if (!(out->status & dev::kStatusBigPending)) return FM_OK;
   {
     Prof p(this, kKidInsBig);
-    if ((st = check_launch(ops_.ins_big(pend_.t, pend_.keys, pend_.vals, pend_.n, Q_, big_, ctr_, stream_),
+    if ((st = check_launch(ops_.ins_big(pend_.t, pend_.keys, pend_.vals, pend_.n, Q_, pend_.big, ctr_, stream_),
                            "k_ins_big")) != FM_OK)
       return st;
   }
-  if ((st = launch_bits(pend_.t, pend_.n, pend_.dedup, false)) != FM_OK) return st;
+  if ((st = launch_bits(pend_.t, pend_.n, pend_.dedup, false, pend_.dslot)) != FM_OK) return st;
   return read_counters(out);
 }
 
@@ -817,14 +876,21 @@ fm_status GpuBackend::ensure_dedup(uint64_t n) {
 }
 
 fm_status GpuBackend::run_insert(const uint8_t* d_keys, const uint8_t* d_vals, uint64_t n, uint8_t* outcome,
-                                 bool defer_big) {
+                                 bool defer_big, const InsertScratch* fixed) {
   fm_status st;
   Table t = t_;
   t.outcome = outcome;
-  if ((st = ensure_scratch(n)) != FM_OK) return st;
+  InsertScratch sc;
+  if (fixed) {
+    sc = *fixed;
+  } else {
+    if ((st = ensure_scratch(n)) != FM_OK) return st;
+    sc = InsertScratch{home_, dslot_, big_, cid_, idx_, cid_s_, idx_s_, sort_tmp_};
+  }
   const uint64_t words = (t_.cap + 63) / 64;
   const uint32_t blk = tune_.insert_block;
-  const bool dedup = (tune_.insert_dedup || serving_) && n > 1;
+  // (deduplication changes no result; its buckets are allocated per size)
+  const bool dedup = !fixed && (tune_.insert_dedup || serving_) && n > 1;
   if (dedup) {
     if ((st = ensure_dedup(n)) != FM_OK) return st;
     t.dd_first = dd_first_;
@@ -837,7 +903,7 @@ fm_status GpuBackend::run_insert(const uint8_t* d_keys, const uint8_t* d_vals, u
   // led without sorting.  Otherwise (the batch may fill the table) group by
   // closure id with a stable radix sort, which also covers "no barrier".
   const bool led = len_ + n < t_.cap && !tune_sort_only_;
-  if (led && tune_.insert_fused && fused_blocks_ && !dedup) {
+  if (led && tune_.insert_fused && fused_blocks_ && !dedup && !fixed) {
     last_insert_ = InsertPath::kFused;
     if (!q_synced_) {
       Prof p(this, kKidInsCopy);
@@ -880,13 +946,13 @@ fm_status GpuBackend::run_insert(const uint8_t* d_keys, const uint8_t* d_vals, u
   if (led) {
     {
       Prof p(this, kKidInsDry);
-      if ((st = check_launch(ops_.ins_dry(t, d_keys, n, Q_, nullptr, dslot_, ctr_, blk, stream_), "k_ins_dry")) !=
+      if ((st = check_launch(ops_.ins_dry(t, d_keys, n, Q_, nullptr, sc.dslot, ctr_, blk, stream_), "k_ins_dry")) !=
           FM_OK)
         return st;
     }
     {
       Prof p(this, kKidInsLeader);
-      if ((st = check_launch(ops_.ins_leader(t, d_keys, d_vals, n, Q_, dslot_, big_, ctr_, tune_.insert_coop_arc,
+      if ((st = check_launch(ops_.ins_leader(t, d_keys, d_vals, n, Q_, sc.dslot, sc.big, ctr_, tune_.insert_coop_arc,
                                              stream_),
                              "k_ins_leader")) != FM_OK)
         return st;
@@ -896,33 +962,33 @@ fm_status GpuBackend::run_insert(const uint8_t* d_keys, const uint8_t* d_vals, u
     // closures, and deferring would only add a round trip.
     const bool defer = defer_big && (dedup || n * 16 < t_.cap);
     if (defer) {
-      pend_ = PendingBig{t, d_keys, d_vals, n, dedup};
+      pend_ = PendingBig{t, d_keys, d_vals, n, dedup, sc.dslot, sc.big};
     } else {
       Prof p(this, kKidInsBig);
-      if ((st = check_launch(ops_.ins_big(t, d_keys, d_vals, n, Q_, big_, ctr_, stream_), "k_ins_big")) != FM_OK)
+      if ((st = check_launch(ops_.ins_big(t, d_keys, d_vals, n, Q_, sc.big, ctr_, stream_), "k_ins_big")) != FM_OK)
         return st;
     }
-    return launch_bits(t, n, dedup, defer);
+    return launch_bits(t, n, dedup, defer, sc.dslot);
   }
   {
     Prof p(this, kKidInsDry);
-    if ((st = check_launch(ops_.ins_dry(t, d_keys, n, Q_, home_, nullptr, ctr_, blk, stream_), "k_ins_dry")) != FM_OK)
+    if ((st = check_launch(ops_.ins_dry(t, d_keys, n, Q_, sc.home, nullptr, ctr_, blk, stream_), "k_ins_dry")) != FM_OK)
       return st;
   }
   {
     Prof p(this, kKidInsClosure);
-    hipLaunchKernelGGL(dev::k_ins_closure, dim3(grid_for(n, blk)), dim3(blk), 0, stream_, t_.cap, Q_, home_, n, cid_,
-                       idx_, ctr_);
+    hipLaunchKernelGGL(dev::k_ins_closure, dim3(grid_for(n, blk)), dim3(blk), 0, stream_, t_.cap, Q_, sc.home, n,
+                       sc.cid, sc.idx, ctr_);
     if ((st = check_launch(hipGetLastError(), "k_ins_closure")) != FM_OK) return st;
   }
   {
     Prof p(this, kKidInsSort);
-    hipError_t e = sort_pairs(cid_, cid_s_, idx_, idx_s_, n, sort_end_bit_, sort_tmp_, nullptr, 0, stream_);
+    hipError_t e = sort_pairs(sc.cid, sc.cid_s, sc.idx, sc.idx_s, n, sort_end_bit_, sc.sort_tmp, nullptr, 0, stream_);
     if (e != hipSuccess) return fail(FM_ERR_KERNEL_LAUNCH, hip_msg("radix sort", e));
   }
   {
     Prof p(this, kKidInsReplay);
-    if ((st = check_launch(ops_.ins_replay(t, d_keys, d_vals, n, cid_s_, idx_s_, ctr_, blk, stream_),
+    if ((st = check_launch(ops_.ins_replay(t, d_keys, d_vals, n, sc.cid_s, sc.idx_s, ctr_, blk, stream_),
                            "k_ins_replay")) != FM_OK)
       return st;
   }
@@ -1305,6 +1371,47 @@ fm_status GpuBackend::bulk_insert_device(const void* d_keys, const void* d_vals,
   }
   Counters c;
   if ((st = read_insert_counters(&c)) != FM_OK) return st;
+  if ((st = check_insert_status(c)) != FM_OK) return st;
+  q_synced_ = q_pending_;  // a clean led insert left Q_ equal to freebits
+  len_ += c.count;
+  if (out_new) *out_new = c.count;
+  return FM_OK;
+}
+
+// fm_bulk_insert_device with every per-call buffer in this GPU's `scratch`
+// array: the same kernels and results, no allocation, no deduplication and
+// no fused launch (neither changes a result), no timing events.
+fm_status GpuBackend::bulk_insert_device_noalloc(const void* d_keys, const void* d_vals, uint64_t n,
+                                                 uint64_t* out_new) {
+  if (n == 0) {
+    if (out_new) *out_new = 0;
+    return FM_OK;
+  }
+  if (n > kNoAllocMaxKeys)
+    return fail(FM_ERR_INVALID_ARGUMENT, "fm_bulk_insert_device_noalloc: " + std::to_string(n) +
+                                             " keys; at most 1048576 per call without allocation");
+  if (len_ + n > t_.cap) {  // gpu.rs:551-559
+    set_table_full(len_, t_.cap);
+    return FM_ERR_TABLE_FULL;
+  }
+  fm_status st;
+  if ((st = use_device()) != FM_OK) return st;
+  uint8_t* base = scratch_ptr(dev_);
+  if (scratch_err_ != hipSuccess || !base)
+    return fail(FM_ERR_DEVICE_INIT, hip_msg("hipGetSymbolAddress (insert scratch)", scratch_err_));
+  if ((st = check_device_ptr(d_keys, vec_align(t_.K), "keys")) != FM_OK ||
+      (st = check_device_ptr(d_vals, vec_align(t_.V), "values")) != FM_OK)
+    return st;
+  auto arr = [&](uint64_t i) { return reinterpret_cast<uint32_t*>(base + i * kNoAllocArray); };
+  const bool led = len_ + n < t_.cap && !tune_sort_only_;
+  const InsertScratch sc = led ? InsertScratch{nullptr, arr(0), arr(1), nullptr, nullptr, nullptr, nullptr, nullptr}
+                               : InsertScratch{arr(0), nullptr, nullptr, arr(1), arr(2), arr(3), arr(4), arr(5)};
+  noalloc_ = true;
+  st = run_insert(static_cast<const uint8_t*>(d_keys), static_cast<const uint8_t*>(d_vals), n, nullptr, true, &sc);
+  Counters c{};
+  if (st == FM_OK) st = read_insert_counters(&c);
+  noalloc_ = false;
+  if (st != FM_OK) return st;
   if ((st = check_insert_status(c)) != FM_OK) return st;
   q_synced_ = q_pending_;  // a clean led insert left Q_ equal to freebits
   len_ += c.count;

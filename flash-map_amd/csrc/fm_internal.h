@@ -46,7 +46,10 @@ class Backend {
   virtual fm_status bulk_get_device(const void*, uint64_t, void*, uint8_t*) { return gpu_required(); }
   virtual fm_status bulk_insert_device(const void*, const void*, uint64_t, uint64_t*, bool /*counted*/) {
     return gpu_required();
+  }  virtual fm_status bulk_insert_device_noalloc(const void*, const void*, uint64_t, uint64_t*) {
+    return gpu_required();
   }
+
   virtual fm_status bulk_remove_device(const void*, uint64_t, uint64_t*) { return gpu_required(); }
   virtual fm_status recount(uint64_t*) { return gpu_required(); }
   virtual fm_status synchronize() { return FM_OK; }

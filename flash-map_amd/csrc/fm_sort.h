@@ -226,14 +226,15 @@ k_sort_scatter(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vi
 
 }  // namespace sort
 
+constexpr uint64_t sort_round256(uint64_t b) { return (b + 255) & ~uint64_t(255); }
+
 // Scratch bytes sort_pairs needs for n pairs (0 for the one-workgroup path).
-inline uint64_t sort_scratch_bytes(uint64_t n) {
+constexpr uint64_t sort_scratch_bytes(uint64_t n) {
   if (n <= sort::kSortSmallMax) return 0;
   const uint64_t nblk = (n + sort::kSortTile - 1) / sort::kSortTile;
   const uint64_t len = nblk * sort::kSortBins;
   const uint64_t chunks = (len + sort::kScanChunk - 1) / sort::kScanChunk;
-  auto up = [](uint64_t b) { return (b + 255) & ~uint64_t(255); };
-  return up(4 * n) + up(4 * n) + up(4 * len) + up(4 * chunks);
+  return 2 * sort_round256(4 * n) + sort_round256(4 * len) + sort_round256(4 * chunks);
 }
 
 // Stable sort of n (key, value) pairs by key bits [0, end_bit) into
@@ -253,12 +254,11 @@ inline hipError_t sort_pairs(const uint32_t* kin, uint32_t* kout, const uint32_t
   const uint64_t nblk = (n + kSortTile - 1) / kSortTile;
   const uint64_t len = nblk * kSortBins;
   const uint64_t chunks = (len + kScanChunk - 1) / kScanChunk;
-  auto up = [](uint64_t b) { return (b + 255) & ~uint64_t(255); };
   uint8_t* p = static_cast<uint8_t*>(tmp);
   uint32_t* tk = reinterpret_cast<uint32_t*>(p);
-  uint32_t* tv = reinterpret_cast<uint32_t*>(p + up(4 * n));
-  uint32_t* H = reinterpret_cast<uint32_t*>(p + 2 * up(4 * n));
-  uint32_t* S = reinterpret_cast<uint32_t*>(p + 2 * up(4 * n) + up(4 * len));
+  uint32_t* tv = reinterpret_cast<uint32_t*>(p + sort_round256(4 * n));
+  uint32_t* H = reinterpret_cast<uint32_t*>(p + 2 * sort_round256(4 * n));
+  uint32_t* S = reinterpret_cast<uint32_t*>(p + 2 * sort_round256(4 * n) + sort_round256(4 * len));
   const int passes = end_bit <= 0 ? 1 : (end_bit + kSortBits - 1) / kSortBits;
   // pass p writes the output arrays when (passes - 1 - p) is even
   const uint32_t* ck = kin;

@@ -24,6 +24,7 @@ __all__ = [
 ]
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libflashmap_amd.so")
+NOALLOC_MAX_KEYS = 1 << 20  # FM_NOALLOC_MAX_KEYS in include/flashmap.h
 
 # fm_status (include/flashmap.h) == FlashMapError variants (src/error.rs:4-39)
 OK, DEVICE_INIT, GPU_ALLOC, KERNEL_LAUNCH, TRANSFER, TABLE_FULL, ZERO_CAPACITY, \
@@ -166,6 +167,7 @@ def lib() -> ctypes.CDLL:
             "fm_bulk_get_values_device": (ctypes.c_int, [vp, vp, u64, vp]),
             "fm_bulk_insert_device": (ctypes.c_int, [vp, vp, vp, u64, pu64]),
             "fm_bulk_insert_device_uncounted": (ctypes.c_int, [vp, vp, vp, u64]),
+            "fm_bulk_insert_device_noalloc": (ctypes.c_int, [vp, vp, vp, u64, pu64]),
             "fm_bulk_remove_device": (ctypes.c_int, [vp, vp, u64, pu64]),
             "fm_recount": (ctypes.c_int, [vp, pu64]),
             "fm_synchronize": (ctypes.c_int, [vp]),
@@ -402,6 +404,14 @@ class FlashMap:
         out = ctypes.c_uint64(0)
         _check(lib().fm_bulk_insert_device(self._h, ctypes.c_void_p(d_keys.ptr),
                                            ctypes.c_void_p(d_vals.ptr), count, ctypes.byref(out)))
+        return int(out.value)
+
+    def bulk_insert_device_noalloc(self, d_keys: DeviceBuffer, d_vals: DeviceBuffer, count: int) -> int:
+        """bulk_insert_device with no allocation: per-call buffers in the
+        GPU's fixed scratch array, at most NOALLOC_MAX_KEYS keys per call."""
+        out = ctypes.c_uint64(0)
+        _check(lib().fm_bulk_insert_device_noalloc(self._h, ctypes.c_void_p(d_keys.ptr),
+                                                   ctypes.c_void_p(d_vals.ptr), count, ctypes.byref(out)))
         return int(out.value)
 
     def bulk_insert_device_uncounted(self, d_keys: DeviceBuffer, d_vals: DeviceBuffer, count: int):

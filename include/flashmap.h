@@ -157,6 +157,19 @@ fm_status fm_bulk_get_values_device(fm_map* m, const void* d_keys,
 fm_status fm_bulk_insert_device(fm_map* m, const void* d_keys,
                                 const void* d_values, uint64_t count,
                                 uint64_t* out_new);
+/* No reference counterpart: fm_bulk_insert_device for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Same results as fm_bulk_insert_device; it calls no
+ * hipMalloc, hipFree or allocating routine: its per-call buffers live in one
+ * 32 MiB device array per GPU, held until the process ends, so count is at
+ * most FM_NOALLOC_MAX_KEYS (FM_ERR_INVALID_ARGUMENT beyond).  Every such call
+ * on a GPU shares that GPU's array: calls that overlap on one GPU (on maps
+ * used from different threads) race.  The GPU backend only (GpuRequired on
+ * the CPU arm). */
+#define FM_NOALLOC_MAX_KEYS (1ull << 20)
+fm_status fm_bulk_insert_device_noalloc(fm_map* m, const void* d_keys,
+                                        const void* d_values, uint64_t count,
+                                        uint64_t* out_new);
 /* bulk_insert_device_uncounted (lib.rs:325-338, gpu.rs:618-672): no
  * TableFull check, no readback, len += count. */
 fm_status fm_bulk_insert_device_uncounted(fm_map* m, const void* d_keys,

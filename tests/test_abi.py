@@ -3,6 +3,7 @@
 The CPU backend is the builder's fallback arm (FlashMapBackend::Rayon,
 src/lib.rs:431-441); it is checked against the oracle like the GPU backend.
 """
+import ctypes
 import os
 import re
 import subprocess
@@ -93,6 +94,8 @@ def test_error_paths_cpu():
         m.upload(b"12345678")
     with pytest.raises(fm.GpuRequired):
         m.recount()
+    out = ctypes.c_uint64(0)  # the insert that allocates nothing is GPU-only too
+    assert fm.lib().fm_bulk_insert_device_noalloc(m._h, None, None, 1, ctypes.byref(out)) == fm.GPU_REQUIRED
     with pytest.raises(ValueError):
         m.bulk_insert(np.zeros(3, dtype=np.uint64), np.zeros(2, dtype=np.uint64))
     with pytest.raises(fm.FlashMapError) as e:
