@@ -81,7 +81,7 @@ struct Ops {
   hipError_t (*ins_dry)(const Table&, const uint8_t*, uint64_t, unsigned long long*, uint32_t*, uint32_t*, Counters*,
                         uint32_t, hipStream_t);
   hipError_t (*ins_leader)(const Table&, const uint8_t*, const uint8_t*, uint64_t, const unsigned long long*,
-                           const uint32_t*, uint32_t*, Counters*, uint32_t, hipStream_t);
+                           const uint32_t*, uint32_t*, Counters*, uint32_t, uint32_t, bool, hipStream_t);
   hipError_t (*ins_big)(const Table&, const uint8_t*, const uint8_t*, uint64_t, const unsigned long long*,
                         const uint32_t*, Counters*, hipStream_t);
   hipError_t (*ins_replay)(const Table&, const uint8_t*, const uint8_t*, uint64_t, const uint32_t*, const uint32_t*,
@@ -136,9 +136,9 @@ hipError_t launch_ins_dedup(const Table& t, const uint8_t* k, uint64_t n, uint32
 template <int KS, int VS>
 hipError_t launch_ins_leader(const Table& t, const uint8_t* k, const uint8_t* v, uint64_t n,
                              const unsigned long long* Q, const uint32_t* dslot, uint32_t* big, Counters* c,
-                             uint32_t coop_arc, hipStream_t s) {
+                             uint32_t coop_arc, uint32_t leader_max, bool plan, hipStream_t s) {
   hipLaunchKernelGGL((dev::k_ins_leader<KS, VS>), dim3(grid_for(n, 64)), dim3(64), 0, s, t, k, v, n, Q, dslot, big,
-                     c, coop_arc);
+                     c, coop_arc, leader_max, plan);
   return hipGetLastError();
 }
 
@@ -358,6 +358,8 @@ class GpuBackend final : public Backend {
   fm_status zero_counters();
   fm_status check_device_ptr(const void* p, uint32_t align, const char* what);
   fm_status check_insert_status(const Counters& c) {
+    if (c.status & dev::kStatusBigPending)  // the host owes a k_ins_big and a settle: counts are partial
+      return fail(FM_ERR_KERNEL_LAUNCH, "insert: closures still queued for the big-closure replay");
     if (c.status & dev::kStatusBudget)
       return fail(FM_ERR_KERNEL_LAUNCH, "insert walk found no free slot in a full table (uncounted inserts overfilled it)");
     if (c.status & dev::kStatusStall) {
@@ -429,6 +431,7 @@ class GpuBackend final : public Backend {
   uint64_t dd_n_ = 0;
   fm_status ensure_dedup(uint64_t n);
   bool tune_sort_only_ = false;  // FM_INSERT_SORT_PATH=1 forces the sort-grouped insert
+  uint32_t leader_max_ = dev::kLeaderMax;  // FM_LEADER_MAX: members a leader lane replays (more: k_ins_big)
   enum class InsertPath { kSort, kLed, kFused };
   InsertPath last_insert_ = InsertPath::kSort;
   uint32_t fused_blocks_ = 0;  // k_ins_fused workgroups per launch at most (0: no cooperative launch)
@@ -543,6 +546,7 @@ void GpuBackend::release() {
   dfree(t_.freebits);
   dfree(t_.emptybits);
   dfree(t_.owner);
+  dfree(t_.rowner);
   dfree(home_);
   dfree(dslot_);
   dfree(big_);
@@ -630,6 +634,7 @@ fm_status GpuBackend::init(const fm_builder& b, std::string* err) {
       (e = hipMalloc(&t_.freebits, words * 8)) != hipSuccess ||
       (e = hipMalloc(&t_.emptybits, words * 8)) != hipSuccess ||
       (e = hipMalloc(&t_.owner, cap * sizeof(uint32_t))) != hipSuccess ||
+      (e = hipMalloc(&t_.rowner, cap * sizeof(uint32_t))) != hipSuccess ||
       (e = hipMalloc(&Q_, words * 8)) != hipSuccess ||
       (e = hipMalloc(&ctr_, sizeof(Counters))) != hipSuccess ||
       (e = hipMalloc(&bar_, sizeof(dev::GridBarrier))) != hipSuccess ||
@@ -644,10 +649,13 @@ fm_status GpuBackend::init(const fm_builder& b, std::string* err) {
   if (ops_.ins_fused_blocks(dev_, &fused_blocks_) != hipSuccess) fused_blocks_ = 0;  // multi-kernel path instead
   if (const char* env = std::getenv("FM_FUSED_SPIN")) fused_spin_ = static_cast<uint32_t>(std::strtoul(env, nullptr, 10));
   if (const char* env = std::getenv("FM_INSERT_SORT_PATH")) tune_sort_only_ = env[0] == '1';
+  if (const char* env = std::getenv("FM_LEADER_MAX"))
+    leader_max_ = static_cast<uint32_t>(std::max(1ul, std::min<unsigned long>(std::strtoul(env, nullptr, 10), dev::kLeaderMax)));
   sort_end_bit_ = 1;
   while ((1ull << sort_end_bit_) <= cap) ++sort_end_bit_;  // ids in [0, cap]
   hipLaunchKernelGGL(dev::k_clear, dim3(grid_for(cap, 256)), dim3(256), 0, stream_, t_);
   hipLaunchKernelGGL(dev::k_fill_u32, dim3(grid_for(cap, 256)), dim3(256), 0, stream_, t_.owner, cap, dev::kNoOwner);
+  hipLaunchKernelGGL(dev::k_fill_u32, dim3(grid_for(cap, 256)), dim3(256), 0, stream_, t_.rowner, cap, dev::kNoOwner);
   if ((e = hipGetLastError()) != hipSuccess || (e = hipStreamSynchronize(stream_)) != hipSuccess) {
     *err = hip_msg("table init", e);
     return FM_ERR_DEVICE_INIT;
@@ -759,7 +767,8 @@ fm_status GpuBackend::launch_bits(const Table& t, uint64_t n, bool dedup, bool d
     // coalesced pass) instead of one atomic per key; at most 1024
     // workgroups, so at most 1024 completion tickets.
     const uint32_t g = std::min<uint32_t>(grid_for(t_.cap, 1024), 1024);
-    hipLaunchKernelGGL(dev::k_ins_bits_rebuild, dim3(g), dim3(1024), 0, stream_, t, ctr_, Q_, h_ctr_, seq, defer_big);
+    // (never deferred: run_insert defers only batches settled per key)
+    hipLaunchKernelGGL(dev::k_ins_bits_rebuild, dim3(g), dim3(1024), 0, stream_, t, ctr_, Q_, h_ctr_, seq);
   } else {
     // 1024-thread workgroups: one completion ticket per 1024 keys
     hipLaunchKernelGGL(dev::k_ins_bits, dim3(grid_for(n, 1024)), dim3(1024), 0, stream_, t, dslot, n, ctr_, Q_,
@@ -769,8 +778,12 @@ fm_status GpuBackend::launch_bits(const Table& t, uint64_t n, bool dedup, bool d
   if ((st = check_launch(hipGetLastError(), "k_ins_bits")) != FM_OK) return st;
   poll_seq_ = seq;
   q_pending_ = true;     // Q_ == freebits once the status shows no failed dry pass
-  ctr_zero_ = true;      // k_ins_bits zeroed the counters (a deferred pass: once the settle after k_ins_big ran) ...
-  ctr_in_host_ = true;   // ... after writing them to h_ctr_
+  // A settle that hands over zeroes the device counters; a deferred one
+  // that reports pending closures keeps them (count, queue) for the settle
+  // after k_ins_big, so they count as zero only once read_insert_counters
+  // has seen a final hand-over.
+  ctr_zero_ = !defer_big;
+  ctr_in_host_ = true;   // the counters (or the pending report) go to h_ctr_
   return FM_OK;
 }
 
@@ -780,16 +793,26 @@ fm_status GpuBackend::launch_bits(const Table& t, uint64_t n, bool dedup, bool d
 // reads the final counters.
 fm_status GpuBackend::read_insert_counters(Counters* out) {
   fm_status st;
-  if ((st = read_counters(out)) != FM_OK) return st;
-  if (!(out->status & dev::kStatusBigPending)) return FM_OK;
+  auto failed = [&](fm_status e) {  // the device counters' state is unknown: clear them next time
+    ctr_zero_ = q_synced_ = false;
+    return e;
+  };
+  const bool handed = ctr_in_host_;  // the settle pass hands the counters over (the sort path does not)
+  if ((st = read_counters(out)) != FM_OK) return failed(st);
+  if (!(out->status & dev::kStatusBigPending)) {
+    if (handed) ctr_zero_ = true;  // the hand-over zeroed them
+    return FM_OK;
+  }
   {
     Prof p(this, kKidInsBig);
     if ((st = check_launch(ops_.ins_big(pend_.t, pend_.keys, pend_.vals, pend_.n, Q_, pend_.big, ctr_, stream_),
                            "k_ins_big")) != FM_OK)
-      return st;
+      return failed(st);
   }
-  if ((st = launch_bits(pend_.t, pend_.n, pend_.dedup, false, pend_.dslot)) != FM_OK) return st;
-  return read_counters(out);
+  if ((st = launch_bits(pend_.t, pend_.n, pend_.dedup, false, pend_.dslot)) != FM_OK) return failed(st);
+  if ((st = read_counters(out)) != FM_OK) return failed(st);
+  ctr_zero_ = !(out->status & dev::kStatusBigPending);
+  return FM_OK;
 }
 
 // Waits for h_ctr_->done == seq by polling host memory, for at most
@@ -797,7 +820,10 @@ fm_status GpuBackend::read_insert_counters(Counters* out) {
 // arrive.  A sleeping stream wait wakes the host well after the kernel's
 // last store; a poll sees the hand-over as it lands.
 bool GpuBackend::host_seq_arrived(uint32_t seq) const {
-  constexpr auto kPollLimit = std::chrono::milliseconds(20);
+  // Up to 2 ms of one host core per counted op (an op longer than that is
+  // waited for by the runtime's sleeping wait instead, whose late wake-up is
+  // small next to it).
+  constexpr auto kPollLimit = std::chrono::milliseconds(2);
   const auto t0 = std::chrono::steady_clock::now();
   for (uint32_t i = 0;; ++i) {
     if (__atomic_load_n(&h_ctr_->done, __ATOMIC_ACQUIRE) == seq) return true;
@@ -953,7 +979,7 @@ fm_status GpuBackend::run_insert(const uint8_t* d_keys, const uint8_t* d_vals, u
     {
       Prof p(this, kKidInsLeader);
       if ((st = check_launch(ops_.ins_leader(t, d_keys, d_vals, n, Q_, sc.dslot, sc.big, ctr_, tune_.insert_coop_arc,
-                                             stream_),
+                                             leader_max_, n * 16 < t_.cap, stream_),
                              "k_ins_leader")) != FM_OK)
         return st;
     }

@@ -7,7 +7,10 @@
 //   vals[cap*V]     bytes
 //   freebits[cap/64]  u64  bit set <=> slot is EMPTY or TOMBSTONE
 //   emptybits[cap/64] u64  bit set <=> slot is EMPTY
-//   owner[cap]      u32   per-slot scratch for remove conflict detection,
+//   owner[cap]      u32   per-slot scratch of inserts: the batch index of
+//                         the key that dry-claimed the slot (read only at
+//                         slots dry-claimed by the same batch, never reset)
+//   rowner[cap]     u32   per-slot scratch of removes (conflict detection),
 //                         0xFFFFFFFF between ops
 //
 // Exactness contract: every op produces exactly the logical slot array that
@@ -68,6 +71,7 @@ struct Table {
   unsigned long long* freebits;
   unsigned long long* emptybits;
   uint32_t* owner;
+  uint32_t* rowner;  // single-map removes (table groups keep theirs in owner)
   uint64_t cap;
   uint64_t mask;
   uint32_t K;
@@ -231,6 +235,16 @@ __device__ __forceinline__ void st_agent(uint32_t* p, uint32_t v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Keeps two loads issued together: an empty asm statement that reads the
+// loaded registers, so the compiler cannot move the second load into the one
+// branch that uses it (which would add a dependent round trip there).
+template <int N>
+__device__ __forceinline__ void keep_loaded(uint32_t f, const Blob<N>& b) {
+  asm volatile("" ::"v"(f));
+#pragma unroll
+  for (int i = 0; i < Blob<N>::W; ++i) asm volatile("" ::"v"(b.w[i]));
+}
+
 
 // First set bit at or after `start`, cyclically; cap if there is none.
 __device__ inline uint64_t bm_next_set(const unsigned long long* bm, uint64_t start, uint64_t cap) {
@@ -268,8 +282,19 @@ __device__ inline bool bm_claim(unsigned long long* bm, uint64_t start, uint64_t
   const uint64_t words = (cap + 63) >> 6;
   uint64_t w = start >> 6;
   unsigned long long m = ~0ull << (start & 63);
+  // The start bit first, with no read before it: the atomic's old word shows
+  // the word's other free bits when the start slot is already taken.
+  unsigned long long cur;
+  {
+    const unsigned long long bit = 1ull << (start & 63);
+    const unsigned long long old = atomicAnd(&bm[w], ~bit);
+    if (old & bit) {
+      *out = start;
+      return true;
+    }
+    cur = old & m;
+  }
   for (uint64_t it = 0; it < budget; ++it) {
-    unsigned long long cur = load_word(&bm[w]) & m;
     while (cur) {
       const int b = __ffsll(cur) - 1;
       const unsigned long long bit = 1ull << b;
@@ -283,6 +308,7 @@ __device__ inline bool bm_claim(unsigned long long* bm, uint64_t start, uint64_t
     }
     m = ~0ull;
     w = (w + 1 == words) ? 0 : w + 1;
+    cur = load_word(&bm[w]);
   }
   return false;
 }
@@ -519,7 +545,10 @@ __device__ inline uint32_t insert_walk(const Table& t, Blob<KS> ck, Blob<VS> cv,
     // Static key sizes: the slot's key is loaded with its flag word (one
     // round trip for the compare or the eviction instead of two).
     Blob<KS> rk;
-    if constexpr (KS > 0) rk.load(t.keys + idx * K, K);
+    if constexpr (KS > 0) {
+      rk.load(t.keys + idx * K, K);
+      keep_loaded(f, rk);
+    }
     const uint32_t st = f & 0xFu;
     const uint64_t d = f >> 4;
     if (st == kOccupied) {
@@ -669,11 +698,10 @@ __device__ uint32_t insert_one_wave(const Table& t, const uint8_t* __restrict__ 
   for (;;) {
     const uint64_t s = (pos + lane) & t.mask;
     const uint32_t f = t.flags[s];
+    Blob<KS> rk;  // read with the flag (one round trip); unused unless occupied
+    rk.load(t.keys + s * KS, KS);
+    keep_loaded(f, rk);
     const bool occ = (f & 0xFu) == kOccupied;
-    Blob<KS> rk;
-#pragma unroll
-    for (int w = 0; w < KW; ++w) rk.w[w] = 0;
-    if (occ) rk.load(t.keys + s * KS, KS);
     auto eq_ballot = [&]() {
       uint32_t diff = occ ? 0u : 1u;
 #pragma unroll
@@ -981,47 +1009,266 @@ __device__ uint32_t replay_arc_lds(const Table& t, const uint8_t* __restrict__ i
   return claimed;
 }
 
-// Leaders order and replay closures of up to kLeaderMax keys whose arc
-// (barrier to barrier) is at most `coop_arc` slots; other closures are
-// queued for k_ins_big.
+// A one-member closure's insert decided from its flags.  Key i alone in its
+// arc walks home .. s at most (s: its dry slot; every slot from home up to
+// s was OCCUPIED before the batch, or the dry pass would have claimed it),
+// and insert_walk's decisions on those slots (pass, compare an equal-dist
+// resident's key, evict) depend on the flags alone except for the key
+// comparisons.  So the flags of home .. s-1 (fl[j] = flag of slot home+j,
+// j < L = s - home, read with the election's bitmap words) give every
+// comparison point and eviction point of the walk as if no key were equal;
+// one round of reads brings the keys at those points and the residents to
+// be moved, the comparisons are taken in walk order (an equal key ends the
+// walk with a value update there, the pairs evicted before it moved), and
+// one round of writes applies the keys and flags.  The value moves (at most
+// kPlanEvicts + 1 copies of V bytes) are returned in `mv` as (source,
+// destination) byte addresses in reverse walk order, for the caller to
+// apply: copying them in that order, each batch of copies reading all its
+// sources before writing, reads every value before it is overwritten.
+// vin: the batch value of key i.  Returns false, with nothing written, when
+// the walk needs more comparisons or evictions than it keeps in registers;
+// the caller then walks.  *c1: 1 if s was claimed.
+constexpr uint32_t kPlanSlots = 16;
+constexpr uint32_t kPlanChecks = 3;
+constexpr uint32_t kPlanEvicts = 2;
+constexpr uint32_t kPlanMoves = kPlanEvicts + 1;
+
+template <int KS, int VS>
+__device__ inline bool insert_planned(const Table& t, const Blob<KS>& ck, const uint8_t* vin, uint64_t home,
+                                      uint64_t s, uint32_t L, const uint32_t (&fl)[kPlanSlots], uint32_t* c1,
+                                      uint64_t (&mv)[2 * kPlanMoves], uint32_t* nmv) {
+  if (L > kPlanSlots) return false;
+  uint32_t p = 0, ne = 0, nc = 0, ep0 = 0, ep1 = 0, cmask = 0, emask = 0;
+  bool ok = true;
+#pragma unroll
+  for (uint32_t j = 0; j < kPlanSlots; ++j) {
+    if (j < L) {
+      const uint32_t f = fl[j];
+      const uint32_t d = f >> 4;
+      if ((f & 0xFu) != kOccupied || p > t.mask) ok = false;  // (the walk would drop a pair past a lap)
+      if (d == p) {  // an equal key would end the walk here
+        cmask |= 1u << j;
+        ++nc;
+        ++p;
+      } else if (p > d) {  // Robin Hood eviction: the carried pair settles, the resident is carried on
+        emask |= 1u << j;
+        if (ne == 0) ep0 = p;
+        else ep1 = p;
+        ++ne;
+        p = d + 1;
+      } else {
+        ++p;
+      }
+    }
+  }
+  if (!ok || p > t.mask || nc > kPlanChecks || ne > kPlanEvicts) return false;
+  const uint32_t K = KS, V = VS;
+  auto slot = [&](uint32_t j) { return (home + j) & t.mask; };
+  const uint32_t e0 = emask ? static_cast<uint32_t>(__ffs(emask) - 1) : 0u;
+  const uint32_t e1 = ne > 1 ? static_cast<uint32_t>(__ffs(emask & (emask - 1)) - 1) : 0u;
+  Blob<KS> ke0, ke1, kc0, kc1, kc2;
+  uint32_t cj0 = 0, cj1 = 0, cj2 = 0;
+  {
+    uint32_t m = cmask;
+    cj0 = m ? static_cast<uint32_t>(__ffs(m) - 1) : 0u;
+    m &= m - 1;
+    cj1 = m ? static_cast<uint32_t>(__ffs(m) - 1) : 0u;
+    m &= m - 1;
+    cj2 = m ? static_cast<uint32_t>(__ffs(m) - 1) : 0u;
+  }
+  if (ne > 0) ke0.load(t.keys + slot(e0) * K, K);
+  if (ne > 1) ke1.load(t.keys + slot(e1) * K, K);
+  if (nc > 0) kc0.load(t.keys + slot(cj0) * K, K);
+  if (nc > 1) kc1.load(t.keys + slot(cj1) * K, K);
+  if (nc > 2) kc2.load(t.keys + slot(cj2) * K, K);
+  // comparisons in walk order; the pair carried at point j is the one that
+  // the evictions before j left (0: the batch pair).  Every candidate pair
+  // is compared and the result selected: selecting among the blobs
+  // themselves would put them in scratch memory.
+  const uint32_t c0 = static_cast<uint32_t>(__popc(emask & ((1u << cj0) - 1u)));
+  const uint32_t c1_ = static_cast<uint32_t>(__popc(emask & ((1u << cj1) - 1u)));
+  const uint32_t c2_ = static_cast<uint32_t>(__popc(emask & ((1u << cj2) - 1u)));
+  const bool q0 = c0 == 0 ? ck.eqb(kc0) : (c0 == 1 ? ke0.eqb(kc0) : ke1.eqb(kc0));
+  const bool q1 = c1_ == 0 ? ck.eqb(kc1) : (c1_ == 1 ? ke0.eqb(kc1) : ke1.eqb(kc1));
+  const bool q2 = c2_ == 0 ? ck.eqb(kc2) : (c2_ == 1 ? ke0.eqb(kc2) : ke1.eqb(kc2));
+  uint32_t stop_j = ~0u;
+  if (nc > 0 && q0) stop_j = cj0;
+  else if (nc > 1 && q1) stop_j = cj1;
+  else if (nc > 2 && q2) stop_j = cj2;
+  const uint32_t applied = stop_j != ~0u ? static_cast<uint32_t>(__popc(emask & ((1u << stop_j) - 1u))) : ne;
+  const uint64_t vb = reinterpret_cast<uint64_t>(t.vals);
+  const uint64_t src_e0 = vb + slot(e0) * V, src_e1 = vb + slot(e1) * V;
+  if (applied >= 1) {
+    ck.store(t.keys + slot(e0) * K, K);
+    t.flags[slot(e0)] = make_flag(kOccupied, ep0);
+  }
+  if (applied >= 2) {
+    ke0.store(t.keys + slot(e1) * K, K);
+    t.flags[slot(e1)] = make_flag(kOccupied, ep1);
+  }
+  // the last pair carried: to the equal key's slot (value only) or to s
+  mv[0] = applied == 0 ? reinterpret_cast<uint64_t>(vin) : (applied == 1 ? src_e0 : src_e1);
+  if (stop_j != ~0u) {
+    mv[1] = vb + slot(stop_j) * V;
+    *c1 = 0;
+  } else {
+    if (ne == 0) ck.store(t.keys + s * K, K);
+    else if (ne == 1) ke0.store(t.keys + s * K, K);
+    else ke1.store(t.keys + s * K, K);
+    t.flags[s] = make_flag(kOccupied, p);
+    mv[1] = vb + s * V;
+    *c1 = 1;
+  }
+  // then the resident of e0 -> e1 (two evictions applied), and the batch
+  // pair -> e0 (fixed indices: a computed one would put mv in scratch)
+  mv[2] = applied >= 2 ? src_e0 : reinterpret_cast<uint64_t>(vin);
+  mv[3] = applied >= 2 ? src_e1 : src_e0;
+  mv[4] = reinterpret_cast<uint64_t>(vin);
+  mv[5] = src_e0;
+  *nmv = 1 + applied;
+  return true;
+}
+
+// Applies a wave's list of V-byte value copies (lst_src/lst_dst, `total`
+// entries in LDS) with all 64 lanes: V/16 lanes per copy, 16 bytes each, so
+// one copy is one request per line instead of V/16.  The list is taken in
+// batches of 64 copies staged through `stage` (64 * V bytes of LDS): every
+// source of a batch is read before any destination of it is written, so
+// lists built in reverse walk order per walk (insert_planned) read every
+// value before it is overwritten.
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+template <int VS>
+__device__ inline void wave_copy_values(const uint64_t* lst_src, const uint64_t* lst_dst, uint32_t total,
+                                        u32x4* stage) {
+  static_assert(VS % 16 == 0 && VS >= 16 && VS <= 1024, "16-byte parts");
+  constexpr uint32_t kParts = VS / 16, kPerRow = 64 / kParts, kRows = kParts;  // 64 copies per batch
+  typedef __attribute__((address_space(1))) const u32x4 gsrc_t;  // global memory (not generic) accesses
+  typedef __attribute__((address_space(1))) u32x4 gdst_t;
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t part = lane % kParts, row_copy = lane / kParts;
+  for (uint32_t c0 = 0; c0 < total; c0 += kPerRow * kRows) {
+#pragma unroll
+    for (uint32_t r = 0; r < kRows; ++r) {  // no branch: every load in flight at once (past the end: the last copy)
+      const uint32_t c = min(c0 + r * kPerRow + row_copy, total - 1);
+      stage[r * 64 + lane] = reinterpret_cast<gsrc_t*>(lst_src[c])[part];
+    }
+    wave_sync();
+#pragma unroll
+    for (uint32_t r = 0; r < kRows; ++r) {
+      const uint32_t c = c0 + r * kPerRow + row_copy;
+      if (c < total) reinterpret_cast<gdst_t*>(lst_dst[c])[part] = stage[r * 64 + lane];
+    }
+    wave_sync();
+  }
+}
+
+// Leaders order and replay closures of up to leader_max (<= kLeaderMax)
+// keys whose arc (barrier to barrier) is at most `coop_arc` slots; other
+// closures are queued for k_ins_big.  With `plan` (sparse batches, where
+// nearly every key is a one-member closure) every lane first reads its own
+// key and the flags of its home .. dry slot together with the election's
+// bitmap words, so a one-member closure is decided and applied with two
+// further rounds of memory traffic (insert_planned).
 template <int KS, int VS>
 __global__ void __launch_bounds__(64)
 k_ins_leader(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __restrict__ in_vals, uint64_t n,
              const unsigned long long* __restrict__ Q, const uint32_t* __restrict__ dslot, uint32_t* __restrict__ big,
-             Counters* ctr, uint32_t coop_arc) {
+             Counters* ctr, uint32_t coop_arc, uint32_t leader_max, bool plan) {
   __shared__ uint32_t lst[kLeaderMax * 64];  // column per lane: lst[j * 64 + lane]
+  // planned one-member inserts for static sizes; values of 16-byte multiples
+  // are copied by the whole wave, smaller ones by each lane
+  constexpr bool kPlan = KS > 0 && KS % 4 == 0 && VS > 0 && VS % 4 == 0;
+  constexpr bool kCoop = kPlan && VS % 16 == 0;
+  __shared__ uint64_t mv_src[kCoop ? 64 * kPlanMoves : 1], mv_dst[kCoop ? 64 * kPlanMoves : 1];
+  __shared__ u32x4 mv_stage[kCoop ? 64 * (VS / 16) : 1];
   const uint32_t K = KS ? KS : t.K;
   const uint32_t V = VS ? VS : t.V;
   const unsigned long long* P = t.freebits;
+  const uint64_t words = (t.cap + 63) >> 6;
   uint32_t* my = lst + threadIdx.x;
   if (ctr->status & kStatusDryFail) return;  // block-uniform
   unsigned long long claimed = 0;
   for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * blockDim.x;; base += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
     if (base >= n) break;
     const uint64_t i = base + threadIdx.x;
-    if (i < n && dslot[i] != kNoSlot) {
-      const uint64_t s = dslot[i];
-      const uint64_t b = bm_prev_set(P, s, t.cap);
-      if (b != s && bit_at(Q, b)) {
+    const uint32_t sd = i < n ? dslot[i] : kNoSlot;
+    uint64_t mv[2 * kPlanMoves];
+    uint32_t nmv = 0;
+    if (sd != kNoSlot) {
+      const uint64_t s = sd;
+      Blob<KS> ck;
+      uint64_t home = 0;
+      uint32_t L = ~0u;
+      uint32_t fl[kPlanSlots];
+      if (kPlan && plan) {
+        ck.load(in_keys + i * K, K);
+        home = ck.hash(K, t.mode) & t.mask;
+        L = static_cast<uint32_t>(min<uint64_t>((s - home) & t.mask, kPlanSlots + 1));
+#pragma unroll
+        for (uint32_t j = 0; j < kPlanSlots; ++j) fl[j] = (L <= kPlanSlots && j < L) ? t.flags[(home + j) & t.mask] : 0u;
+      }
+      // The free-slot and claim words at the dry slot, read together (and
+      // with the flags above); most closures begin, and end, inside that word.
+      uint64_t cw = s >> 6;
+      unsigned long long cp = P[cw], cq = Q[cw];
+      if (kPlan && plan) {
+#pragma unroll
+        for (uint32_t j = 0; j < kPlanSlots; ++j) asm volatile("" ::"v"(fl[j]));
+      }
+      const unsigned long long below = cp & ((1ull << (s & 63)) - 1ull);
+      uint64_t b;
+      unsigned long long qb;
+      if (below) {
+        b = (cw << 6) + static_cast<uint64_t>(63 - __clzll(below));
+        qb = cq;
+      } else {
+        b = bm_prev_set(P, s, t.cap);
+        qb = Q[b >> 6];
+      }
+      if (b != s && ((qb >> (b & 63)) & 1ull)) {
         uint32_t m = 0;
         uint64_t pos = s;
         bool long_arc = false;
         for (;;) {
-          const uint64_t x = bm_next_set(P, pos, t.cap);
-          if (bit_at(Q, x)) {  // the closure's barrier
+          // next pre-batch free slot at/after pos (cyclic), from the cached
+          // word pair or the next ones
+          if ((pos >> 6) != cw) {
+            cw = pos >> 6;
+            cp = P[cw];
+            cq = Q[cw];
+          }
+          unsigned long long bits = cp & (~0ull << (pos & 63));
+          for (uint64_t it = 0; !bits && it <= words; ++it) {  // b is set: found within a lap
+            cw = (cw + 1 == words) ? 0 : cw + 1;
+            cp = P[cw];
+            cq = Q[cw];
+            bits = cp;
+          }
+          const uint64_t x = (cw << 6) + static_cast<uint64_t>(__ffsll(bits) - 1);
+          if ((cq >> (x & 63)) & 1ull) {  // the closure's barrier
             long_arc = ((x - b - 1) & t.mask) > coop_arc;
             break;
           }
-          if (m == kLeaderMax) {
+          if (m == leader_max) {
             m = kLeaderMax + 1;
             break;
           }
-          my[m * 64] = t.owner[x];
+          my[m * 64] = (x == s) ? static_cast<uint32_t>(i) : t.owner[x];  // owner[s] == i (k_ins_dry)
           ++m;
           pos = (x + 1 == t.cap) ? 0 : x + 1;
         }
+        uint32_t c1 = 0;
         if (m > kLeaderMax || long_arc) {
           big[atomicAdd(&ctr->nbig, 1u)] = static_cast<uint32_t>(s);
+        } else if (kPlan && plan && m == 1 &&
+                   insert_planned<KS, VS>(t, ck, in_vals + dd_value_index(t, i) * V, home, s, L, fl, &c1, mv, &nmv)) {
+          claimed += c1;
+          if (t.outcome) t.outcome[i] = static_cast<uint8_t>(c1);
+        } else if (m == 1 && !(kPlan && plan)) {  // key i alone, dense batch: walked from its own pair
+          c1 = insert_one<KS, VS, false>(t, in_keys + i * K, in_vals + dd_value_index(t, i) * V, K, V, ctr);
+          claimed += c1;
+          if (t.outcome) t.outcome[i] = static_cast<uint8_t>(c1);
         } else {
           for (uint32_t a = 1; a < m; ++a) {  // insertion sort by batch index
             const uint32_t v = my[a * 64];
@@ -1034,12 +1281,45 @@ k_ins_leader(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __rest
           }
           for (uint32_t j = 0; j < m; ++j) {
             const uint64_t k = my[j * 64];
-            const uint32_t c1 =
-                insert_one<KS, VS, false>(t, in_keys + k * K, in_vals + dd_value_index(t, k) * V, K, V, ctr);
+            c1 = insert_one<KS, VS, false>(t, in_keys + k * K, in_vals + dd_value_index(t, k) * V, K, V, ctr);
             claimed += c1;
             if (t.outcome) t.outcome[k] = static_cast<uint8_t>(c1);
           }
         }
+      }
+    }
+    if constexpr (kPlan && !kCoop) {
+      // the planned inserts' value copies, each lane its own: every source
+      // read before the first write
+      Blob<VS> a, b, c;
+      if (nmv > 0) a.load(reinterpret_cast<const uint8_t*>(mv[0]), V);
+      if (nmv > 1) b.load(reinterpret_cast<const uint8_t*>(mv[2]), V);
+      if (nmv > 2) c.load(reinterpret_cast<const uint8_t*>(mv[4]), V);
+      if (nmv > 0) a.store(reinterpret_cast<uint8_t*>(mv[1]), V);
+      if (nmv > 1) b.store(reinterpret_cast<uint8_t*>(mv[3]), V);
+      if (nmv > 2) c.store(reinterpret_cast<uint8_t*>(mv[5]), V);
+    }
+    if constexpr (kCoop) {
+      // the planned inserts' value copies, by the whole wave (wave-uniform)
+      uint32_t incl = nmv;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = static_cast<uint32_t>(__shfl_up(static_cast<int>(incl), off, 64));
+        if (threadIdx.x >= static_cast<uint32_t>(off)) incl += y;
+      }
+      const uint32_t total = static_cast<uint32_t>(__shfl(static_cast<int>(incl), 63, 64));
+      if (total) {
+        const uint32_t first = incl - nmv;
+#pragma unroll
+        for (uint32_t k = 0; k < kPlanMoves; ++k) {
+          if (k < nmv) {
+            mv_src[first + k] = mv[2 * k];
+            mv_dst[first + k] = mv[2 * k + 1];
+          }
+        }
+        wave_sync();
+        wave_copy_values<VS>(mv_src, mv_dst, total, mv_stage);
+        wave_sync();
       }
     }
   }
@@ -1236,7 +1516,7 @@ __device__ inline void hand_pending(Counters* ctr, Counters* host_out, uint32_t 
 }
 
 // Publishes the batch's real claims (a subset of its dry claims) in both
-// bitmaps and resets owner[] for the next op.  Q (optional): the dry pass's
+// bitmaps.  Q (optional): the dry pass's
 // claim bitmap; a dry claim that ended unused (the key updated an existing
 // slot) gets its Q bit back, which leaves Q equal to the new freebits, so
 // the next insert needs no copy.  host_out (optional): the last workgroup
@@ -1267,7 +1547,6 @@ k_ins_bits(Table t, const uint32_t* __restrict__ dslot, uint64_t n, Counters* ct
     } else if (Q) {
       atomicOr(&Q[s >> 6], bit);
     }
-    t.owner[s] = kNoOwner;
   }
   // No fence: this kernel changes none of the published fields and every
   // workgroup's read of ctr->status precedes its ticket.  (An agent-scope
@@ -1286,14 +1565,12 @@ k_ins_bits(Table t, const uint32_t* __restrict__ dslot, uint64_t n, Counters* ct
 // from the flags, one wave per 64 slots, instead of one atomic per key.
 // After the replay the flags are the truth, so freebits (not OCCUPIED),
 // emptybits (EMPTY) and Q (= freebits) come out exactly as k_ins_bits
-// leaves them, and every owner[] entry is reset.  A failed dry pass left
+// leaves them.  A failed dry pass left
 // the flags untouched, so the words come out unchanged.  The last
 // workgroup hands the counters to host memory as k_ins_bits does.
 static __global__ void __launch_bounds__(1024)
-k_ins_bits_rebuild(Table t, Counters* ctr, unsigned long long* Q, Counters* host_out, uint32_t seq,
-                   bool defer_big) {
-  const bool pending = defer_big && ld_agent(&ctr->nbig) != 0;  // as in k_ins_bits
-  const uint64_t words = pending ? 0 : (t.cap + 63) >> 6;
+k_ins_bits_rebuild(Table t, Counters* ctr, unsigned long long* Q, Counters* host_out, uint32_t seq) {
+  const uint64_t words = (t.cap + 63) >> 6;
   const uint32_t lane = threadIdx.x & 63u;
   const uint64_t wave = (static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
   const uint64_t waves = (static_cast<uint64_t>(gridDim.x) * blockDim.x) >> 6;
@@ -1307,14 +1584,11 @@ k_ins_bits_rebuild(Table t, Counters* ctr, unsigned long long* Q, Counters* host
       t.emptybits[w] = eb;
       if (Q) Q[w] = fb;
     }
-    if (i < t.cap) t.owner[i] = kNoOwner;
   }
   if (!host_out) return;
   __syncthreads();
-  if (threadIdx.x == 0 && atomicAdd(&ctr->done, 1u) == gridDim.x - 1) {
-    if (pending) hand_pending(ctr, host_out, seq);
-    else hand_to_host(ctr, host_out, seq, __hip_atomic_load(&ctr->count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-  }
+  if (threadIdx.x == 0 && atomicAdd(&ctr->done, 1u) == gridDim.x - 1)
+    hand_to_host(ctr, host_out, seq, __hip_atomic_load(&ctr->count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
 }
 
 // ---------------------------------------------------------------------------
@@ -2160,7 +2434,7 @@ k_ins_fused(Table t, FusedInsert a) {
 // ---------------------------------------------------------------------------
 
 // Pass 1: first event of each key on the pre-batch table; matches register
-// their index in owner[slot] with atomicMin (the earliest remover wins).
+// their index in rowner[slot] with atomicMin (the earliest remover wins).
 template <int KS, int G>
 __global__ void __launch_bounds__(1024)
 k_rem_probe(Table t, const uint8_t* __restrict__ qkeys, uint64_t n, unsigned long long* __restrict__ ev) {
@@ -2178,7 +2452,7 @@ k_rem_probe(Table t, const uint8_t* __restrict__ qkeys, uint64_t n, unsigned lon
     group_probe<KS, G>(t, q, K, home, gl, gshift, &kind, &slot);
     if (gl == 0) {
       ev[qi] = (static_cast<unsigned long long>(kind) << 32) | slot;
-      if (kind == kEvMatch) atomicMin(&t.owner[slot], static_cast<uint32_t>(qi));
+      if (kind == kEvMatch) atomicMin(&t.rowner[slot], static_cast<uint32_t>(qi));
     }
   }
 }
@@ -2196,18 +2470,18 @@ k_rem_check(Table t, const unsigned long long* __restrict__ ev, uint64_t n, uint
     uint8_t a = 0;
     bool conflict = false;
     if (kind == kEvMatch) {
-      const uint32_t o = t.owner[slot];
+      const uint32_t o = t.rowner[slot];
       a = (o == static_cast<uint32_t>(i)) ? 1 : 0;
       conflict = !a;
     } else if (kind == kEvTermOcc) {
-      conflict = t.owner[slot] < static_cast<uint32_t>(i);
+      conflict = t.rowner[slot] < static_cast<uint32_t>(i);
     }
     action[i] = a;
     if (conflict) atomicOr(&ctr->status, kStatusConflict);
   }
 }
 
-// Pass 3: apply (only when the batch had no conflict) and reset owner[].
+// Pass 3: apply (only when the batch had no conflict) and reset rowner[].
 // host_out (optional): the last workgroup hands the final counters to host
 // memory (hand_to_host).  Each workgroup then adds its count and its
 // completion ticket in one atomic on ctr->count (ticket above bit
@@ -2232,7 +2506,7 @@ k_rem_apply(Table t, const unsigned long long* __restrict__ ev, const uint8_t* _
         if (t.freecopy) atomicOr(&t.freecopy[slot >> 6], 1ull << (slot & 63));
         removed = 1;
       }
-      t.owner[slot] = kNoOwner;
+      t.rowner[slot] = kNoOwner;
     }
     if (t.outcome && i < n && !conflict) t.outcome[i] = static_cast<uint8_t>(removed);
     total += removed;

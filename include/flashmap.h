@@ -128,9 +128,12 @@ fm_status fm_last_table_full(uint64_t* occupied, uint64_t* capacity,
  * (lib.rs:235).  Device buffers come from fm_alloc_device / fm_upload (or
  * any device allocation on the map's device, 16-byte aligned).  Work is
  * ordered on the map's stream.  Calls that return a count return once the
- * op is done: the caller's buffers are read and every table write is made
- * (the op's last kernel hands the count to host memory and may still be
- * exiting; fm_synchronize also waits for that). */
+ * op's last kernel has handed the count to host memory: the caller's
+ * buffers are read and the op's table writes are visible to later work on
+ * the map's stream, which runs after them.  Work on any other stream (or a
+ * host read of the caller's buffers after a device write by the op) must
+ * call fm_synchronize first: that kernel may still be exiting and its
+ * writes are not yet released beyond the map's stream. */
 
 /* FlashMap::device (lib.rs:218-224): the hipStream_t of the map. */
 void* fm_device_stream(fm_map* m);

@@ -610,10 +610,11 @@ def test_gpu_max_capacity_wraps_at_2_31():
 def test_gpu_sparse_batches_that_queue_big_closures(K, V):
     """Counted inserts of sparse batches (n * 16 < cap) launch no k_ins_big up
     front: when the leader queues closures, the settle pass reports them and
-    the host runs k_ins_big and the settle again.  In a near-full table most
-    such batches queue closures and the smallest queue none; batches of
-    updates (with repeated keys) sit between them.  The whole slot array is
-    compared with the oracle after every op."""
+    the host runs k_ins_big and the settle again.  In a near-full table such
+    batches queue closures (batches of updates, with repeated keys, sit
+    between them); in a lightly loaded one a sparse batch queues none and
+    settles in one pass.  The whole slot array is compared with the oracle
+    after every op, and the launch counts show which branch ran."""
     say(f"sparse batches with queued closures K={K} V={V}")
     rng = np.random.default_rng(700 + K)
     cap = 1 << 13
@@ -629,4 +630,19 @@ def test_gpu_sparse_batches_that_queue_big_closures(K, V):
     ops.append(("rem", keys[:3000:3]))
     ops.append(("ins", keys[:400:2], vals[:200]))
     ops.append(("get", keys))
-    apply_and_compare(m, o, ops)
+    apply_and_compare(m, o, ops[:1])
+    m.profile(True)  # the sparse inserts only
+    apply_and_compare(m, o, ops[1:])
+    prof = m.profile_read()
+    sparse = sum(1 for op in ops[1:] if op[0] == "ins")
+    settles = prof.get("k_ins_bits", (0, 0))[0]
+    assert prof.get("k_ins_big", (0, 0))[0] >= 1, prof  # some batch queued closures: deferred, then replayed
+    assert sparse < settles <= 2 * sparse, (sparse, settles)  # ... and settled a second time
+    # the other branch: a sparse batch into a lightly loaded table queues
+    # nothing and settles in one pass
+    m2, o2 = gpu_make(cap, K, V), OracleAdapter(cap, K, V)
+    apply_and_compare(m2, o2, [("ins", keys[:2000], vals[:2000])])
+    m2.profile(True)
+    apply_and_compare(m2, o2, [("ins", keys[2000:2100], vals[2000:2100])])
+    prof2 = m2.profile_read()
+    assert prof2.get("k_ins_bits", (0, 0))[0] == 1 and prof2.get("k_ins_big", (0, 0))[0] == 0, prof2
