@@ -82,6 +82,9 @@ struct Ops {
                         uint32_t, hipStream_t);
   hipError_t (*ins_leader)(const Table&, const uint8_t*, const uint8_t*, uint64_t, const unsigned long long*,
                            const uint32_t*, uint32_t*, Counters*, uint32_t, uint32_t, bool, hipStream_t);
+  // null for the generic sizes
+  hipError_t (*ins_arcs)(const Table&, const uint8_t*, const uint8_t*, const unsigned long long*, uint32_t*,
+                         Counters*, hipStream_t);
   hipError_t (*ins_big)(const Table&, const uint8_t*, const uint8_t*, uint64_t, const unsigned long long*,
                         const uint32_t*, Counters*, hipStream_t);
   hipError_t (*ins_replay)(const Table&, const uint8_t*, const uint8_t*, uint64_t, const uint32_t*, const uint32_t*,
@@ -137,8 +140,21 @@ template <int KS, int VS>
 hipError_t launch_ins_leader(const Table& t, const uint8_t* k, const uint8_t* v, uint64_t n,
                              const unsigned long long* Q, const uint32_t* dslot, uint32_t* big, Counters* c,
                              uint32_t coop_arc, uint32_t leader_max, bool plan, hipStream_t s) {
-  hipLaunchKernelGGL((dev::k_ins_leader<KS, VS>), dim3(grid_for(n, 64)), dim3(64), 0, s, t, k, v, n, Q, dslot, big,
-                     c, coop_arc, leader_max, plan);
+  constexpr bool kCanPlan = KS > 0 && KS % 4 == 0 && VS > 0 && VS % 4 == 0;
+  if (kCanPlan && plan)
+    hipLaunchKernelGGL((dev::k_ins_leader<KS, VS, kCanPlan>), dim3(grid_for(n, 64)), dim3(64), 0, s, t, k, v, n, Q,
+                       dslot, big, c, coop_arc, leader_max);
+  else
+    hipLaunchKernelGGL((dev::k_ins_leader<KS, VS, false>), dim3(grid_for(n, 64)), dim3(64), 0, s, t, k, v, n, Q,
+                       dslot, big, c, coop_arc, leader_max);
+  return hipGetLastError();
+}
+
+// One wave per bitmap word (cap >= 256, checked by the caller).
+template <int KS, int VS>
+hipError_t launch_ins_arcs(const Table& t, const uint8_t* k, const uint8_t* v, const unsigned long long* Q,
+                           uint32_t* big, Counters* c, hipStream_t s) {
+  hipLaunchKernelGGL((dev::k_ins_arcs<KS, VS>), dim3(grid_for(t.cap, 1024)), dim3(1024), 0, s, t, k, v, Q, big, c);
   return hipGetLastError();
 }
 
@@ -228,6 +244,8 @@ Ops make_ops() {
   o.ins_dry = &launch_ins_dry<KS>;
   o.ins_leader = &launch_ins_leader<KS, VS>;
   o.ins_big = &launch_ins_big<KS, VS>;
+  if constexpr (KS > 0 && VS > 0 && KS % 4 == 0 && VS % 4 == 0) o.ins_arcs = &launch_ins_arcs<KS, VS>;
+  else o.ins_arcs = nullptr;
   o.ins_replay = &launch_ins_replay<KS, VS>;
   o.rem_probe = &launch_rem_probe<KS>;
   o.rem_runs = &launch_rem_runs<KS>;
@@ -311,6 +329,11 @@ class GpuBackend final : public Backend {
     if ((st = run_insert(static_cast<const uint8_t*>(d_keys), static_cast<const uint8_t*>(d_vals), n, d_claimed)) !=
         FM_OK)
       return st;
+    // No status readback here: len_ is an upper bound while serving (an
+    // insert adds its whole batch, a remove subtracts once its flags are
+    // back), so a led insert's dry pass cannot fail and its settle pass
+    // leaves Q_ equal to freebits: the next insert needs no bitmap copy.
+    q_synced_ = last_insert_ == InsertPath::kLed && q_pending_;
     len_ += n;
     return FM_OK;
   }
@@ -432,6 +455,7 @@ class GpuBackend final : public Backend {
   fm_status ensure_dedup(uint64_t n);
   bool tune_sort_only_ = false;  // FM_INSERT_SORT_PATH=1 forces the sort-grouped insert
   uint32_t leader_max_ = dev::kLeaderMax;  // FM_LEADER_MAX: members a leader lane replays (more: k_ins_big)
+  bool arcs_on_ = true;                    // FM_INSERT_ARCS=0: dense batches led by k_ins_leader instead
   enum class InsertPath { kSort, kLed, kFused };
   InsertPath last_insert_ = InsertPath::kSort;
   uint32_t fused_blocks_ = 0;  // k_ins_fused workgroups per launch at most (0: no cooperative launch)
@@ -476,7 +500,7 @@ class GpuBackend final : public Backend {
   enum {
     kKidGet, kKidInsCopy, kKidInsDry, kKidInsClosure, kKidInsSort, kKidInsReplay, kKidRemProbe, kKidRemCheck,
     kKidRemApply, kKidRemRuns, kKidRemSort, kKidRemReplay, kKidCopyTable, kKidInsLeader, kKidInsBig, kKidInsBits,
-    kKidInsFused, kKidInsDedup, kNumKid
+    kKidInsFused, kKidInsDedup, kKidInsArcs, kNumKid
   };
   struct PendingTiming {
     int kid;
@@ -649,6 +673,7 @@ fm_status GpuBackend::init(const fm_builder& b, std::string* err) {
   if (ops_.ins_fused_blocks(dev_, &fused_blocks_) != hipSuccess) fused_blocks_ = 0;  // multi-kernel path instead
   if (const char* env = std::getenv("FM_FUSED_SPIN")) fused_spin_ = static_cast<uint32_t>(std::strtoul(env, nullptr, 10));
   if (const char* env = std::getenv("FM_INSERT_SORT_PATH")) tune_sort_only_ = env[0] == '1';
+  if (const char* env = std::getenv("FM_INSERT_ARCS")) arcs_on_ = std::strcmp(env, "0") != 0;
   if (const char* env = std::getenv("FM_LEADER_MAX"))
     leader_max_ = static_cast<uint32_t>(std::max(1ul, std::min<unsigned long>(std::strtoul(env, nullptr, 10), dev::kLeaderMax)));
   sort_end_bit_ = 1;
@@ -976,10 +1001,18 @@ fm_status GpuBackend::run_insert(const uint8_t* d_keys, const uint8_t* d_vals, u
           FM_OK)
         return st;
     }
-    {
+    // Sparse batches: leaders per key, one-member closures planned.  Dense
+    // ones (n * 16 >= cap, most bitmap words hold claims): one wave per
+    // bitmap word replays the arcs of up to 64 slots that start in it.
+    const bool plan = n * 16 < t_.cap;
+    if (!plan && arcs_on_ && ops_.ins_arcs && t_.cap >= 256) {
+      Prof p(this, kKidInsArcs);
+      if ((st = check_launch(ops_.ins_arcs(t, d_keys, d_vals, Q_, sc.big, ctr_, stream_), "k_ins_arcs")) != FM_OK)
+        return st;
+    } else {
       Prof p(this, kKidInsLeader);
       if ((st = check_launch(ops_.ins_leader(t, d_keys, d_vals, n, Q_, sc.dslot, sc.big, ctr_, tune_.insert_coop_arc,
-                                             leader_max_, n * 16 < t_.cap, stream_),
+                                             leader_max_, plan, stream_),
                              "k_ins_leader")) != FM_OK)
         return st;
     }
@@ -1114,7 +1147,7 @@ const char* GpuBackend::kid_name(int k) {
                                        "ins_radix_sort", "k_ins_replay",  "k_rem_probe",  "k_rem_check",
                                        "k_rem_apply",  "k_rem_runs",      "rem_radix_sort", "k_rem_replay",
                                        "copy_table",   "k_ins_leader",    "k_ins_big",    "k_ins_bits",
-                                       "k_ins_fused",  "k_ins_dedup"};
+                                       "k_ins_fused",  "k_ins_dedup",     "k_ins_arcs"};
   return names[k];
 }
 

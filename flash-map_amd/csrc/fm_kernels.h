@@ -1163,22 +1163,264 @@ __device__ inline void wave_copy_values(const uint64_t* lst_src, const uint64_t*
   }
 }
 
+// ---------------------------------------------------------------------------
+// Dense batches: arcs replayed in registers, one wave per bitmap word.  A
+// closure's arc is the run of slots between two barriers (pre-batch free
+// slots that no dry claim took), and its members walk only inside it (the
+// exactness contract above).  The wave of word w takes every arc that
+// starts in w's 64 slots and has at most kArcSlots slots: lane = slot, the
+// arc's flags, keys and value sources in registers, and the members (the
+// owners of its dry claims) inserted in batch order, each walk decided by
+// one wave scan, the prefix-minimum form of replay_arc_lds over a single
+// window (so the result is insert_walk's, slot for slot).  The arcs that
+// end inside the word are replayed together in one round, the one that
+// runs into the next word in a second.  Longer arcs are queued for
+// k_ins_big by the dry slot of their first member, the entry k_ins_leader
+// queues.  Every changed slot is written once at the end, its value read
+// from its pre-batch slot or its batch position before any value is
+// written.  Requires cap >= 256 (the window's three bitmap words differ).
+// ---------------------------------------------------------------------------
+constexpr uint32_t kArcSlots = 64;
+constexpr uint32_t kVsBatch = 0x100u;  // value source: the batch value of the member in lane (vs & 63)
+
+// One round: lanes are the slots base .. base+63, `act` marks the slots of
+// the arcs replayed and `mem` their dry-claimed slots.  Returns the slots
+// claimed (the same in every lane).
+template <int KS, int VS>
+__device__ __forceinline__ uint32_t arcs_round(const Table& t, const uint8_t* __restrict__ in_keys,
+                                               const uint8_t* __restrict__ in_vals, uint64_t base,
+                                               unsigned long long act, unsigned long long mem, Counters* ctr) {
+  constexpr int KW = KS / 4;
+  const uint32_t lane = threadIdx.x & 63u;
+  const bool in = (act >> lane) & 1ull;
+  const bool is_mem = (mem >> lane) & 1ull;
+  const uint64_t q = (base + lane) & t.mask;
+  uint32_t f = 0, o = 0;
+  Blob<KS> rk;  // the slot's key
+  if (in) {
+    f = t.flags[q];
+    rk.load(t.keys + q * KS, KS);
+    if (is_mem) o = t.owner[q];
+    keep_loaded<KS>(f, rk);
+  }
+  Blob<KS> mk;  // a member's key, in the lane of its dry slot
+  uint32_t hp = 0, vi = 0;
+  if (is_mem) {
+    mk.load(in_keys + static_cast<uint64_t>(o) * KS, KS);
+    vi = static_cast<uint32_t>(dd_value_index(t, o));
+    hp = static_cast<uint32_t>(((mk.hash(KS, t.mode) & t.mask) - base) & t.mask);  // home lane
+    if (hp > lane) {  // cannot happen: a member's home lies in its arc, at or before its dry slot
+      atomicOr(&ctr->status, kStatusBudget);
+      hp = lane;
+    }
+  }
+  bool occ = in && (f & 0xFu) == kOccupied;
+  uint32_t dist = f >> 4;
+  uint32_t vs = lane;  // value source: the pre-batch value of slot lane, or kVsBatch | member lane
+  bool dirty = false;
+  // members in batch order (bitonic over (batch index, lane); other lanes last)
+  unsigned long long sk = is_mem ? (static_cast<unsigned long long>(o) << 6) | lane : (~0ull << 6) | lane;
+#pragma unroll
+  for (uint32_t size = 2; size <= 64; size <<= 1) {
+#pragma unroll
+    for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+      const unsigned long long other = __shfl_xor(sk, static_cast<int>(stride), 64);
+      const bool keep_min = ((lane & stride) == 0) == ((lane & size) == 0);
+      sk = keep_min ? (other < sk ? other : sk) : (other > sk ? other : sk);
+    }
+  }
+  const uint32_t nmem = static_cast<uint32_t>(__popcll(mem));
+  constexpr uint32_t kCarried = 64u << 7;  // scan key of the pair carried in at the home slot
+  uint32_t claimed = 0;
+  for (uint32_t j = 0; j < nmem; ++j) {
+    const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(sk)), j));
+    const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(sk >> 32)), j));
+    const uint32_t src = lo & 63u;
+    const uint32_t oj = (lo >> 6) | (hi << 26);
+    Blob<KS> ck;  // the member's key, in every lane
+#pragma unroll
+    for (int w = 0; w < KW; ++w)
+      ck.w[w] = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(mk.w[w]), static_cast<int>(src)));
+    const uint32_t h = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(hp), static_cast<int>(src)));
+    // As in replay_arc_lds: with a_k = dist_k - (k - h) per occupied slot k
+    // at or after the home h, eviction points are the strict new minima of
+    // the a's (the carried pair counting as 0 before slot h), the pair
+    // arriving at k is the one holding the running minimum and its position
+    // there is the minimum + (k - h).  The walk ends at the first free slot,
+    // or at an equal key where a_k equals the running minimum.  Keys are
+    // packed (a + 64) << 7 | (lane + 1): the minimum keeps the earlier lane
+    // on ties, the carried pair (index 0) first of all.  Distances above 127
+    // are clipped: such a resident's a is above every minimum that matters.
+    const bool inw = lane >= h;
+    const int ak = static_cast<int>(dist < 127u ? dist : 127u) - static_cast<int>(lane - h);
+    uint32_t pk = (inw && occ) ? (static_cast<uint32_t>(ak + 64) << 7) | (lane + 1) : 0xFFFFFFFFu;
+#pragma unroll
+    for (uint32_t off = 1; off < 64; off <<= 1) {
+      const uint32_t y = static_cast<uint32_t>(__shfl_up(static_cast<int>(pk), off, 64));
+      if (lane >= off && y < pk) pk = y;
+    }
+    uint32_t ex = static_cast<uint32_t>(__shfl_up(static_cast<int>(pk), 1, 64));
+    if (lane <= h || ex > kCarried) ex = kCarried;
+    const int ev = static_cast<int>(ex >> 7) - 64;
+    const int ei = static_cast<int>(ex & 127u) - 1;  // -1: the carried pair
+    const int eis = ei < 0 ? 0 : ei;
+    Blob<KS> hk;  // key of the pair arriving here
+#pragma unroll
+    for (int w = 0; w < KW; ++w) {
+      const uint32_t x = static_cast<uint32_t>(__shfl(static_cast<int>(rk.w[w]), eis, 64));
+      hk.w[w] = ei < 0 ? ck.w[w] : x;
+    }
+    const uint32_t hvs_l = static_cast<uint32_t>(__shfl(static_cast<int>(vs), eis, 64));
+    const uint32_t hvs = ei < 0 ? (kVsBatch | src) : hvs_l;
+    const uint32_t ppos = static_cast<uint32_t>(ev + static_cast<int>(lane - h));
+    bool eq = occ;
+#pragma unroll
+    for (int w = 0; w < KW; ++w) eq = eq && rk.w[w] == hk.w[w];
+    const bool stop = inw && (!occ || (ak == ev && eq));
+    const unsigned long long stops = __ballot(stop);
+    const unsigned long long occs = __ballot(occ);
+    if (!stops) {  // cannot happen: the arc's free slots end every walk
+      if (lane == 0) atomicOr(&ctr->status, kStatusBudget);
+      continue;
+    }
+    const uint32_t E = static_cast<uint32_t>(__ffsll(stops) - 1);
+    if (!((act >> E) & 1ull) && lane == 0) atomicOr(&ctr->status, kStatusBudget);  // cannot happen: left the arc
+    if (lane < E && inw && occ && ak < ev) {  // evictions on the way
+      rk = hk;
+      vs = hvs;
+      dist = ppos;
+      dirty = true;
+    }
+    const bool efree = !((occs >> E) & 1ull);
+    if (lane == E) {
+      if (efree) {  // claim the free slot
+        occ = true;
+        dist = ppos;
+        rk = hk;
+      }
+      vs = hvs;  // claimed, or the equal key's value updated
+      dirty = true;
+    }
+    claimed += efree ? 1u : 0u;
+    if (t.outcome && lane == 0) t.outcome[oj] = efree ? 1 : 0;
+  }
+  // Values: every source read before any write.  Each store waits for the
+  // load that fetched its data, and one load instruction covers every lane,
+  // so no lane overwrites a slot another lane has still to read.
+  const uint32_t vsrc = vs & 63u;
+  const uint32_t vi_src = static_cast<uint32_t>(__shfl(static_cast<int>(vi), static_cast<int>(vsrc), 64));
+  const bool vmove = in && dirty && vs != lane;
+  Blob<VS> v;
+  if (vmove) {
+    if (vs & kVsBatch) v.load(in_vals + static_cast<uint64_t>(vi_src) * VS, VS);
+    else v.load(t.vals + ((base + vsrc) & t.mask) * VS, VS);
+  }
+  if (in && dirty) {
+    t.flags[q] = make_flag(kOccupied, dist);
+    rk.store(t.keys + q * KS, KS);
+    if (vmove) v.store(t.vals + q * VS, VS);
+  }
+  return claimed;
+}
+
+// The first dry-claimed slot at or after start (cyclic) that comes before
+// the next barrier, or kNoSlot when a barrier comes first.
+__device__ inline uint32_t arcs_first_member(const unsigned long long* P, const unsigned long long* Q,
+                                             uint64_t start, uint64_t words) {
+  uint64_t w = start >> 6;
+  unsigned long long m = ~0ull << (start & 63);
+  for (uint64_t it = 0; it <= words; ++it) {
+    const unsigned long long p = P[w], qq = Q[w];
+    const unsigned long long cb = p & ~qq & m, bb = p & qq & m;
+    if (cb && (!bb || __ffsll(cb) < __ffsll(bb))) return static_cast<uint32_t>((w << 6) + __ffsll(cb) - 1);
+    if (bb) return kNoSlot;
+    m = ~0ull;
+    w = w + 1 == words ? 0 : w + 1;
+  }
+  return kNoSlot;
+}
+
+template <int KS, int VS>
+__global__ void __launch_bounds__(1024)
+k_ins_arcs(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __restrict__ in_vals,
+           const unsigned long long* __restrict__ Q, uint32_t* __restrict__ big, Counters* ctr) {
+  static_assert(KS > 0 && VS > 0 && KS % 4 == 0 && VS % 4 == 0, "static sizes");
+  const unsigned long long* P = t.freebits;
+  const uint64_t words = t.cap >> 6;
+  const uint64_t w = (static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;  // wave-uniform
+  uint32_t claimed = 0;
+  if (w < words && !(ctr->status & kStatusDryFail)) {
+    const uint64_t wp = w == 0 ? words - 1 : w - 1, wn = w + 1 == words ? 0 : w + 1;
+    const unsigned long long p0 = P[wp], p1 = P[w], p2 = P[wn];
+    const unsigned long long q0 = Q[wp], q1 = Q[w], q2 = Q[wn];
+    const unsigned long long c1 = p1 & ~q1, c2 = p2 & ~q2;              // dry claims
+    const unsigned long long b0 = p0 & q0, b1 = p1 & q1, b2 = p2 & q2;  // barriers
+    unsigned long long st = ~b1 & ((b1 << 1) | (b0 >> 63));            // arcs starting here
+    unsigned long long act = 0, mem = 0;
+    uint32_t cross = 64, cross_len = 0;
+    while (st) {
+      const uint32_t k = static_cast<uint32_t>(__ffsll(st) - 1);
+      st &= st - 1;
+      const unsigned long long z1 = b1 & (~0ull << k);
+      if (z1) {  // ends inside the word
+        const uint32_t x = static_cast<uint32_t>(__ffsll(z1) - 1);
+        const unsigned long long bits = (1ull << x) - (1ull << k);
+        if (c1 & bits) {
+          act |= bits;
+          mem |= c1 & bits;
+        }
+        continue;
+      }
+      const unsigned long long cw = c1 & (~0ull << k);
+      uint32_t s = kNoSlot;
+      if (b2) {
+        const uint32_t x = static_cast<uint32_t>(__ffsll(b2) - 1);
+        const unsigned long long c2b = c2 & ((1ull << x) - 1ull);
+        if (!(cw | c2b)) continue;  // no member
+        if (64u - k + x <= kArcSlots) {
+          cross = k;
+          cross_len = 64u - k + x;
+          continue;
+        }
+        s = cw ? static_cast<uint32_t>((w << 6) + __ffsll(cw) - 1) : static_cast<uint32_t>((wn << 6) + __ffsll(c2b) - 1);
+      } else if (cw) {
+        s = static_cast<uint32_t>((w << 6) + __ffsll(cw) - 1);
+      } else if (c2) {
+        s = static_cast<uint32_t>((wn << 6) + __ffsll(c2) - 1);
+      } else {
+        s = arcs_first_member(P, Q, (wn + 1 == words ? 0 : wn + 1) << 6, words);
+      }
+      if (s != kNoSlot && (threadIdx.x & 63u) == 0) big[atomicAdd(&ctr->nbig, 1u)] = s;  // longer than kArcSlots
+    }
+    uint32_t got = 0;
+    if (act) got += arcs_round<KS, VS>(t, in_keys, in_vals, w << 6, act, mem, ctr);
+    if (cross < 64) {
+      const unsigned long long a2 = cross_len == 64 ? ~0ull : ((1ull << cross_len) - 1ull);
+      const unsigned long long m2 = (cross == 0 ? c1 : ((c1 >> cross) | (c2 << (64u - cross)))) & a2;
+      got += arcs_round<KS, VS>(t, in_keys, in_vals, (w << 6) + cross, a2, m2, ctr);
+    }
+    if ((threadIdx.x & 63u) == 0) claimed = got;
+  }
+  block_add(&ctr->count, claimed);
+}
+
 // Leaders order and replay closures of up to leader_max (<= kLeaderMax)
 // keys whose arc (barrier to barrier) is at most `coop_arc` slots; other
-// closures are queued for k_ins_big.  With `plan` (sparse batches, where
+// closures are queued for k_ins_big.  With PLAN (sparse batches, where
 // nearly every key is a one-member closure) every lane first reads its own
 // key and the flags of its home .. dry slot together with the election's
 // bitmap words, so a one-member closure is decided and applied with two
-// further rounds of memory traffic (insert_planned).
-template <int KS, int VS>
+// further rounds of memory traffic (insert_planned).  The instance without
+// it (dense batches when k_ins_arcs is not used) holds no plan registers.
+template <int KS, int VS, bool PLAN>
 __global__ void __launch_bounds__(64)
 k_ins_leader(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __restrict__ in_vals, uint64_t n,
              const unsigned long long* __restrict__ Q, const uint32_t* __restrict__ dslot, uint32_t* __restrict__ big,
-             Counters* ctr, uint32_t coop_arc, uint32_t leader_max, bool plan) {
+             Counters* ctr, uint32_t coop_arc, uint32_t leader_max) {
   __shared__ uint32_t lst[kLeaderMax * 64];  // column per lane: lst[j * 64 + lane]
   // planned one-member inserts for static sizes; values of 16-byte multiples
   // are copied by the whole wave, smaller ones by each lane
-  constexpr bool kPlan = KS > 0 && KS % 4 == 0 && VS > 0 && VS % 4 == 0;
+  constexpr bool kPlan = PLAN && KS > 0 && KS % 4 == 0 && VS > 0 && VS % 4 == 0;
   constexpr bool kCoop = kPlan && VS % 16 == 0;
   __shared__ uint64_t mv_src[kCoop ? 64 * kPlanMoves : 1], mv_dst[kCoop ? 64 * kPlanMoves : 1];
   __shared__ u32x4 mv_stage[kCoop ? 64 * (VS / 16) : 1];
@@ -1201,7 +1443,7 @@ k_ins_leader(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __rest
       uint64_t home = 0;
       uint32_t L = ~0u;
       uint32_t fl[kPlanSlots];
-      if (kPlan && plan) {
+      if (kPlan) {
         ck.load(in_keys + i * K, K);
         home = ck.hash(K, t.mode) & t.mask;
         L = static_cast<uint32_t>(min<uint64_t>((s - home) & t.mask, kPlanSlots + 1));
@@ -1212,7 +1454,7 @@ k_ins_leader(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __rest
       // with the flags above); most closures begin, and end, inside that word.
       uint64_t cw = s >> 6;
       unsigned long long cp = P[cw], cq = Q[cw];
-      if (kPlan && plan) {
+      if (kPlan) {
 #pragma unroll
         for (uint32_t j = 0; j < kPlanSlots; ++j) asm volatile("" ::"v"(fl[j]));
       }
@@ -1261,11 +1503,11 @@ k_ins_leader(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __rest
         uint32_t c1 = 0;
         if (m > kLeaderMax || long_arc) {
           big[atomicAdd(&ctr->nbig, 1u)] = static_cast<uint32_t>(s);
-        } else if (kPlan && plan && m == 1 &&
+        } else if (kPlan && m == 1 &&
                    insert_planned<KS, VS>(t, ck, in_vals + dd_value_index(t, i) * V, home, s, L, fl, &c1, mv, &nmv)) {
           claimed += c1;
           if (t.outcome) t.outcome[i] = static_cast<uint8_t>(c1);
-        } else if (m == 1 && !(kPlan && plan)) {  // key i alone, dense batch: walked from its own pair
+        } else if (m == 1 && !(kPlan)) {  // key i alone, dense batch: walked from its own pair
           c1 = insert_one<KS, VS, false>(t, in_keys + i * K, in_vals + dd_value_index(t, i) * V, K, V, ctr);
           claimed += c1;
           if (t.outcome) t.outcome[i] = static_cast<uint8_t>(c1);
