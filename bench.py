@@ -323,8 +323,8 @@ def run_c2(args):
         return (keys_per_launch * n / (ms / 1e3) / 1e6) if ms else None
 
     ins_names = ["ins_bitmap_copy", "k_ins_dry", "k_ins_closure", "ins_radix_sort", "k_ins_replay", "k_ins_leader",
-                 "k_ins_big", "k_ins_bits"]
-    rem_names = ["k_rem_probe", "k_rem_check", "k_rem_apply", "k_rem_runs", "rem_radix_sort", "k_rem_replay"]
+                 "k_ins_arcs", "k_ins_big", "k_ins_bits"]
+    rem_names = ["k_rem_probe", "k_rem_check", "k_rem_apply", "k_rem_replay"]
     get_rate = op_rate(["k_get"], step.n_get, "k_get")
     ins_rate = op_rate(ins_names, batch_n, "k_ins_dry")
     rem_rate = op_rate(rem_names, step.n_rem, "k_rem_probe")

@@ -90,10 +90,8 @@ struct Ops {
   hipError_t (*ins_replay)(const Table&, const uint8_t*, const uint8_t*, uint64_t, const uint32_t*, const uint32_t*,
                            Counters*, uint32_t, hipStream_t);
   hipError_t (*rem_probe)(const Table&, const uint8_t*, uint64_t, unsigned long long*, uint32_t, uint32_t, hipStream_t);
-  hipError_t (*rem_runs)(const Table&, const uint8_t*, uint64_t, uint32_t*, uint32_t*, const Counters*, uint32_t,
-                         hipStream_t);
-  hipError_t (*rem_replay)(const Table&, const uint8_t*, uint64_t, const uint32_t*, const uint32_t*, Counters*, bool,
-                           uint32_t, hipStream_t);
+  hipError_t (*rem_replay)(const Table&, const uint8_t*, uint64_t, unsigned long long*, const uint32_t*,
+                           const uint32_t*, Counters*, bool, uint32_t, uint32_t, Counters*, uint32_t, hipStream_t);
   hipError_t (*ins_fused)(const Table&, const dev::FusedInsert&, uint32_t, hipStream_t);
   hipError_t (*ins_dedup)(const Table&, const uint8_t*, uint64_t, uint32_t, hipStream_t);
   hipError_t (*ins_fused_blocks)(int, uint32_t*);
@@ -194,17 +192,11 @@ hipError_t launch_rem_probe(const Table& t, const uint8_t* q, uint64_t n, unsign
 }
 
 template <int KS>
-hipError_t launch_rem_runs(const Table& t, const uint8_t* q, uint64_t n, uint32_t* cid, uint32_t* idx,
-                           const Counters* gate, uint32_t block, hipStream_t s) {
-  hipLaunchKernelGGL((dev::k_rem_runs<KS>), dim3(grid_for(n, block)), dim3(block), 0, s, t, q, n, cid, idx, gate);
-  return hipGetLastError();
-}
-
-template <int KS>
-hipError_t launch_rem_replay(const Table& t, const uint8_t* q, uint64_t n, const uint32_t* cid, const uint32_t* idx,
-                             Counters* c, bool gated, uint32_t block, hipStream_t s) {
-  hipLaunchKernelGGL((dev::k_rem_replay<KS>), dim3(grid_for(n, block)), dim3(block), 0, s, t, q, n, cid, idx, c,
-                     gated);
+hipError_t launch_rem_replay(const Table& t, const uint8_t* q, uint64_t n, unsigned long long* dirty,
+                             const uint32_t* druns, const uint32_t* nxt, Counters* c, bool gated, uint32_t status,
+                             uint32_t nd, Counters* host_out, uint32_t seq, hipStream_t s) {
+  hipLaunchKernelGGL((dev::k_rem_replay<KS>), dim3(dev::kRemReplayBlocks), dim3(64), 0, s, t, q, n, dirty, druns, nxt,
+                     c, gated, status, nd, host_out, seq);
   return hipGetLastError();
 }
 
@@ -248,7 +240,6 @@ Ops make_ops() {
   else o.ins_arcs = nullptr;
   o.ins_replay = &launch_ins_replay<KS, VS>;
   o.rem_probe = &launch_rem_probe<KS>;
-  o.rem_runs = &launch_rem_runs<KS>;
   o.rem_replay = &launch_rem_replay<KS>;
   o.ins_fused = &launch_ins_fused<KS, VS>;
   o.ins_dedup = &launch_ins_dedup<KS>;
@@ -492,6 +483,7 @@ class GpuBackend final : public Backend {
   void* sort_tmp_ = nullptr;
   size_t sort_tmp_bytes_ = 0;
   unsigned long long* Q_ = nullptr;
+  unsigned long long* dirty_ = nullptr;  // remove conflicts: runs to replay, all zero between ops
   Counters* ctr_ = nullptr;
   dev::GridBarrier* bar_ = nullptr;  // the fused insert's barrier words (zero between launches)
   Counters* h_ctr_ = nullptr;  // pinned
@@ -583,6 +575,7 @@ void GpuBackend::release() {
   if (sort_tmp_) (void)hipFree(sort_tmp_);
   sort_tmp_ = nullptr;
   dfree(Q_);
+  dfree(dirty_);
   dfree(ctr_);
   dfree(bar_);
   dfree(dd_first_);
@@ -659,7 +652,8 @@ fm_status GpuBackend::init(const fm_builder& b, std::string* err) {
       (e = hipMalloc(&t_.emptybits, words * 8)) != hipSuccess ||
       (e = hipMalloc(&t_.owner, cap * sizeof(uint32_t))) != hipSuccess ||
       (e = hipMalloc(&t_.rowner, cap * sizeof(uint32_t))) != hipSuccess ||
-      (e = hipMalloc(&Q_, words * 8)) != hipSuccess ||
+      (e = hipMalloc(&Q_, words * 8)) != hipSuccess || (e = hipMalloc(&dirty_, words * 8)) != hipSuccess ||
+      (e = hipMemset(dirty_, 0, words * 8)) != hipSuccess ||
       (e = hipMalloc(&ctr_, sizeof(Counters))) != hipSuccess ||
       (e = hipMalloc(&bar_, sizeof(dev::GridBarrier))) != hipSuccess ||
       (e = hipMemset(bar_, 0, sizeof(dev::GridBarrier))) != hipSuccess ||
@@ -1076,67 +1070,51 @@ fm_status GpuBackend::run_remove(const uint8_t* d_keys, uint64_t n, uint64_t* re
   }
   {
     Prof p(this, kKidRemCheck);
-    hipLaunchKernelGGL(dev::k_rem_check, dim3(grid_for(n, blk)), dim3(blk), 0, stream_, t_, ev_, n, action_, ctr_);
+    hipLaunchKernelGGL(dev::k_rem_check, dim3(grid_for(n, blk)), dim3(blk), 0, stream_, t_, ev_, n, action_, dirty_,
+                       cid_s_, ctr_);
     if ((st = check_launch(hipGetLastError(), "k_rem_check")) != FM_OK) return st;
   }
+  // A counted remove reads the counters next: k_rem_apply hands them to
+  // h_ctr_ (zeroing ctr_) with the status and the dirty-run count, and the
+  // host polls for the hand-over; after a conflict it launches the replay
+  // of the dirty runs and reads its count the same way.  An asynchronous
+  // remove launches the replay unconditionally, gated on the device (it
+  // exits at once unless k_rem_check flagged a conflict), so it reads
+  // nothing back; its counts come from the outcome flags.
+  const uint32_t seq = async ? 0 : next_ctr_seq();
+  Counters* host = async ? nullptr : h_ctr_;
   {
     Prof p(this, kKidRemApply);
-    // A counted remove reads the counters next: the last workgroup hands
-    // them to h_ctr_ (zeroing ctr_) and the host polls for the hand-over.
-    const uint32_t seq = async ? 0 : next_ctr_seq();
-    hipLaunchKernelGGL(dev::k_rem_apply, dim3(grid_for(n, blk)), dim3(blk), 0, stream_, t, ev_, action_, n, ctr_,
-                       async ? nullptr : h_ctr_, seq);
+    hipLaunchKernelGGL(dev::k_rem_apply, dim3(grid_for(n, blk)), dim3(blk), 0, stream_, t, ev_, action_, n, dirty_,
+                       idx_, ctr_, host, seq);
     if ((st = check_launch(hipGetLastError(), "k_rem_apply")) != FM_OK) return st;
-    if (!async) {
-      ctr_zero_ = ctr_in_host_ = true;
-      poll_seq_ = seq;
-    }
   }
   if (async) {
-    // No status readback: the replay steps are launched unconditionally and
-    // run only if k_rem_check flagged a conflict (k_rem_apply then left the
-    // table untouched).  Removed counts come from the outcome flags.
-    {
-      Prof p(this, kKidRemRuns);
-      if ((st = check_launch(ops_.rem_runs(t_, d_keys, n, cid_, idx_, ctr_, blk, stream_), "k_rem_runs")) != FM_OK)
-        return st;
-    }
-    {
-      Prof p(this, kKidRemSort);
-      // gated like the runs and the replay: every sort kernel exits at once
-      // unless k_rem_check flagged a conflict
-      hipError_t e = sort_pairs(cid_, cid_s_, idx_, idx_s_, n, sort_end_bit_, sort_tmp_, &ctr_->status,
-                                dev::kStatusConflict, stream_);
-      if (e != hipSuccess) return fail(FM_ERR_KERNEL_LAUNCH, hip_msg("radix sort", e));
-    }
     Prof p(this, kKidRemReplay);
-    return check_launch(ops_.rem_replay(t, d_keys, n, cid_s_, idx_s_, ctr_, true, blk, stream_), "k_rem_replay");
+    return check_launch(ops_.rem_replay(t, d_keys, n, dirty_, cid_s_, idx_, ctr_, true, 0, 0, nullptr, 0, stream_),
+                        "k_rem_replay");
   }
+  ctr_zero_ = ctr_in_host_ = true;
+  poll_seq_ = seq;
   Counters c;
   if ((st = read_counters(&c)) != FM_OK) return st;
+  uint64_t count = c.count;
   if (c.status & dev::kStatusConflict) {
-    // Exact replay grouped by EMPTY-delimited runs, from the untouched
-    // pre-batch state.
-    if ((st = zero_counters()) != FM_OK) return st;
-    {
-      Prof p(this, kKidRemRuns);
-      if ((st = check_launch(ops_.rem_runs(t_, d_keys, n, cid_, idx_, nullptr, blk, stream_), "k_rem_runs")) != FM_OK)
-        return st;
-    }
-    {
-      Prof p(this, kKidRemSort);
-      hipError_t e = sort_pairs(cid_, cid_s_, idx_, idx_s_, n, sort_end_bit_, sort_tmp_, nullptr, 0, stream_);
-      if (e != hipSuccess) return fail(FM_ERR_KERNEL_LAUNCH, hip_msg("radix sort", e));
-    }
+    const uint32_t seq2 = next_ctr_seq();
     {
       Prof p(this, kKidRemReplay);
-      if ((st = check_launch(ops_.rem_replay(t, d_keys, n, cid_s_, idx_s_, ctr_, false, blk, stream_), "k_rem_replay")) !=
-          FM_OK)
+      if ((st = check_launch(ops_.rem_replay(t, d_keys, n, dirty_, cid_s_, idx_, ctr_, false, c.status, c.nbig, h_ctr_,
+                                             seq2, stream_),
+                             "k_rem_replay")) != FM_OK)
         return st;
     }
-    if ((st = read_counters(&c)) != FM_OK) return st;
+    ctr_zero_ = ctr_in_host_ = true;
+    poll_seq_ = seq2;
+    Counters c2;
+    if ((st = read_counters(&c2)) != FM_OK) return st;
+    count += c2.count;
   }
-  *removed = c.count;
+  *removed = count;
   return FM_OK;
 }
 

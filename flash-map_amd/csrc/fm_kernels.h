@@ -57,6 +57,7 @@ constexpr uint32_t kStatusConflict = 2u;  // remove batch needs exact replay
 constexpr uint32_t kStatusBudget = 4u;    // an insert walk exceeded cap+2 steps
 constexpr uint32_t kStatusStall = 8u;     // fused insert: a grid barrier timed out
 constexpr uint32_t kStatusBigPending = 16u;  // host copy only: k_ins_big and the settle still to run
+constexpr uint32_t kStatusConflictAll = 32u; // remove conflict with no EMPTY slot: the whole batch is one run
 
 // Event kinds of a probe (get / remove).
 constexpr uint32_t kEvNone = 0u;      // probed all cap slots
@@ -1176,7 +1177,8 @@ __device__ inline void wave_copy_values(const uint64_t* lst_src, const uint64_t*
 // end inside the word are replayed together in one round, the one that
 // runs into the next word in a second.  Longer arcs are queued for
 // k_ins_big by the dry slot of their first member, the entry k_ins_leader
-// queues.  Every changed slot is written once at the end, its value read
+// queues.  The arcs of a round replay side by side, one member each per
+// step.  Every changed slot is written once at the end, its value read
 // from its pre-batch slot or its batch position before any value is
 // written.  Requires cap >= 256 (the window's three bitmap words differ).
 // ---------------------------------------------------------------------------
@@ -1184,17 +1186,30 @@ constexpr uint32_t kArcSlots = 64;
 constexpr uint32_t kVsBatch = 0x100u;  // value source: the batch value of the member in lane (vs & 63)
 
 // One round: lanes are the slots base .. base+63, `act` marks the slots of
-// the arcs replayed and `mem` their dry-claimed slots.  Returns the slots
-// claimed (the same in every lane).
+// the arcs replayed, `starts` their first slots and `mem` their dry-claimed
+// slots.  Every arc replays its j-th member (in batch order) in step j, so
+// a round takes as many steps as its largest arc has members.  Returns the
+// slots claimed (the same in every lane).
 template <int KS, int VS>
 __device__ __forceinline__ uint32_t arcs_round(const Table& t, const uint8_t* __restrict__ in_keys,
                                                const uint8_t* __restrict__ in_vals, uint64_t base,
-                                               unsigned long long act, unsigned long long mem, Counters* ctr) {
+                                               unsigned long long act, unsigned long long starts,
+                                               unsigned long long mem, Counters* ctr) {
   constexpr int KW = KS / 4;
   const uint32_t lane = threadIdx.x & 63u;
   const bool in = (act >> lane) & 1ull;
   const bool is_mem = (mem >> lane) & 1ull;
   const uint64_t q = (base + lane) & t.mask;
+  // this lane's arc: first lane a0, length alen, members mcount, and the
+  // sorted position of its first member (members of earlier arcs sort first)
+  const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+  const unsigned long long sm = starts & upto;
+  const uint32_t a0 = sm ? 63u - static_cast<uint32_t>(__clzll(sm)) : 0u;
+  const unsigned long long rest = ~(act >> a0);
+  const uint32_t alen = rest ? static_cast<uint32_t>(__ffsll(rest) - 1) : 64u - a0;
+  const unsigned long long amask = (alen == 64 ? ~0ull : ((1ull << alen) - 1ull)) << a0;
+  const uint32_t mcount = static_cast<uint32_t>(__popcll(mem & amask));
+  const uint32_t mst = static_cast<uint32_t>(__popcll(mem & ((1ull << a0) - 1ull)));
   uint32_t f = 0, o = 0;
   Blob<KS> rk;  // the slot's key
   if (in) {
@@ -1209,7 +1224,7 @@ __device__ __forceinline__ uint32_t arcs_round(const Table& t, const uint8_t* __
     mk.load(in_keys + static_cast<uint64_t>(o) * KS, KS);
     vi = static_cast<uint32_t>(dd_value_index(t, o));
     hp = static_cast<uint32_t>(((mk.hash(KS, t.mode) & t.mask) - base) & t.mask);  // home lane
-    if (hp > lane) {  // cannot happen: a member's home lies in its arc, at or before its dry slot
+    if (hp > lane || hp < a0) {  // cannot happen: a member's home lies in its arc, at or before its dry slot
       atomicOr(&ctr->status, kStatusBudget);
       hp = lane;
     }
@@ -1218,8 +1233,10 @@ __device__ __forceinline__ uint32_t arcs_round(const Table& t, const uint8_t* __
   uint32_t dist = f >> 4;
   uint32_t vs = lane;  // value source: the pre-batch value of slot lane, or kVsBatch | member lane
   bool dirty = false;
-  // members in batch order (bitonic over (batch index, lane); other lanes last)
-  unsigned long long sk = is_mem ? (static_cast<unsigned long long>(o) << 6) | lane : (~0ull << 6) | lane;
+  // members by (arc, batch index); other lanes last
+  unsigned long long sk = is_mem ? (static_cast<unsigned long long>(a0) << 38) |
+                                       (static_cast<unsigned long long>(o) << 6) | lane
+                                 : (~0ull << 6) | lane;
 #pragma unroll
   for (uint32_t size = 2; size <= 64; size <<= 1) {
 #pragma unroll
@@ -1229,39 +1246,43 @@ __device__ __forceinline__ uint32_t arcs_round(const Table& t, const uint8_t* __
       sk = keep_min ? (other < sk ? other : sk) : (other > sk ? other : sk);
     }
   }
-  const uint32_t nmem = static_cast<uint32_t>(__popcll(mem));
-  constexpr uint32_t kCarried = 64u << 7;  // scan key of the pair carried in at the home slot
+  // As in replay_arc_lds: with a_k = dist_k - (k - h) per occupied slot k
+  // at or after the member's home h, the eviction points are the strict new
+  // minima of the a's (the carried pair counting as 0 before slot h), the
+  // pair arriving at k is the one holding the running minimum and its
+  // position there is the minimum + (k - h).  The walk ends at the first free
+  // slot, or at an equal key where a_k equals the running minimum.  Scan keys
+  // are (63 - a0) << 21 | (a + 64) << 7 | (lane + 1): a lane never takes the
+  // minimum of an earlier arc (larger high bits), the minimum keeps the
+  // earlier lane on ties, and the carried pair (index 0) comes first.
+  // Distances above 127 are clipped: such a resident's a is above every
+  // minimum that matters.
+  constexpr uint32_t kCarried = 64u << 7;
+  const uint32_t hib = (63u - a0) << 21;
   uint32_t claimed = 0;
-  for (uint32_t j = 0; j < nmem; ++j) {
-    const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(sk)), j));
-    const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(sk >> 32)), j));
-    const uint32_t src = lo & 63u;
-    const uint32_t oj = (lo >> 6) | (hi << 26);
-    Blob<KS> ck;  // the member's key, in every lane
+  for (uint32_t j = 0;; ++j) {
+    const bool live = in && j < mcount;  // this lane's arc inserts a member in this step
+    if (!__ballot(live)) break;
+    const int pos = static_cast<int>(live ? mst + j : 0u);
+    const uint32_t slo = static_cast<uint32_t>(__shfl(static_cast<int>(static_cast<uint32_t>(sk)), pos, 64));
+    const uint32_t shi = static_cast<uint32_t>(__shfl(static_cast<int>(static_cast<uint32_t>(sk >> 32)), pos, 64));
+    const int src = static_cast<int>(slo & 63u);
+    const uint32_t oj = (slo >> 6) | (shi << 26);
+    Blob<KS> ck;  // the member's key
 #pragma unroll
-    for (int w = 0; w < KW; ++w)
-      ck.w[w] = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(mk.w[w]), static_cast<int>(src)));
-    const uint32_t h = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(hp), static_cast<int>(src)));
-    // As in replay_arc_lds: with a_k = dist_k - (k - h) per occupied slot k
-    // at or after the home h, eviction points are the strict new minima of
-    // the a's (the carried pair counting as 0 before slot h), the pair
-    // arriving at k is the one holding the running minimum and its position
-    // there is the minimum + (k - h).  The walk ends at the first free slot,
-    // or at an equal key where a_k equals the running minimum.  Keys are
-    // packed (a + 64) << 7 | (lane + 1): the minimum keeps the earlier lane
-    // on ties, the carried pair (index 0) first of all.  Distances above 127
-    // are clipped: such a resident's a is above every minimum that matters.
-    const bool inw = lane >= h;
+    for (int w = 0; w < KW; ++w) ck.w[w] = static_cast<uint32_t>(__shfl(static_cast<int>(mk.w[w]), src, 64));
+    const uint32_t h = static_cast<uint32_t>(__shfl(static_cast<int>(hp), src, 64));
+    const bool inw = live && lane >= h;
     const int ak = static_cast<int>(dist < 127u ? dist : 127u) - static_cast<int>(lane - h);
-    uint32_t pk = (inw && occ) ? (static_cast<uint32_t>(ak + 64) << 7) | (lane + 1) : 0xFFFFFFFFu;
+    uint32_t pk = (inw && occ) ? hib | (static_cast<uint32_t>(ak + 64) << 7) | (lane + 1) : 0xFFFFFFFFu;
 #pragma unroll
     for (uint32_t off = 1; off < 64; off <<= 1) {
       const uint32_t y = static_cast<uint32_t>(__shfl_up(static_cast<int>(pk), off, 64));
       if (lane >= off && y < pk) pk = y;
     }
     uint32_t ex = static_cast<uint32_t>(__shfl_up(static_cast<int>(pk), 1, 64));
-    if (lane <= h || ex > kCarried) ex = kCarried;
-    const int ev = static_cast<int>(ex >> 7) - 64;
+    if (lane <= h || ex > (hib | kCarried)) ex = hib | kCarried;
+    const int ev = static_cast<int>((ex >> 7) & 255u) - 64;
     const int ei = static_cast<int>(ex & 127u) - 1;  // -1: the carried pair
     const int eis = ei < 0 ? 0 : ei;
     Blob<KS> hk;  // key of the pair arriving here
@@ -1271,7 +1292,7 @@ __device__ __forceinline__ uint32_t arcs_round(const Table& t, const uint8_t* __
       hk.w[w] = ei < 0 ? ck.w[w] : x;
     }
     const uint32_t hvs_l = static_cast<uint32_t>(__shfl(static_cast<int>(vs), eis, 64));
-    const uint32_t hvs = ei < 0 ? (kVsBatch | src) : hvs_l;
+    const uint32_t hvs = ei < 0 ? (kVsBatch | static_cast<uint32_t>(src)) : hvs_l;
     const uint32_t ppos = static_cast<uint32_t>(ev + static_cast<int>(lane - h));
     bool eq = occ;
 #pragma unroll
@@ -1279,20 +1300,18 @@ __device__ __forceinline__ uint32_t arcs_round(const Table& t, const uint8_t* __
     const bool stop = inw && (!occ || (ak == ev && eq));
     const unsigned long long stops = __ballot(stop);
     const unsigned long long occs = __ballot(occ);
-    if (!stops) {  // cannot happen: the arc's free slots end every walk
-      if (lane == 0) atomicOr(&ctr->status, kStatusBudget);
-      continue;
-    }
-    const uint32_t E = static_cast<uint32_t>(__ffsll(stops) - 1);
-    if (!((act >> E) & 1ull) && lane == 0) atomicOr(&ctr->status, kStatusBudget);  // cannot happen: left the arc
-    if (lane < E && inw && occ && ak < ev) {  // evictions on the way
+    const unsigned long long mine = live ? (stops & (~0ull << h)) : 0ull;
+    const uint32_t E = mine ? static_cast<uint32_t>(__ffsll(mine) - 1) : 64u;  // this arc's walk ends here
+    const bool eok = live && E < a0 + alen;
+    if (live && !eok && lane == h) atomicOr(&ctr->status, kStatusBudget);  // cannot happen: left the arc
+    if (eok && lane < E && inw && occ && ak < ev) {  // evictions on the way
       rk = hk;
       vs = hvs;
       dist = ppos;
       dirty = true;
     }
-    const bool efree = !((occs >> E) & 1ull);
-    if (lane == E) {
+    const bool efree = eok && !((occs >> E) & 1ull);
+    if (eok && lane == E) {
       if (efree) {  // claim the free slot
         occ = true;
         dist = ppos;
@@ -1300,9 +1319,9 @@ __device__ __forceinline__ uint32_t arcs_round(const Table& t, const uint8_t* __
       }
       vs = hvs;  // claimed, or the equal key's value updated
       dirty = true;
+      if (t.outcome) t.outcome[oj] = efree ? 1 : 0;
     }
-    claimed += efree ? 1u : 0u;
-    if (t.outcome && lane == 0) t.outcome[oj] = efree ? 1 : 0;
+    claimed += static_cast<uint32_t>(__popcll(__ballot(eok && lane == E && efree)));
   }
   // Values: every source read before any write.  Each store waits for the
   // load that fetched its data, and one load instruction covers every lane,
@@ -1356,7 +1375,7 @@ k_ins_arcs(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __restri
     const unsigned long long c1 = p1 & ~q1, c2 = p2 & ~q2;              // dry claims
     const unsigned long long b0 = p0 & q0, b1 = p1 & q1, b2 = p2 & q2;  // barriers
     unsigned long long st = ~b1 & ((b1 << 1) | (b0 >> 63));            // arcs starting here
-    unsigned long long act = 0, mem = 0;
+    unsigned long long act = 0, mem = 0, starts_in = 0;
     uint32_t cross = 64, cross_len = 0;
     while (st) {
       const uint32_t k = static_cast<uint32_t>(__ffsll(st) - 1);
@@ -1368,6 +1387,7 @@ k_ins_arcs(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __restri
         if (c1 & bits) {
           act |= bits;
           mem |= c1 & bits;
+          starts_in |= 1ull << k;
         }
         continue;
       }
@@ -1393,11 +1413,11 @@ k_ins_arcs(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __restri
       if (s != kNoSlot && (threadIdx.x & 63u) == 0) big[atomicAdd(&ctr->nbig, 1u)] = s;  // longer than kArcSlots
     }
     uint32_t got = 0;
-    if (act) got += arcs_round<KS, VS>(t, in_keys, in_vals, w << 6, act, mem, ctr);
+    if (act) got += arcs_round<KS, VS>(t, in_keys, in_vals, w << 6, act, starts_in, mem, ctr);
     if (cross < 64) {
       const unsigned long long a2 = cross_len == 64 ? ~0ull : ((1ull << cross_len) - 1ull);
       const unsigned long long m2 = (cross == 0 ? c1 : ((c1 >> cross) | (c2 << (64u - cross)))) & a2;
-      got += arcs_round<KS, VS>(t, in_keys, in_vals, (w << 6) + cross, a2, m2, ctr);
+      got += arcs_round<KS, VS>(t, in_keys, in_vals, (w << 6) + cross, a2, 1ull, m2, ctr);
     }
     if ((threadIdx.x & 63u) == 0) claimed = got;
   }
@@ -2701,10 +2721,18 @@ k_rem_probe(Table t, const uint8_t* __restrict__ qkeys, uint64_t n, unsigned lon
 
 // Pass 2: conflict detection.  Key i keeps its pre-batch event unless an
 // earlier key tombstones that slot: a duplicate match (owner != i) or an
-// occupied terminator removed by an earlier key (owner < i).
+// occupied terminator removed by an earlier key (owner < i).  Removes
+// neither cross nor create EMPTY slots, so keys interact only inside the
+// run that ends at the first EMPTY slot at/after their home (the first
+// EMPTY at/after their event slot: the probe stops at EMPTY).  A conflict
+// marks its run dirty (bit r of `dirty`, which is all zero between ops);
+// the key that sets the bit lists the run (druns, counted in ctr->nbig) and
+// empties its key list (head owner[r], an insert scratch entry that inserts
+// only read after writing it).  A conflict with no EMPTY slot anywhere makes
+// the whole batch one run (kStatusConflictAll).
 static __global__ void __launch_bounds__(1024)
 k_rem_check(Table t, const unsigned long long* __restrict__ ev, uint64_t n, uint8_t* __restrict__ action,
-            Counters* ctr) {
+            unsigned long long* __restrict__ dirty, uint32_t* __restrict__ druns, Counters* ctr) {
   for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
        i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
     const uint32_t kind = static_cast<uint32_t>(ev[i] >> 32);
@@ -2719,30 +2747,59 @@ k_rem_check(Table t, const unsigned long long* __restrict__ ev, uint64_t n, uint
       conflict = t.rowner[slot] < static_cast<uint32_t>(i);
     }
     action[i] = a;
-    if (conflict) atomicOr(&ctr->status, kStatusConflict);
+    if (conflict) {
+      const uint64_t r = bm_next_set(t.emptybits, slot, t.cap);
+      if (r >= t.cap) {
+        atomicOr(&ctr->status, kStatusConflict | kStatusConflictAll);
+      } else {
+        atomicOr(&ctr->status, kStatusConflict);
+        const unsigned long long bit = 1ull << (r & 63);
+        if (!(atomicOr(&dirty[r >> 6], bit) & bit)) {
+          t.owner[r] = kNoOwner;
+          druns[atomicAdd(&ctr->nbig, 1u)] = static_cast<uint32_t>(r);
+        }
+      }
+    }
   }
 }
 
-// Pass 3: apply (only when the batch had no conflict) and reset rowner[].
-// host_out (optional): the last workgroup hands the final counters to host
-// memory (hand_to_host).  Each workgroup then adds its count and its
-// completion ticket in one atomic on ctr->count (ticket above bit
-// kRemTicketShift), so the last one holds every other's count without a
-// fence.  Without host_out the counters stay on the device (an asynchronous
-// remove's gated replay reads the status after this pass).
+// Pass 3: apply, and reset rowner[].  With no conflict every match that
+// won its slot is tombstoned.  With conflicts, the keys of dirty runs are
+// linked into their run's list (nxt[i] = previous head) and left to
+// k_rem_replay, the others applied here (a clean run's keys interact with
+// no key of another run); with kStatusConflictAll nothing is applied.
+// host_out (optional): the last workgroup hands the counters, status and
+// dirty-run count included, to host memory (hand_to_host).  Each workgroup
+// adds its count and its completion ticket in one atomic on ctr->count
+// (ticket above bit kRemTicketShift), so the last one holds every other's
+// count without a fence.  Without host_out the counters stay on the device
+// for a gated k_rem_replay.
 constexpr int kRemTicketShift = 40;  // counts < 2^40 keys, tickets < 2^24 workgroups
 static __global__ void __launch_bounds__(1024)
 k_rem_apply(Table t, const unsigned long long* __restrict__ ev, const uint8_t* __restrict__ action, uint64_t n,
-            Counters* ctr, Counters* host_out, uint32_t seq) {
-  const bool conflict = (ctr->status & kStatusConflict) != 0;
+            const unsigned long long* __restrict__ dirty, uint32_t* __restrict__ nxt, Counters* ctr,
+            Counters* host_out, uint32_t seq) {
+  const uint32_t status = ctr->status;
+  const bool conflict = (status & kStatusConflict) != 0;
+  const bool all = (status & kStatusConflictAll) != 0;
   unsigned long long total = 0;
   for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * blockDim.x;; base += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
     if (base >= n) break;
     const uint64_t i = base + threadIdx.x;
+    if (i >= n) continue;
+    const uint32_t kind = static_cast<uint32_t>(ev[i] >> 32);
+    const uint64_t slot = ev[i] & 0xFFFFFFFFull;
+    bool deferred = all;
+    if (conflict && !all) {
+      const uint64_t r = kind == kEvTermEmpty ? slot : bm_next_set(t.emptybits, slot, t.cap);
+      if ((dirty[r >> 6] >> (r & 63)) & 1ull) {
+        nxt[i] = atomicExch(&t.owner[r], static_cast<uint32_t>(i));
+        deferred = true;
+      }
+    }
     uint32_t removed = 0;
-    if (i < n && static_cast<uint32_t>(ev[i] >> 32) == kEvMatch) {
-      const uint64_t slot = ev[i] & 0xFFFFFFFFull;
-      if (!conflict && action[i]) {
+    if (kind == kEvMatch) {
+      if (!deferred && action[i]) {
         t.flags[slot] = kTombstone;  // raw 2: distance bits cleared
         atomicOr(&t.freebits[slot >> 6], 1ull << (slot & 63));
         if (t.freecopy) atomicOr(&t.freecopy[slot >> 6], 1ull << (slot & 63));
@@ -2750,7 +2807,7 @@ k_rem_apply(Table t, const unsigned long long* __restrict__ ev, const uint8_t* _
       }
       t.rowner[slot] = kNoOwner;
     }
-    if (t.outcome && i < n && !conflict) t.outcome[i] = static_cast<uint8_t>(removed);
+    if (t.outcome && !deferred) t.outcome[i] = static_cast<uint8_t>(removed);
     total += removed;
   }
   if (!host_out) {
@@ -2762,26 +2819,6 @@ k_rem_apply(Table t, const unsigned long long* __restrict__ ev, const uint8_t* _
     constexpr unsigned long long kMask = (1ull << kRemTicketShift) - 1;
     const unsigned long long old = atomicAdd(&ctr->count, (1ull << kRemTicketShift) + sum);
     if ((old >> kRemTicketShift) == gridDim.x - 1) hand_to_host(ctr, host_out, seq, (old & kMask) + sum);
-  }
-}
-
-// Exact replay, step 1: run id = first EMPTY slot at/after home (removes
-// never cross or create EMPTY slots).  No EMPTY slot: one run.
-// gate (asynchronous removes): when set, the replay steps run only if the
-// check pass found a conflict, so the host need not read the status first.
-template <int KS>
-__global__ void __launch_bounds__(1024)
-k_rem_runs(Table t, const uint8_t* __restrict__ qkeys, uint64_t n, uint32_t* __restrict__ cid,
-           uint32_t* __restrict__ idx, const Counters* gate) {
-  if (gate && !(gate->status & kStatusConflict)) return;
-  const uint32_t K = KS ? KS : t.K;
-  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
-       i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
-    Blob<KS> q;
-    q.load(qkeys + i * K, K);
-    const uint64_t h = q.hash(K, t.mode) & t.mask;
-    cid[i] = static_cast<uint32_t>(bm_next_set(t.emptybits, h, t.cap));
-    idx[i] = static_cast<uint32_t>(i);
   }
 }
 
@@ -2810,27 +2847,80 @@ __device__ inline uint32_t remove_one(const Table& t, const uint8_t* __restrict_
   return 0;
 }
 
-// Exact replay, step 2: one lane per run, keys in batch order.
+// Pass 4 (runs only after a conflict): one lane per dirty run replays the
+// run's keys in batch order from the run's pre-batch state (their list,
+// kRunKeys keys at a time insertion-sorted in LDS), clears the run's dirty
+// bit, and the whole batch in order under kStatusConflictAll.  host_out
+// (optional): the last workgroup hands the counters over as k_rem_apply
+// does.
+constexpr uint32_t kRunKeys = 64;
+constexpr uint32_t kRemReplayBlocks = 64;
+// gated: status and dirty-run count from ctr (left there by k_rem_apply
+// without host_out), the kernel exiting at once without a conflict;
+// otherwise they come as arguments (read back from the hand-over).
 template <int KS>
-__global__ void __launch_bounds__(1024)
-k_rem_replay(Table t, const uint8_t* __restrict__ qkeys, uint64_t n, const uint32_t* __restrict__ cid_s,
-             const uint32_t* __restrict__ idx_s, Counters* ctr, bool gated) {
-  if (gated && !(ctr->status & kStatusConflict)) return;
+__global__ void __launch_bounds__(64)
+k_rem_replay(Table t, const uint8_t* __restrict__ qkeys, uint64_t n, unsigned long long* __restrict__ dirty,
+             const uint32_t* __restrict__ druns, const uint32_t* __restrict__ nxt, Counters* ctr, bool gated,
+             uint32_t status_arg, uint32_t nd_arg, Counters* host_out, uint32_t seq) {
+  __shared__ uint32_t col[kRunKeys * 64];  // column per lane: col[j * 64 + lane]
+  const uint32_t status = gated ? ctr->status : status_arg;
+  if (!(status & kStatusConflict)) return;  // grid-uniform: nothing to replay
   const uint32_t K = KS ? KS : t.K;
+  const uint32_t lane = threadIdx.x;
   unsigned long long removed = 0;
-  for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * blockDim.x;; base += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
-    if (base >= n) break;
-    const uint64_t s = base + threadIdx.x;
-    if (s < n && (s == 0 || cid_s[s] != cid_s[s - 1])) {
-      const uint32_t c = cid_s[s];
-      for (uint64_t u = s; u < n && cid_s[u] == c; ++u) {
-        const uint32_t r = remove_one<KS>(t, qkeys + idx_s[u] * K, K);
+  if (status & kStatusConflictAll) {
+    if (blockIdx.x == 0 && lane == 0) {
+      for (uint64_t i = 0; i < n; ++i) {
+        const uint32_t r = remove_one<KS>(t, qkeys + i * K, K);
         removed += r;
-        if (t.outcome) t.outcome[idx_s[u]] = static_cast<uint8_t>(r);
+        if (t.outcome) t.outcome[i] = static_cast<uint8_t>(r);
       }
     }
+  } else {
+    const uint32_t nd = gated ? ctr->nbig : nd_arg;
+    auto one = [&](uint32_t k) {
+      const uint32_t x = remove_one<KS>(t, qkeys + static_cast<uint64_t>(k) * K, K);
+      removed += x;
+      if (t.outcome) t.outcome[k] = static_cast<uint8_t>(x);
+    };
+    for (uint32_t d = blockIdx.x * 64 + lane; d < nd; d += gridDim.x * 64) {
+      const uint32_t r = druns[d];
+      const uint32_t head = t.owner[r];
+      // kRunKeys keys at a time, in batch order: each pass keeps the
+      // smallest indices above the last one replayed (one pass for most runs)
+      uint32_t after = 0;
+      bool first = true;
+      for (;;) {
+        uint32_t m = 0;
+        for (uint32_t k = head; k != kNoOwner; k = nxt[k]) {
+          if (!first && k <= after) continue;
+          if (m == kRunKeys && k > col[(kRunKeys - 1) * 64 + lane]) continue;
+          uint32_t c = m < kRunKeys ? m++ : kRunKeys - 1;  // insertion sort, the largest dropped when full
+          while (c > 0 && col[(c - 1) * 64 + lane] > k) {
+            col[c * 64 + lane] = col[(c - 1) * 64 + lane];
+            --c;
+          }
+          col[c * 64 + lane] = k;
+        }
+        for (uint32_t j = 0; j < m; ++j) one(col[j * 64 + lane]);
+        if (m < kRunKeys) break;
+        after = col[(kRunKeys - 1) * 64 + lane];
+        first = false;
+      }
+      atomicAnd(&dirty[r >> 6], ~(1ull << (r & 63)));
+    }
   }
-  block_add(&ctr->count, removed);
+  if (!host_out) {
+    block_add(&ctr->count, removed);
+    return;
+  }
+  const unsigned long long sum = block_sum(removed);
+  if (threadIdx.x == 0) {
+    constexpr unsigned long long kMask = (1ull << kRemTicketShift) - 1;
+    const unsigned long long old = atomicAdd(&ctr->count, (1ull << kRemTicketShift) + sum);
+    if ((old >> kRemTicketShift) == gridDim.x - 1) hand_to_host(ctr, host_out, seq, (old & kMask) + sum);
+  }
 }
 
 // ---------------------------------------------------------------------------

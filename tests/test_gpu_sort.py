@@ -1,8 +1,8 @@
-"""The device sort that groups keys for exact replays (csrc/fm_sort.h),
-exercised through its two users and compared with the oracle: the
-sort-grouped insert (a batch that may fill the table, forced here with
-FM_INSERT_SORT_PATH=1) and the conflict replay of remove batches with
-duplicate keys, counted and inside the serving loop (gated on the device).
+"""The device sort that groups keys by closure for the sort-grouped insert
+(csrc/fm_sort.h; a batch that may fill the table, forced here with
+FM_INSERT_SORT_PATH=1), compared with the oracle, and large remove batches
+with duplicate keys (their conflicting runs replayed in batch order),
+counted and inside the serving loop.
 
 Sizes cover the one-workgroup LDS sort (n <= 16384), and radix sorts of one
 pass (ids below 2^11), two passes (even: the first pass writes the scratch
@@ -62,8 +62,8 @@ def test_sort_grouped_insert_large_batches(cap, n, monkeypatch):
 
 @pytest.mark.parametrize("cap,n", [(1 << 10, 20_000), (1 << 16, 12_000), (1 << 16, 60_000), (1 << 20, 200_000)])
 def test_remove_conflict_replay_large(cap, n):
-    # duplicate removes force the exact replay (k_rem_check flags a conflict);
-    # ids in [0, cap]: 1024 slots -> one radix pass over 20k keys
+    # duplicate removes force the exact replay of their runs (k_rem_check
+    # flags a conflict)
     say(f"remove conflict replay cap={cap} n={n}")
     m, o = gpu_make(cap, 8, 8), OracleAdapter(cap, 8, 8)
     rng = np.random.default_rng(n ^ cap)
@@ -73,14 +73,11 @@ def test_remove_conflict_replay_large(cap, n):
     ops += [("rem", rk), ("get", fill)]
     more = rows(rng, cap // 4, 8, cap * 4)
     ops += [("ins", more, rng.integers(0, 256, (len(more), 8), dtype=np.uint8)), ("rem", np.concatenate([more, more]))]
-    m.profile(True)
     apply_and_compare(m, o, ops)
-    assert m.profile_read().get("rem_radix_sort", (0, 0))[0] >= 2  # both remove batches were replayed
 
 
 def test_served_removes_with_duplicates_large():
-    # serving removes run the gated replay with no readback; large duplicate
-    # batches take the multi-pass radix sort on the device
+    # serving removes run the gated replay with no readback
     say("served large duplicate removes")
     cap = 1 << 18
     rng = np.random.default_rng(5)
