@@ -81,7 +81,8 @@ struct Ops {
   hipError_t (*ins_dry)(const Table&, const uint8_t*, uint64_t, unsigned long long*, uint32_t*, uint32_t*, Counters*,
                         uint32_t, hipStream_t);
   hipError_t (*ins_leader)(const Table&, const uint8_t*, const uint8_t*, uint64_t, const unsigned long long*,
-                           const uint32_t*, uint32_t*, Counters*, uint32_t, uint32_t, bool, hipStream_t);
+                           const uint32_t*, uint32_t*, Counters*, uint32_t, uint32_t, bool, Counters*, uint32_t,
+                           hipStream_t);
   // null for the generic sizes
   hipError_t (*ins_arcs)(const Table&, const uint8_t*, const uint8_t*, const unsigned long long*, uint32_t*,
                          Counters*, hipStream_t);
@@ -124,7 +125,10 @@ hipError_t launch_get(const Table& t, const uint8_t* q, uint64_t n, uint8_t* ov,
 template <int KS>
 hipError_t launch_ins_dry(const Table& t, const uint8_t* k, uint64_t n, unsigned long long* Q, uint32_t* home,
                           uint32_t* dslot, Counters* c, uint32_t block, hipStream_t s) {
-  hipLaunchKernelGGL((dev::k_ins_dry<KS>), dim3(grid_for(n, block)), dim3(block), 0, s, t, k, n, Q, home, dslot, c);
+  // with next bitmaps to fill, at most 4 words copied per thread
+  const uint64_t threads = t.nfree ? std::max<uint64_t>(n, ((t.cap + 63) / 64 + 3) / 4) : n;
+  hipLaunchKernelGGL((dev::k_ins_dry<KS>), dim3(grid_for(threads, block)), dim3(block), 0, s, t, k, n, Q, home, dslot,
+                     c);
   return hipGetLastError();
 }
 
@@ -137,14 +141,15 @@ hipError_t launch_ins_dedup(const Table& t, const uint8_t* k, uint64_t n, uint32
 template <int KS, int VS>
 hipError_t launch_ins_leader(const Table& t, const uint8_t* k, const uint8_t* v, uint64_t n,
                              const unsigned long long* Q, const uint32_t* dslot, uint32_t* big, Counters* c,
-                             uint32_t coop_arc, uint32_t leader_max, bool plan, hipStream_t s) {
+                             uint32_t coop_arc, uint32_t leader_max, bool plan, Counters* host_out, uint32_t seq,
+                             hipStream_t s) {
   constexpr bool kCanPlan = KS > 0 && KS % 4 == 0 && VS > 0 && VS % 4 == 0;
   if (kCanPlan && plan)
     hipLaunchKernelGGL((dev::k_ins_leader<KS, VS, kCanPlan>), dim3(grid_for(n, 64)), dim3(64), 0, s, t, k, v, n, Q,
-                       dslot, big, c, coop_arc, leader_max);
+                       dslot, big, c, coop_arc, leader_max, host_out, seq);
   else
     hipLaunchKernelGGL((dev::k_ins_leader<KS, VS, false>), dim3(grid_for(n, 64)), dim3(64), 0, s, t, k, v, n, Q,
-                       dslot, big, c, coop_arc, leader_max);
+                       dslot, big, c, coop_arc, leader_max, host_out, seq);
   return hipGetLastError();
 }
 
@@ -421,7 +426,10 @@ class GpuBackend final : public Backend {
   // fixed: use these buffers (no allocation, no deduplication, no fused launch).
   fm_status run_insert(const uint8_t* d_keys, const uint8_t* d_vals, uint64_t n, uint8_t* outcome = nullptr,
                        bool defer_big = false, const InsertScratch* fixed = nullptr);
-  fm_status launch_bits(const Table& t, uint64_t n, bool dedup, bool defer_big, const uint32_t* dslot);
+  // early_seq != 0: the leader hands the counters over with that sequence
+  // number, and the settle pass only settles (k_ins_bits `early`).
+  fm_status launch_bits(const Table& t, uint64_t n, bool dedup, bool defer_big, const uint32_t* dslot,
+                        uint32_t early_seq = 0);
   fm_status read_insert_counters(Counters* out);
   // async: no status readback and no count (serving reads the outcome
   // flags); the conflict replay is gated on the device.
@@ -447,6 +455,7 @@ class GpuBackend final : public Backend {
   bool tune_sort_only_ = false;  // FM_INSERT_SORT_PATH=1 forces the sort-grouped insert
   uint32_t leader_max_ = dev::kLeaderMax;  // FM_LEADER_MAX: members a leader lane replays (more: k_ins_big)
   bool arcs_on_ = true;                    // FM_INSERT_ARCS=0: dense batches led by k_ins_leader instead
+  bool next_bits_on_ = true;               // FM_INSERT_NEXT_BITS=0: sparse batches settled by k_ins_bits
   enum class InsertPath { kSort, kLed, kFused };
   InsertPath last_insert_ = InsertPath::kSort;
   uint32_t fused_blocks_ = 0;  // k_ins_fused workgroups per launch at most (0: no cooperative launch)
@@ -466,8 +475,17 @@ class GpuBackend final : public Backend {
     bool dedup;
     const uint32_t* dslot;
     uint32_t* big;
+    unsigned long long* Q;  // the batch's claim bitmap
+    bool next_bits;         // claims went to the next bitmaps (Table::nfree): nothing left to settle
   };
   PendingBig pend_{};
+  // The spare bitmap set of inserts that publish their claims into the next
+  // bitmaps (Table::nfree, nempty, nq); swapped with freebits, emptybits and
+  // Q_ after such a batch.  Allocated on first use.
+  unsigned long long* nfree_ = nullptr;
+  unsigned long long* nempty_ = nullptr;
+  unsigned long long* nq_ = nullptr;
+  bool ensure_next_bits();
 
   // scratch (grown geometrically, reused)
   uint64_t scratch_n_ = 0;
@@ -575,6 +593,9 @@ void GpuBackend::release() {
   if (sort_tmp_) (void)hipFree(sort_tmp_);
   sort_tmp_ = nullptr;
   dfree(Q_);
+  dfree(nfree_);
+  dfree(nempty_);
+  dfree(nq_);
   dfree(dirty_);
   dfree(ctr_);
   dfree(bar_);
@@ -668,6 +689,7 @@ fm_status GpuBackend::init(const fm_builder& b, std::string* err) {
   if (const char* env = std::getenv("FM_FUSED_SPIN")) fused_spin_ = static_cast<uint32_t>(std::strtoul(env, nullptr, 10));
   if (const char* env = std::getenv("FM_INSERT_SORT_PATH")) tune_sort_only_ = env[0] == '1';
   if (const char* env = std::getenv("FM_INSERT_ARCS")) arcs_on_ = std::strcmp(env, "0") != 0;
+  if (const char* env = std::getenv("FM_INSERT_NEXT_BITS")) next_bits_on_ = std::strcmp(env, "0") != 0;
   if (const char* env = std::getenv("FM_LEADER_MAX"))
     leader_max_ = static_cast<uint32_t>(std::max(1ul, std::min<unsigned long>(std::strtoul(env, nullptr, 10), dev::kLeaderMax)));
   sort_end_bit_ = 1;
@@ -767,6 +789,19 @@ bool GpuBackend::ensure_host_staging(uint64_t in, uint64_t out) {
   return true;
 }
 
+bool GpuBackend::ensure_next_bits() {
+  if (nfree_) return true;
+  const uint64_t bytes = (t_.cap + 63) / 64 * 8;
+  if (hipMalloc(&nfree_, bytes) == hipSuccess && hipMalloc(&nempty_, bytes) == hipSuccess &&
+      hipMalloc(&nq_, bytes) == hipSuccess)
+    return true;
+  (void)hipGetLastError();  // no spare set: the settle pass publishes instead
+  dfree(nfree_);
+  dfree(nempty_);
+  dfree(nq_);
+  return false;
+}
+
 // Zeroes the counters for an op that reads them back with read_counters.
 fm_status GpuBackend::zero_counters() {
   hipError_t e = ctr_zero_ ? hipSuccess : hipMemsetAsync(ctr_, 0, sizeof(Counters), stream_);
@@ -778,9 +813,11 @@ fm_status GpuBackend::zero_counters() {
 
 // The settle pass of a led insert: publishes the real claims, restores Q_
 // and hands the counters to h_ctr_ (see k_ins_bits).
-fm_status GpuBackend::launch_bits(const Table& t, uint64_t n, bool dedup, bool defer_big, const uint32_t* dslot) {
+fm_status GpuBackend::launch_bits(const Table& t, uint64_t n, bool dedup, bool defer_big, const uint32_t* dslot,
+                                  uint32_t early_seq) {
   Prof p(this, kKidInsBits);
-  const uint32_t seq = next_ctr_seq();
+  const bool early = early_seq != 0;
+  const uint32_t seq = early ? early_seq : next_ctr_seq();
   if (!dedup && n * 16 >= t_.cap) {
     // Most bitmap words change: recompute them all from the flags (one
     // coalesced pass) instead of one atomic per key; at most 1024
@@ -791,7 +828,7 @@ fm_status GpuBackend::launch_bits(const Table& t, uint64_t n, bool dedup, bool d
   } else {
     // 1024-thread workgroups: one completion ticket per 1024 keys
     hipLaunchKernelGGL(dev::k_ins_bits, dim3(grid_for(n, 1024)), dim3(1024), 0, stream_, t, dslot, n, ctr_, Q_,
-                       h_ctr_, seq, defer_big);
+                       early ? nullptr : h_ctr_, seq, defer_big, early);
   }
   fm_status st;
   if ((st = check_launch(hipGetLastError(), "k_ins_bits")) != FM_OK) return st;
@@ -824,11 +861,14 @@ fm_status GpuBackend::read_insert_counters(Counters* out) {
   }
   {
     Prof p(this, kKidInsBig);
-    if ((st = check_launch(ops_.ins_big(pend_.t, pend_.keys, pend_.vals, pend_.n, Q_, pend_.big, ctr_, stream_),
+    if ((st = check_launch(ops_.ins_big(pend_.t, pend_.keys, pend_.vals, pend_.n, pend_.Q, pend_.big, ctr_, stream_),
                            "k_ins_big")) != FM_OK)
       return failed(st);
   }
-  if ((st = launch_bits(pend_.t, pend_.n, pend_.dedup, false, pend_.dslot)) != FM_OK) return failed(st);
+  // the settle and hand-over (only the hand-over when the claims went to
+  // the next bitmaps)
+  if ((st = launch_bits(pend_.t, pend_.next_bits ? 0 : pend_.n, pend_.dedup, false, pend_.dslot)) != FM_OK)
+    return failed(st);
   if ((st = read_counters(out)) != FM_OK) return failed(st);
   ctr_zero_ = !(out->status & dev::kStatusBigPending);
   return FM_OK;
@@ -989,39 +1029,67 @@ fm_status GpuBackend::run_insert(const uint8_t* d_keys, const uint8_t* d_vals, u
   }
   last_insert_ = led ? InsertPath::kLed : InsertPath::kSort;
   if (led) {
+    // Sparse batches: leaders per key, one-member closures planned.  Dense
+    // ones (n * 16 >= cap, most bitmap words hold claims): one wave per
+    // bitmap word replays the arcs of up to 64 slots that start in it.
+    const bool plan = n * 16 < t_.cap;
+    // Only sparse batches (settled per key) defer: a batch that covers a
+    // large share of the table (n * 16 >= cap) nearly always queues big
+    // closures, and deferring would only add a round trip.
+    const bool defer = defer_big && (dedup || n * 16 < t_.cap);
+    const bool arcs = !plan && arcs_on_ && ops_.ins_arcs && t_.cap >= 256;
+    // A deferred batch led by k_ins_leader hands its count over from the
+    // leader's last workgroup.  Its claims then either go to the next
+    // bitmaps as the replay makes them (sparse batches without
+    // deduplication, where copying the bitmaps costs less than a settle
+    // pass and a kernel boundary: at most 24 MiB or 200 B per key of copy
+    // traffic), or are settled by k_ins_bits while the host returns (the
+    // settle reads no batch input).
+    const bool early = defer && !arcs;
+    const uint32_t seq = early ? next_ctr_seq() : 0;
+    const bool next_bits = plan && !dedup && !fixed && next_bits_on_ &&
+                           words * 40 <= std::max<uint64_t>(24ull << 20, n * 200) && ensure_next_bits();
+    if (next_bits) {
+      t.nfree = nfree_;
+      t.nempty = nempty_;
+      t.nq = nq_;
+    }
     {
       Prof p(this, kKidInsDry);
       if ((st = check_launch(ops_.ins_dry(t, d_keys, n, Q_, nullptr, sc.dslot, ctr_, blk, stream_), "k_ins_dry")) !=
           FM_OK)
         return st;
     }
-    // Sparse batches: leaders per key, one-member closures planned.  Dense
-    // ones (n * 16 >= cap, most bitmap words hold claims): one wave per
-    // bitmap word replays the arcs of up to 64 slots that start in it.
-    const bool plan = n * 16 < t_.cap;
-    if (!plan && arcs_on_ && ops_.ins_arcs && t_.cap >= 256) {
+    if (arcs) {
       Prof p(this, kKidInsArcs);
       if ((st = check_launch(ops_.ins_arcs(t, d_keys, d_vals, Q_, sc.big, ctr_, stream_), "k_ins_arcs")) != FM_OK)
         return st;
     } else {
       Prof p(this, kKidInsLeader);
       if ((st = check_launch(ops_.ins_leader(t, d_keys, d_vals, n, Q_, sc.dslot, sc.big, ctr_, tune_.insert_coop_arc,
-                                             leader_max_, plan, stream_),
+                                             leader_max_, plan, early ? h_ctr_ : nullptr, seq, stream_),
                              "k_ins_leader")) != FM_OK)
         return st;
     }
-    // Only sparse batches (settled per key) defer: a batch that covers a
-    // large share of the table (n * 16 >= cap) nearly always queues big
-    // closures, and deferring would only add a round trip.
-    const bool defer = defer_big && (dedup || n * 16 < t_.cap);
+    unsigned long long* const Qb = Q_;  // this batch's claim bitmap
+    if (next_bits) {  // later ops see the next bitmaps (k_ins_big below still reads the batch's)
+      std::swap(t_.freebits, nfree_);
+      std::swap(t_.emptybits, nempty_);
+      std::swap(Q_, nq_);
+    }
     if (defer) {
-      pend_ = PendingBig{t, d_keys, d_vals, n, dedup, sc.dslot, sc.big};
+      pend_ = PendingBig{t, d_keys, d_vals, n, dedup, sc.dslot, sc.big, Qb, next_bits};
     } else {
       Prof p(this, kKidInsBig);
-      if ((st = check_launch(ops_.ins_big(t, d_keys, d_vals, n, Q_, sc.big, ctr_, stream_), "k_ins_big")) != FM_OK)
+      if ((st = check_launch(ops_.ins_big(t, d_keys, d_vals, n, Qb, sc.big, ctr_, stream_), "k_ins_big")) != FM_OK)
         return st;
     }
-    return launch_bits(t, n, dedup, defer, sc.dslot);
+    if (!next_bits) return launch_bits(t, n, dedup, defer, sc.dslot, seq);
+    q_pending_ = true;  // Q_ (the next claim bitmap) == freebits once the status shows no failed dry pass
+    ctr_zero_ = false;  // an uncounted batch leaves its count; a counted one's hand-over may leave the queue
+    ctr_in_host_ = early;
+    poll_seq_ = seq;
+    return FM_OK;
   }
   {
     Prof p(this, kKidInsDry);

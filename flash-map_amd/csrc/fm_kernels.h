@@ -92,7 +92,23 @@ struct Table {
   uint32_t* dd_last;
   uint32_t* dd_bucket;
   uint64_t dd_smask;
+  // Per-op insert, when set: the bitmaps the map holds after this batch.
+  // The dry pass copies freebits into nfree and nq and emptybits into
+  // nempty; every real claim then clears its bit in all three (claim_next)
+  // as the replay makes it, while freebits, emptybits and the claim bitmap
+  // stay as the batch found them for the closure elections.  No settle pass.
+  unsigned long long* nfree;
+  unsigned long long* nempty;
+  unsigned long long* nq;
 };
+
+// A real claim of slot s in the next bitmaps (see Table::nfree).
+__device__ __forceinline__ void claim_next(const Table& t, uint64_t s) {
+  const unsigned long long m = ~(1ull << (s & 63));
+  atomicAnd(&t.nfree[s >> 6], m);
+  atomicAnd(&t.nempty[s >> 6], m);
+  atomicAnd(&t.nq[s >> 6], m);
+}
 
 constexpr uint32_t kDupKey = 0x80000000u;
 constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
@@ -474,6 +490,16 @@ k_ins_dry(Table t, const uint8_t* __restrict__ in_keys, uint64_t n, unsigned lon
           uint32_t* __restrict__ home_out, uint32_t* __restrict__ dslot, Counters* ctr) {
   const uint32_t K = KS ? KS : t.K;
   const uint64_t budget = ((t.cap + 63) >> 6) + n + 2;
+  if (t.nfree) {  // the next bitmaps start as the pre-batch ones
+    const uint64_t words = (t.cap + 63) >> 6;
+    for (uint64_t w = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; w < words;
+         w += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+      const unsigned long long f = t.freebits[w];
+      t.nfree[w] = f;
+      t.nq[w] = f;
+      t.nempty[w] = t.emptybits[w];
+    }
+  }
   for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
        i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
     Blob<KS> k;
@@ -1424,9 +1450,40 @@ k_ins_arcs(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __restri
   block_add(&ctr->count, claimed);
 }
 
+// Completion tickets of the leader's hand-over ride in the high bits of
+// Counters::count (a count stays below 2^40).
+constexpr uint32_t kTicketShift = 40;
+constexpr unsigned long long kTicket = 1ull << kTicketShift;
+
+// The leader's hand-over (last workgroup; every other one has added its
+// count).  No closure queued for k_ins_big: the count is final and goes to
+// the host now; the device counters are zeroed here (zero_final: claims go
+// to the next bitmaps, no settle pass follows) or by the settle pass after
+// the leader (k_ins_bits with `early`).  Closures queued: the host copy
+// carries kStatusBigPending and the device keeps the count (tickets taken
+// out) and the queue for k_ins_big and the hand-over after it.
+// host_out->done takes `seq` with a system-scope release, as hand_to_host.
+__device__ inline void hand_early(Counters* ctr, Counters* host_out, uint32_t seq, unsigned long long count,
+                                  bool zero_final) {
+  volatile Counters* h = host_out;
+  const uint32_t nbig = ld_agent(&ctr->nbig);
+  const uint32_t status = ld_agent(&ctr->status);
+  if (!nbig && zero_final) *ctr = Counters{};
+  else __hip_atomic_store(&ctr->count, count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  h->status = nbig ? (status | kStatusBigPending) : status;
+  h->nbig = nbig;
+  h->count = count;
+  h->nsmall = 0;
+  __hip_atomic_store(&host_out->done, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 // Leaders order and replay closures of up to leader_max (<= kLeaderMax)
 // keys whose arc (barrier to barrier) is at most `coop_arc` slots; other
-// closures are queued for k_ins_big.  With PLAN (sparse batches, where
+// closures are queued for k_ins_big.  With host_out (counted sparse
+// inserts) every workgroup adds its claims and a completion ticket to
+// ctr->count in one atomic, and the last one hands the count to the host
+// (hand_early) before the settle pass runs: the settle reads no batch input,
+// so the host can return while it finishes on the map's stream.  With PLAN (sparse batches, where
 // nearly every key is a one-member closure) every lane first reads its own
 // key and the flags of its home .. dry slot together with the election's
 // bitmap words, so a one-member closure is decided and applied with two
@@ -1436,7 +1493,7 @@ template <int KS, int VS, bool PLAN>
 __global__ void __launch_bounds__(64)
 k_ins_leader(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __restrict__ in_vals, uint64_t n,
              const unsigned long long* __restrict__ Q, const uint32_t* __restrict__ dslot, uint32_t* __restrict__ big,
-             Counters* ctr, uint32_t coop_arc, uint32_t leader_max) {
+             Counters* ctr, uint32_t coop_arc, uint32_t leader_max, Counters* host_out, uint32_t seq) {
   __shared__ uint32_t lst[kLeaderMax * 64];  // column per lane: lst[j * 64 + lane]
   // planned one-member inserts for static sizes; values of 16-byte multiples
   // are copied by the whole wave, smaller ones by each lane
@@ -1449,10 +1506,10 @@ k_ins_leader(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __rest
   const unsigned long long* P = t.freebits;
   const uint64_t words = (t.cap + 63) >> 6;
   uint32_t* my = lst + threadIdx.x;
-  if (ctr->status & kStatusDryFail) return;  // block-uniform
+  const bool failed = (ctr->status & kStatusDryFail) != 0;  // block-uniform
   unsigned long long claimed = 0;
   for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * blockDim.x;; base += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
-    if (base >= n) break;
+    if (failed || base >= n) break;
     const uint64_t i = base + threadIdx.x;
     const uint32_t sd = i < n ? dslot[i] : kNoSlot;
     uint64_t mv[2 * kPlanMoves];
@@ -1526,10 +1583,13 @@ k_ins_leader(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __rest
         } else if (kPlan && m == 1 &&
                    insert_planned<KS, VS>(t, ck, in_vals + dd_value_index(t, i) * V, home, s, L, fl, &c1, mv, &nmv)) {
           claimed += c1;
+          if (c1 && t.nfree) claim_next(t, s);
           if (t.outcome) t.outcome[i] = static_cast<uint8_t>(c1);
         } else if (m == 1 && !(kPlan)) {  // key i alone, dense batch: walked from its own pair
-          c1 = insert_one<KS, VS, false>(t, in_keys + i * K, in_vals + dd_value_index(t, i) * V, K, V, ctr);
+          uint64_t at = 0;
+          c1 = insert_one<KS, VS, false>(t, in_keys + i * K, in_vals + dd_value_index(t, i) * V, K, V, ctr, &at);
           claimed += c1;
+          if (c1 && t.nfree) claim_next(t, at);
           if (t.outcome) t.outcome[i] = static_cast<uint8_t>(c1);
         } else {
           for (uint32_t a = 1; a < m; ++a) {  // insertion sort by batch index
@@ -1543,8 +1603,10 @@ k_ins_leader(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __rest
           }
           for (uint32_t j = 0; j < m; ++j) {
             const uint64_t k = my[j * 64];
-            c1 = insert_one<KS, VS, false>(t, in_keys + k * K, in_vals + dd_value_index(t, k) * V, K, V, ctr);
+            uint64_t at = 0;
+            c1 = insert_one<KS, VS, false>(t, in_keys + k * K, in_vals + dd_value_index(t, k) * V, K, V, ctr, &at);
             claimed += c1;
+            if (c1 && t.nfree) claim_next(t, at);
             if (t.outcome) t.outcome[k] = static_cast<uint8_t>(c1);
           }
         }
@@ -1585,7 +1647,16 @@ k_ins_leader(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __rest
       }
     }
   }
-  block_add(&ctr->count, claimed);
+  if (!host_out) {
+    block_add(&ctr->count, claimed);
+    return;
+  }
+  const unsigned long long sum = block_sum(claimed);
+  if (threadIdx.x == 0) {
+    const unsigned long long old = atomicAdd(&ctr->count, sum + kTicket);
+    if ((old >> kTicketShift) == gridDim.x - 1u)
+      hand_early(ctr, host_out, seq, (old + sum) & (kTicket - 1u), t.nfree != nullptr);
+  }
 }
 
 // Replays one closure with a whole wave (64 active lanes).  The arc runs
@@ -1717,6 +1788,25 @@ __device__ uint32_t replay_arc(const Table& t, const uint8_t* __restrict__ in_ke
       }
       if (back) atomicOr(&Q[w], back);
     }
+  } else if (t.nfree) {  // the arc's real claims, in the next bitmaps
+    __threadfence_block();  // the walks' flag stores, seen by every lane
+    for (uint64_t wi = lane; wi < nwords; wi += 64) {
+      const uint64_t w = (w0 + wi) % words;
+      unsigned long long bits = P[w] & ~load_word(&Q[w]);
+      unsigned long long taken = 0;
+      while (bits) {
+        const int b = __ffsll(bits) - 1;
+        bits &= bits - 1;
+        const uint64_t p = (w << 6) + b;
+        if (((p + t.cap - s) & t.mask) >= arc) continue;
+        if ((t.flags[p] & 0xFu) == kOccupied) taken |= 1ull << b;
+      }
+      if (taken) {
+        atomicAnd(&t.nfree[w], ~taken);
+        atomicAnd(&t.nempty[w], ~taken);
+        atomicAnd(&t.nq[w], ~taken);
+      }
+    }
   }
   return claimed;
 }
@@ -1786,11 +1876,16 @@ __device__ inline void hand_pending(Counters* ctr, Counters* host_out, uint32_t 
 // (hand_to_host).  defer_big: k_ins_big has not run; if the leader queued
 // closures for it, their slots are not final, so the pass settles nothing
 // (deduplication state included) and hands over kStatusBigPending instead.
+// early: the leader has already handed the counters over (hand_early); with
+// closures pending the pass does nothing (k_ins_big and a settle with a
+// hand-over follow), otherwise it settles and its last workgroup zeroes the
+// counters for the next op.
 static __global__ void __launch_bounds__(1024)
 k_ins_bits(Table t, const uint32_t* __restrict__ dslot, uint64_t n, Counters* ctr, unsigned long long* Q,
-           Counters* host_out, uint32_t seq, bool defer_big) {
+           Counters* host_out, uint32_t seq, bool defer_big, bool early) {
   const bool failed = (ctr->status & kStatusDryFail) != 0;
-  const bool pending = defer_big && ld_agent(&ctr->nbig) != 0;  // the same in every workgroup
+  const bool pending = (defer_big || early) && ld_agent(&ctr->nbig) != 0;  // the same in every workgroup
+  if (early && pending) return;
   const uint64_t end = pending ? 0 : n;
   for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < end;
        i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
@@ -1814,10 +1909,11 @@ k_ins_bits(Table t, const uint32_t* __restrict__ dslot, uint64_t n, Counters* ct
   // workgroup's read of ctr->status precedes its ticket.  (An agent-scope
   // fence writes back the L2 on this part, which costs more than the whole
   // kernel; only the last workgroup's hand-over releases, once.)
-  if (!host_out) return;
+  if (!host_out && !early) return;
   __syncthreads();
   if (threadIdx.x == 0 && atomicAdd(&ctr->done, 1u) == gridDim.x - 1) {
-    if (pending) hand_pending(ctr, host_out, seq);
+    if (early) *ctr = Counters{};
+    else if (pending) hand_pending(ctr, host_out, seq);
     else hand_to_host(ctr, host_out, seq, __hip_atomic_load(&ctr->count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
   }
 }

@@ -606,16 +606,21 @@ def test_gpu_max_capacity_wraps_at_2_31():
         fm.FlashMap.builder(cap + 1, 8, 8).require_gpu().build()
 
 
+@pytest.mark.parametrize("next_bits", [True, False])
 @pytest.mark.parametrize("K,V", [(8, 8), (32, 128)])
-def test_gpu_sparse_batches_that_queue_big_closures(K, V):
+def test_gpu_sparse_batches_that_queue_big_closures(K, V, next_bits, monkeypatch):
     """Counted inserts of sparse batches (n * 16 < cap) launch no k_ins_big up
-    front: when the leader queues closures, the settle pass reports them and
-    the host runs k_ins_big and the settle again.  In a near-full table such
-    batches queue closures (batches of updates, with repeated keys, sit
-    between them); in a lightly loaded one a sparse batch queues none and
-    settles in one pass.  The whole slot array is compared with the oracle
-    after every op, and the launch counts show which branch ran."""
-    say(f"sparse batches with queued closures K={K} V={V}")
+    front: when the leader queues closures, its hand-over reports them and
+    the host runs k_ins_big and a hand-over pass.  The claims either go to
+    the next bitmaps as the replay makes them (no settle pass at all when
+    nothing is queued) or, with FM_INSERT_NEXT_BITS=0, are published by the
+    settle pass k_ins_bits (once, or again after k_ins_big).  In a near-full
+    table such batches queue closures (batches of updates, with repeated
+    keys, sit between them); in a lightly loaded one a sparse batch queues
+    none.  The whole slot array is compared with the oracle after every op,
+    and the launch counts show which branch ran."""
+    say(f"sparse batches with queued closures K={K} V={V} next_bits={next_bits}")
+    monkeypatch.setenv("FM_INSERT_NEXT_BITS", "1" if next_bits else "0")  # read when a map is built
     rng = np.random.default_rng(700 + K)
     cap = 1 << 13
     m, o = gpu_make(cap, K, V), OracleAdapter(cap, K, V)
@@ -636,13 +641,18 @@ def test_gpu_sparse_batches_that_queue_big_closures(K, V):
     prof = m.profile_read()
     sparse = sum(1 for op in ops[1:] if op[0] == "ins")
     settles = prof.get("k_ins_bits", (0, 0))[0]
-    assert prof.get("k_ins_big", (0, 0))[0] >= 1, prof  # some batch queued closures: deferred, then replayed
-    assert sparse < settles <= 2 * sparse, (sparse, settles)  # ... and settled a second time
+    bigs = prof.get("k_ins_big", (0, 0))[0]
+    assert bigs >= 1, prof  # some batch queued closures: deferred, then replayed
+    if next_bits:
+        assert settles == bigs, (bigs, settles)  # one hand-over pass after each replay, no settle
+    else:
+        assert sparse < settles <= 2 * sparse, (sparse, settles)  # ... and settled a second time
     # the other branch: a sparse batch into a lightly loaded table queues
-    # nothing and settles in one pass
+    # nothing and settles in one pass (or none)
     m2, o2 = gpu_make(cap, K, V), OracleAdapter(cap, K, V)
     apply_and_compare(m2, o2, [("ins", keys[:2000], vals[:2000])])
     m2.profile(True)
     apply_and_compare(m2, o2, [("ins", keys[2000:2100], vals[2000:2100])])
     prof2 = m2.profile_read()
-    assert prof2.get("k_ins_bits", (0, 0))[0] == 1 and prof2.get("k_ins_big", (0, 0))[0] == 0, prof2
+    assert prof2.get("k_ins_bits", (0, 0))[0] == (0 if next_bits else 1), prof2
+    assert prof2.get("k_ins_big", (0, 0))[0] == 0, prof2
