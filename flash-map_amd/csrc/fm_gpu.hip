@@ -87,7 +87,7 @@ struct Ops {
   hipError_t (*ins_arcs)(const Table&, const uint8_t*, const uint8_t*, const unsigned long long*, uint32_t*,
                          Counters*, hipStream_t);
   hipError_t (*ins_big)(const Table&, const uint8_t*, const uint8_t*, uint64_t, const unsigned long long*,
-                        const uint32_t*, Counters*, hipStream_t);
+                        const uint32_t*, Counters*, Counters*, uint32_t, hipStream_t);
   hipError_t (*ins_replay)(const Table&, const uint8_t*, const uint8_t*, uint64_t, const uint32_t*, const uint32_t*,
                            Counters*, uint32_t, hipStream_t);
   hipError_t (*rem_probe)(const Table&, const uint8_t*, uint64_t, unsigned long long*, uint32_t, uint32_t, hipStream_t);
@@ -164,8 +164,8 @@ hipError_t launch_ins_arcs(const Table& t, const uint8_t* k, const uint8_t* v, c
 // One wave per queued closure; enough waves to cover ~4 per SIMD.
 template <int KS, int VS>
 hipError_t launch_ins_big(const Table& t, const uint8_t* k, const uint8_t* v, uint64_t n, const unsigned long long* Q,
-                          const uint32_t* big, Counters* c, hipStream_t s) {
-  hipLaunchKernelGGL((dev::k_ins_big<KS, VS>), dim3(1024), dim3(64), 0, s, t, k, v, n, Q, big, c);
+                          const uint32_t* big, Counters* c, Counters* host_out, uint32_t seq, hipStream_t s) {
+  hipLaunchKernelGGL((dev::k_ins_big<KS, VS>), dim3(1024), dim3(64), 0, s, t, k, v, n, Q, big, c, host_out, seq);
   return hipGetLastError();
 }
 
@@ -485,6 +485,10 @@ class GpuBackend final : public Backend {
   unsigned long long* nfree_ = nullptr;
   unsigned long long* nempty_ = nullptr;
   unsigned long long* nq_ = nullptr;
+  // Each set is one block of three words-long arrays (free, empty, claim),
+  // so a kernel reaches the other two from the first (Table::nfree).
+  unsigned long long* bits_a_ = nullptr;  // the set allocated with the map
+  unsigned long long* bits_b_ = nullptr;  // the spare set
   bool ensure_next_bits();
 
   // scratch (grown geometrically, reused)
@@ -577,8 +581,8 @@ void GpuBackend::release() {
   dfree(t_.keys);
   dfree(t_.vals);
   dfree(t_.flags);
-  dfree(t_.freebits);
-  dfree(t_.emptybits);
+  dfree(bits_a_);
+  t_.freebits = t_.emptybits = nullptr;
   dfree(t_.owner);
   dfree(t_.rowner);
   dfree(home_);
@@ -592,10 +596,8 @@ void GpuBackend::release() {
   dfree(action_);
   if (sort_tmp_) (void)hipFree(sort_tmp_);
   sort_tmp_ = nullptr;
-  dfree(Q_);
-  dfree(nfree_);
-  dfree(nempty_);
-  dfree(nq_);
+  dfree(bits_b_);
+  nfree_ = nempty_ = nq_ = Q_ = nullptr;
   dfree(dirty_);
   dfree(ctr_);
   dfree(bar_);
@@ -669,11 +671,10 @@ fm_status GpuBackend::init(const fm_builder& b, std::string* err) {
   if ((e = hipMalloc(&t_.keys, cap * t_.K + 16)) != hipSuccess ||
       (e = hipMalloc(&t_.vals, cap * t_.V + 16)) != hipSuccess ||
       (e = hipMalloc(&t_.flags, cap * sizeof(uint32_t))) != hipSuccess ||
-      (e = hipMalloc(&t_.freebits, words * 8)) != hipSuccess ||
-      (e = hipMalloc(&t_.emptybits, words * 8)) != hipSuccess ||
+      (e = hipMalloc(&bits_a_, 3 * words * 8)) != hipSuccess ||
       (e = hipMalloc(&t_.owner, cap * sizeof(uint32_t))) != hipSuccess ||
       (e = hipMalloc(&t_.rowner, cap * sizeof(uint32_t))) != hipSuccess ||
-      (e = hipMalloc(&Q_, words * 8)) != hipSuccess || (e = hipMalloc(&dirty_, words * 8)) != hipSuccess ||
+      (e = hipMalloc(&dirty_, words * 8)) != hipSuccess ||
       (e = hipMemset(dirty_, 0, words * 8)) != hipSuccess ||
       (e = hipMalloc(&ctr_, sizeof(Counters))) != hipSuccess ||
       (e = hipMalloc(&bar_, sizeof(dev::GridBarrier))) != hipSuccess ||
@@ -682,6 +683,9 @@ fm_status GpuBackend::init(const fm_builder& b, std::string* err) {
     *err = hip_msg("hipMalloc (table)", e);
     return FM_ERR_GPU_ALLOC;
   }
+  t_.freebits = bits_a_;
+  t_.emptybits = bits_a_ + words;
+  Q_ = bits_a_ + 2 * words;
   std::memset(h_ctr_, 0, sizeof(Counters));  // done == 0 matches no sequence number
   scratch_err_ = scratch_setup(dev_);  // checked by the insert that allocates nothing
   ops_ = select_ops(t_.K, t_.V);
@@ -694,7 +698,8 @@ fm_status GpuBackend::init(const fm_builder& b, std::string* err) {
     leader_max_ = static_cast<uint32_t>(std::max(1ul, std::min<unsigned long>(std::strtoul(env, nullptr, 10), dev::kLeaderMax)));
   sort_end_bit_ = 1;
   while ((1ull << sort_end_bit_) <= cap) ++sort_end_bit_;  // ids in [0, cap]
-  hipLaunchKernelGGL(dev::k_clear, dim3(grid_for(cap, 256)), dim3(256), 0, stream_, t_);
+  hipLaunchKernelGGL(dev::k_clear, dim3(grid_for(cap, 256)), dim3(256), 0, stream_, t_, Q_);
+  q_synced_ = true;  // k_clear filled Q_ like freebits
   hipLaunchKernelGGL(dev::k_fill_u32, dim3(grid_for(cap, 256)), dim3(256), 0, stream_, t_.owner, cap, dev::kNoOwner);
   hipLaunchKernelGGL(dev::k_fill_u32, dim3(grid_for(cap, 256)), dim3(256), 0, stream_, t_.rowner, cap, dev::kNoOwner);
   if ((e = hipGetLastError()) != hipSuccess || (e = hipStreamSynchronize(stream_)) != hipSuccess) {
@@ -791,15 +796,16 @@ bool GpuBackend::ensure_host_staging(uint64_t in, uint64_t out) {
 
 bool GpuBackend::ensure_next_bits() {
   if (nfree_) return true;
-  const uint64_t bytes = (t_.cap + 63) / 64 * 8;
-  if (hipMalloc(&nfree_, bytes) == hipSuccess && hipMalloc(&nempty_, bytes) == hipSuccess &&
-      hipMalloc(&nq_, bytes) == hipSuccess)
-    return true;
-  (void)hipGetLastError();  // no spare set: the settle pass publishes instead
-  dfree(nfree_);
-  dfree(nempty_);
-  dfree(nq_);
-  return false;
+  const uint64_t words = (t_.cap + 63) / 64;
+  if (hipMalloc(&bits_b_, 3 * words * 8) != hipSuccess) {
+    (void)hipGetLastError();  // no spare set: the settle pass publishes instead
+    bits_b_ = nullptr;
+    return false;
+  }
+  nfree_ = bits_b_;
+  nempty_ = bits_b_ + words;
+  nq_ = bits_b_ + 2 * words;
+  return true;
 }
 
 // Zeroes the counters for an op that reads them back with read_counters.
@@ -861,13 +867,19 @@ fm_status GpuBackend::read_insert_counters(Counters* out) {
   }
   {
     Prof p(this, kKidInsBig);
-    if ((st = check_launch(ops_.ins_big(pend_.t, pend_.keys, pend_.vals, pend_.n, pend_.Q, pend_.big, ctr_, stream_),
+    // claims in the next bitmaps: k_ins_big hands the counters over itself
+    const uint32_t seq = pend_.next_bits ? next_ctr_seq() : 0;
+    if ((st = check_launch(ops_.ins_big(pend_.t, pend_.keys, pend_.vals, pend_.n, pend_.Q, pend_.big, ctr_,
+                                        pend_.next_bits ? h_ctr_ : nullptr, seq, stream_),
                            "k_ins_big")) != FM_OK)
       return failed(st);
+    if (pend_.next_bits) {
+      poll_seq_ = seq;
+      ctr_in_host_ = true;
+    }
   }
-  // the settle and hand-over (only the hand-over when the claims went to
-  // the next bitmaps)
-  if ((st = launch_bits(pend_.t, pend_.next_bits ? 0 : pend_.n, pend_.dedup, false, pend_.dslot)) != FM_OK)
+  // otherwise the settle pass and its hand-over
+  if (!pend_.next_bits && (st = launch_bits(pend_.t, pend_.n, pend_.dedup, false, pend_.dslot)) != FM_OK)
     return failed(st);
   if ((st = read_counters(out)) != FM_OK) return failed(st);
   ctr_zero_ = !(out->status & dev::kStatusBigPending);
@@ -1047,7 +1059,7 @@ fm_status GpuBackend::run_insert(const uint8_t* d_keys, const uint8_t* d_vals, u
     // settle reads no batch input).
     const bool early = defer && !arcs;
     const uint32_t seq = early ? next_ctr_seq() : 0;
-    const bool next_bits = plan && !dedup && !fixed && next_bits_on_ &&
+    const bool next_bits = (plan || arcs) && !dedup && !fixed && next_bits_on_ &&
                            words * 40 <= std::max<uint64_t>(24ull << 20, n * 200) && ensure_next_bits();
     if (next_bits) {
       t.nfree = nfree_;
@@ -1077,18 +1089,24 @@ fm_status GpuBackend::run_insert(const uint8_t* d_keys, const uint8_t* d_vals, u
       std::swap(t_.emptybits, nempty_);
       std::swap(Q_, nq_);
     }
+    // A counted dense batch with its claims in the next bitmaps: k_ins_big
+    // (launched even with no closure queued) hands the counters over.
+    const bool big_hands = next_bits && defer_big && !defer;
+    const uint32_t big_seq = big_hands ? next_ctr_seq() : 0;
     if (defer) {
       pend_ = PendingBig{t, d_keys, d_vals, n, dedup, sc.dslot, sc.big, Qb, next_bits};
     } else {
       Prof p(this, kKidInsBig);
-      if ((st = check_launch(ops_.ins_big(t, d_keys, d_vals, n, Qb, sc.big, ctr_, stream_), "k_ins_big")) != FM_OK)
+      if ((st = check_launch(ops_.ins_big(t, d_keys, d_vals, n, Qb, sc.big, ctr_, big_hands ? h_ctr_ : nullptr,
+                                          big_seq, stream_),
+                             "k_ins_big")) != FM_OK)
         return st;
     }
     if (!next_bits) return launch_bits(t, n, dedup, defer, sc.dslot, seq);
     q_pending_ = true;  // Q_ (the next claim bitmap) == freebits once the status shows no failed dry pass
     ctr_zero_ = false;  // an uncounted batch leaves its count; a counted one's hand-over may leave the queue
-    ctr_in_host_ = early;
-    poll_seq_ = seq;
+    ctr_in_host_ = early || big_hands;
+    poll_seq_ = early ? seq : big_seq;
     return FM_OK;
   }
   {
@@ -1357,8 +1375,9 @@ fm_status GpuBackend::clear() {
   fm_status st;
   if ((st = use_device()) != FM_OK) return st;
   q_synced_ = false;
-  hipLaunchKernelGGL(dev::k_clear, dim3(grid_for(t_.cap, 256)), dim3(256), 0, stream_, t_);
+  hipLaunchKernelGGL(dev::k_clear, dim3(grid_for(t_.cap, 256)), dim3(256), 0, stream_, t_, Q_);
   if ((st = check_launch(hipGetLastError(), "k_clear")) != FM_OK) return st;
+  q_synced_ = true;  // Q_ filled like freebits
   hipError_t e = hipStreamSynchronize(stream_);
   if (e != hipSuccess) return fail(FM_ERR_KERNEL_LAUNCH, hip_msg("clear", e));
   len_ = 0;

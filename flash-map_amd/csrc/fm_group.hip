@@ -839,7 +839,7 @@ fm_status GroupImpl::init(uint32_t n, const uint64_t* caps, uint32_t K, uint32_t
     t.K = K;
     t.V = V;
     t.mode = mode_;
-    hipLaunchKernelGGL(dev::k_clear, dim3(gridn(t.cap, 256)), dim3(256), 0, stream_, t);
+    hipLaunchKernelGGL(dev::k_clear, dim3(gridn(t.cap, 256)), dim3(256), 0, stream_, t, nullptr);
   }
   hipLaunchKernelGGL(dev::k_fill_u32, dim3(gridn(total_slots_, 256)), dim3(256), 0, stream_, owner_all_, total_slots_,
                      dev::kNoOwner);

@@ -97,6 +97,7 @@ struct Table {
   // nempty; every real claim then clears its bit in all three (claim_next)
   // as the replay makes it, while freebits, emptybits and the claim bitmap
   // stay as the batch found them for the closure elections.  No settle pass.
+  // The three are one block: nempty == nfree + words, nq == nfree + 2 words.
   unsigned long long* nfree;
   unsigned long long* nempty;
   unsigned long long* nq;
@@ -422,6 +423,26 @@ __device__ inline unsigned long long block_sum(unsigned long long v) {
 __device__ inline void block_add(unsigned long long* dst, unsigned long long v) {
   const unsigned long long sum = block_sum(v);
   if (threadIdx.x == 0 && sum) atomicAdd(dst, sum);
+}
+
+// A workgroup barrier over LDS only: the waves' global stores and atomics
+// stay in flight (__syncthreads waits for every one of them first).
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+// block_add for a kernel whose waves end with global stores or atomics in
+// flight: the reduction waits for LDS only.
+__device__ inline void block_add_nowait(unsigned long long* dst, unsigned long long v) {
+  __shared__ unsigned long long part[16];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  lds_barrier();  // part[] may still be read by a previous call
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+  lds_barrier();
+  if (threadIdx.x == 0) {
+    unsigned long long sum = 0;
+    for (uint32_t w = 0; w < (blockDim.x + 63) / 64; ++w) sum += part[w];
+    if (sum) atomicAdd(dst, sum);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1220,7 +1241,8 @@ template <int KS, int VS>
 __device__ __forceinline__ uint32_t arcs_round(const Table& t, const uint8_t* __restrict__ in_keys,
                                                const uint8_t* __restrict__ in_vals, uint64_t base,
                                                unsigned long long act, unsigned long long starts,
-                                               unsigned long long mem, Counters* ctr) {
+                                               unsigned long long mem, Counters* ctr,
+                                               unsigned long long& claims) {
   constexpr int KW = KS / 4;
   const uint32_t lane = threadIdx.x & 63u;
   const bool in = (act >> lane) & 1ull;
@@ -1286,6 +1308,7 @@ __device__ __forceinline__ uint32_t arcs_round(const Table& t, const uint8_t* __
   constexpr uint32_t kCarried = 64u << 7;
   const uint32_t hib = (63u - a0) << 21;
   uint32_t claimed = 0;
+  claims = 0;  // the round's claims (free slots taken), lane l = slot base + l
   for (uint32_t j = 0;; ++j) {
     const bool live = in && j < mcount;  // this lane's arc inserts a member in this step
     if (!__ballot(live)) break;
@@ -1347,7 +1370,9 @@ __device__ __forceinline__ uint32_t arcs_round(const Table& t, const uint8_t* __
       dirty = true;
       if (t.outcome) t.outcome[oj] = efree ? 1 : 0;
     }
-    claimed += static_cast<uint32_t>(__popcll(__ballot(eok && lane == E && efree)));
+    const unsigned long long took = __ballot(eok && lane == E && efree);
+    claimed += static_cast<uint32_t>(__popcll(took));
+    claims |= took;
   }
   // Values: every source read before any write.  Each store waits for the
   // load that fetched its data, and one load instruction covers every lane,
@@ -1368,6 +1393,30 @@ __device__ __forceinline__ uint32_t arcs_round(const Table& t, const uint8_t* __
   return claimed;
 }
 
+// Clears claim bits in the next bitmaps: mask m of the 64 slots from slot
+// b (b may start inside a word; cap is a multiple of 64).  The three next
+// bitmaps are one block (nempty = nfree + words, nq = nfree + 2 * words), so
+// only t.nfree is read: the dense kernel's SGPR budget decides whether two
+// of its 1024-thread workgroups fit on a CU.
+__device__ inline void claim_next_mask(const Table& t, uint64_t b, unsigned long long m) {
+  if (!m) return;
+  const uint64_t words = t.cap >> 6;
+  const uint32_t sh = static_cast<uint32_t>(b & 63u);
+  const uint64_t w0 = b >> 6, w1 = w0 + 1 == words ? 0 : w0 + 1;
+  const unsigned long long lo = m << sh, hi = sh ? m >> (64u - sh) : 0ull;
+  unsigned long long* nb = t.nfree;
+  if (lo) {
+    atomicAnd(&nb[w0], ~lo);
+    atomicAnd(&nb[words + w0], ~lo);
+    atomicAnd(&nb[2 * words + w0], ~lo);
+  }
+  if (hi) {
+    atomicAnd(&nb[w1], ~hi);
+    atomicAnd(&nb[words + w1], ~hi);
+    atomicAnd(&nb[2 * words + w1], ~hi);
+  }
+}
+
 // The first dry-claimed slot at or after start (cyclic) that comes before
 // the next barrier, or kNoSlot when a barrier comes first.
 __device__ inline uint32_t arcs_first_member(const unsigned long long* P, const unsigned long long* Q,
@@ -1385,8 +1434,11 @@ __device__ inline uint32_t arcs_first_member(const unsigned long long* P, const 
   return kNoSlot;
 }
 
+// At most 72 SGPRs (78 with VCC and the other reserved ones): two of its
+// 1024-thread workgroups then fit on a CU (measured on MI355X: 20 waves per
+// CU at 78 SGPRs, 11 at 88, and the kernel 81 against 124 us on 2^20 slots).
 template <int KS, int VS>
-__global__ void __launch_bounds__(1024)
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_num_sgpr(72), amdgpu_waves_per_eu(8)))
 k_ins_arcs(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __restrict__ in_vals,
            const unsigned long long* __restrict__ Q, uint32_t* __restrict__ big, Counters* ctr) {
   static_assert(KS > 0 && VS > 0 && KS % 4 == 0 && VS % 4 == 0, "static sizes");
@@ -1439,15 +1491,38 @@ k_ins_arcs(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __restri
       if (s != kNoSlot && (threadIdx.x & 63u) == 0) big[atomicAdd(&ctr->nbig, 1u)] = s;  // longer than kArcSlots
     }
     uint32_t got = 0;
-    if (act) got += arcs_round<KS, VS>(t, in_keys, in_vals, w << 6, act, starts_in, mem, ctr);
+    // claims in the next bitmaps go out after both rounds: a store or atomic
+    // still in flight holds up the second round's loads (one counter)
+    unsigned long long cl1 = 0, cl2 = 0;
+    const bool nb = t.nfree != nullptr;
+    if (act) got += arcs_round<KS, VS>(t, in_keys, in_vals, w << 6, act, starts_in, mem, ctr, cl1);
     if (cross < 64) {
       const unsigned long long a2 = cross_len == 64 ? ~0ull : ((1ull << cross_len) - 1ull);
       const unsigned long long m2 = (cross == 0 ? c1 : ((c1 >> cross) | (c2 << (64u - cross)))) & a2;
-      got += arcs_round<KS, VS>(t, in_keys, in_vals, (w << 6) + cross, a2, 1ull, m2, ctr);
+      got += arcs_round<KS, VS>(t, in_keys, in_vals, (w << 6) + cross, a2, 1ull, m2, ctr, cl2);
+    }
+    if (nb && (threadIdx.x & 63u) == 0) {
+      claim_next_mask(t, w << 6, cl1);
+      claim_next_mask(t, (w << 6) + cross, cl2);
     }
     if ((threadIdx.x & 63u) == 0) claimed = got;
   }
-  block_add(&ctr->count, claimed);
+  block_add_nowait(&ctr->count, claimed);
+}
+
+// The last workgroup of an insert writes the op's final counters to
+// host_out (pinned host memory) and zeroes them for the next op.  host_out->
+// done then takes the op's sequence number with a system-scope release, so a
+// host that polls it reads the other fields complete without waiting for the
+// kernel to end.  The release writes back this XCD's L2 once per op.
+__device__ inline void hand_to_host(Counters* ctr, Counters* host_out, uint32_t seq, unsigned long long count) {
+  volatile Counters* h = host_out;
+  h->status = ld_agent(&ctr->status);
+  h->nbig = ld_agent(&ctr->nbig);
+  h->count = count;
+  h->nsmall = 0;
+  *ctr = Counters{};
+  __hip_atomic_store(&host_out->done, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 // Completion tickets of the leader's hand-over ride in the high bits of
@@ -1811,11 +1886,15 @@ __device__ uint32_t replay_arc(const Table& t, const uint8_t* __restrict__ in_ke
   return claimed;
 }
 
-// One single-wave workgroup per queued closure.
+// One single-wave workgroup per queued closure.  With host_out (counted
+// inserts whose claims go to the next bitmaps) every workgroup takes a
+// completion ticket on ctr->count once it is done, and the last one hands
+// the counters to the host (hand_to_host).
 template <int KS, int VS>
 __global__ void __launch_bounds__(64)
 k_ins_big(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __restrict__ in_vals, uint64_t n,
-          const unsigned long long* __restrict__ Q, const uint32_t* __restrict__ big, Counters* ctr) {
+          const unsigned long long* __restrict__ Q, const uint32_t* __restrict__ big, Counters* ctr,
+          Counters* host_out, uint32_t seq) {
   __shared__ uint32_t lst[kBigMax];
   __shared__ uint32_t carry[2 * kMaxGenericSize / 4];
   __shared__ uint32_t cnt;
@@ -1824,9 +1903,9 @@ k_ins_big(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __restric
   const uint32_t nbig = __hip_atomic_load(&ctr->nbig, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   // Closures are handed out one at a time (ctr->nsmall is the ticket; their
   // replay times differ by orders of magnitude).  Workgroups beyond the
-  // queue's length leave at once (the others take every ticket below it).
-  if (blockIdx.x >= nbig) return;
-  for (;;) {
+  // queue's length take none (the others take every ticket below it).
+  const uint32_t parts = min(nbig, gridDim.x);  // the workgroups that take a closure
+  for (bool more = blockIdx.x < parts; more;) {
     if (threadIdx.x == 0) cur = atomicAdd(&ctr->nsmall, 1u);
     __syncthreads();
     const uint32_t c = cur;
@@ -1836,21 +1915,18 @@ k_ins_big(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __restric
                                                        big[c], lst, kBigMax, carry, &cnt, ctr, arc);
     if (threadIdx.x == 0 && claimed) atomicAdd(&ctr->count, static_cast<unsigned long long>(claimed));
   }
-}
-
-// The last workgroup of an insert writes the op's final counters to
-// host_out (pinned host memory) and zeroes them for the next op.  host_out->
-// done then takes the op's sequence number with a system-scope release, so a
-// host that polls it reads the other fields complete without waiting for the
-// kernel to end.  The release writes back this XCD's L2 once per op.
-__device__ inline void hand_to_host(Counters* ctr, Counters* host_out, uint32_t seq, unsigned long long count) {
-  volatile Counters* h = host_out;
-  h->status = ld_agent(&ctr->status);
-  h->nbig = ld_agent(&ctr->nbig);
-  h->count = count;
-  h->nsmall = 0;
-  *ctr = Counters{};
-  __hip_atomic_store(&host_out->done, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  // Tickets from the workgroups that took closures only (tickets on one
+  // word serialise); a workgroup's count adds precede its ticket on that
+  // word.  With no closure queued, workgroup 0 hands over at once.
+  if (host_out && threadIdx.x == 0) {
+    if (parts == 0) {
+      if (blockIdx.x == 0)
+        hand_to_host(ctr, host_out, seq, __hip_atomic_load(&ctr->count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    } else if (blockIdx.x < parts) {
+      const unsigned long long old = atomicAdd(&ctr->count, kTicket);
+      if ((old >> kTicketShift) == parts - 1u) hand_to_host(ctr, host_out, seq, old & (kTicket - 1u));
+    }
+  }
 }
 
 // The hand-over of a settle pass that found closures queued for k_ins_big
@@ -3029,7 +3105,7 @@ static __global__ void k_fill_u32(uint32_t* __restrict__ p, uint64_t n, uint32_t
 }
 
 // clear (kernels.cu:319-326): flags EMPTY; both bitmaps all-free.
-static __global__ void k_clear(Table t) {
+static __global__ void k_clear(Table t, unsigned long long* q) {  // q (optional): a claim bitmap to keep equal to freebits
   const uint64_t words = (t.cap + 63) >> 6;
   const unsigned long long full = t.cap >= 64 ? ~0ull : ((1ull << t.cap) - 1ull);
   for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < t.cap;
@@ -3038,6 +3114,7 @@ static __global__ void k_clear(Table t) {
     if (i < words) {
       t.freebits[i] = full;
       t.emptybits[i] = full;
+      if (q) q[i] = full;
     }
   }
 }

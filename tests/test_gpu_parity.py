@@ -611,10 +611,10 @@ def test_gpu_max_capacity_wraps_at_2_31():
 def test_gpu_sparse_batches_that_queue_big_closures(K, V, next_bits, monkeypatch):
     """Counted inserts of sparse batches (n * 16 < cap) launch no k_ins_big up
     front: when the leader queues closures, its hand-over reports them and
-    the host runs k_ins_big and a hand-over pass.  The claims either go to
-    the next bitmaps as the replay makes them (no settle pass at all when
-    nothing is queued) or, with FM_INSERT_NEXT_BITS=0, are published by the
-    settle pass k_ins_bits (once, or again after k_ins_big).  In a near-full
+    the host runs k_ins_big.  The claims either go to the next bitmaps as
+    the replay makes them (no settle pass at all: the leader or k_ins_big
+    hands the count over) or, with FM_INSERT_NEXT_BITS=0, are published by
+    the settle pass k_ins_bits (once, or again after k_ins_big).  In a near-full
     table such batches queue closures (batches of updates, with repeated
     keys, sit between them); in a lightly loaded one a sparse batch queues
     none.  The whole slot array is compared with the oracle after every op,
@@ -644,7 +644,7 @@ def test_gpu_sparse_batches_that_queue_big_closures(K, V, next_bits, monkeypatch
     bigs = prof.get("k_ins_big", (0, 0))[0]
     assert bigs >= 1, prof  # some batch queued closures: deferred, then replayed
     if next_bits:
-        assert settles == bigs, (bigs, settles)  # one hand-over pass after each replay, no settle
+        assert settles == 0, (bigs, settles)  # k_ins_big hands the count over after each replay
     else:
         assert sparse < settles <= 2 * sparse, (sparse, settles)  # ... and settled a second time
     # the other branch: a sparse batch into a lightly loaded table queues
