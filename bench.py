@@ -192,6 +192,9 @@ def parse_args(argv=None):
                          "host: host-buffer calls on 1M [u8;32]->[u8;128] pairs")
     ap.add_argument("--tables", type=int, default=512, help="c4: number of tables")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
+    ap.add_argument("--get-shape", default="",
+                    help="G,B: get launch shape (group width, block size) instead of the autotuner's "
+                         "(c1, c2, c3; counter runs use it to profile the shape the timed runs choose)")
     ap.add_argument("--requests", type=int, default=1000, help="c5: requests per offered load")
     ap.add_argument("--rates", default="2000,10000,50000", help="c5: offered loads, requests/s")
     ap.add_argument("--detail", default="", help="all: also write every configuration's full line to this file")
@@ -257,13 +260,14 @@ def run_c2(args):
     # on or indexes by a value, so each step does the same work; the checked
     # step above ran on a fully restored table.
     maps = [A, B]
-    get_tune = None
-    if not args.no_autotune:
+    get_tune = pinned_get_shape(args)
+    if get_tune is None and not args.no_autotune:
         # get launch shape (group width x block) for this table and query mix
         get_tune = snap.autotune_get(step.get_keys, step.n_get, reps=5)
+        log(f"get autotune: {get_tune}")
+    if get_tune is not None:
         for m in maps:
             m.set_tuning(get_group=get_tune["get_group"], get_block=get_tune["get_block"])
-        log(f"get autotune: {get_tune}")
     A.copy_from(snap, structure_only=True)
     B.copy_from(snap, structure_only=True)
     for i in range(args.warmup):
@@ -471,6 +475,14 @@ def line_rem(K):
     return 64 * 2 + K
 
 
+def pinned_get_shape(args):
+    """The --get-shape G,B launch shape as an autotune-style record, or None."""
+    if not args.get_shape:
+        return None
+    g, b = (int(x) for x in args.get_shape.split(","))
+    return {"get_group": g, "get_block": b, "pinned": True}
+
+
 def load_pmc(name):
     path = os.path.join(ROOT, "profiles", name)
     if os.path.exists(path):
@@ -503,15 +515,16 @@ def run_c1(args):
            "get": lambda: m.bulk_get_device(q, nq, out, found),
            "remove": lambda: m.bulk_remove_device(rk, nr),
            "get2": lambda: m.bulk_get_device(q, nq, out, found)}
-    get_tune = None
-    if not args.no_autotune:
+    get_tune = pinned_get_shape(args)
+    if get_tune is None and not args.no_autotune:
         # get launch shape for this table and query mix, tuned on the
         # populated table (gets change nothing); every step then clears it
         m.clear()
         m.bulk_insert_device(ik, iv, ni)
         get_tune = m.autotune_get(q, nq, reps=5)
-        m.set_tuning(get_group=get_tune["get_group"], get_block=get_tune["get_block"])
         log(f"c1: get autotune {get_tune}")
+    if get_tune is not None:
+        m.set_tuning(get_group=get_tune["get_group"], get_block=get_tune["get_block"])
     dev = {k: [] for k in names}
     wall = {k: [] for k in names}
     kern = {k: {} for k in names}
@@ -669,9 +682,11 @@ def run_c3(args):
         r, d, w, ks = timed_op(m, lambda: m.bulk_insert_device(dk, dv, nl))
         assert r == nl, f"load inserted {r} of {nl}"
         if i == 0 and not args.no_autotune:
-            get_tune = m.autotune_get(dp, min(npb, 10_000_000), reps=3)
+            get_tune = pinned_get_shape(args)
+            if get_tune is None:
+                get_tune = m.autotune_get(dp, min(npb, 10_000_000), reps=3)
+                log(f"c3: get autotune {get_tune}")
             m.set_tuning(get_group=get_tune["get_group"], get_block=get_tune["get_block"])
-            log(f"c3: get autotune {get_tune}")
         _, d2, w2, ks2 = timed_op(m, lambda: m.bulk_get_device(dp, npb, out, found))
         if i == 0 and not args.no_check:
             # Property check: every inserted probe returns the row id its key
