@@ -317,6 +317,11 @@ fm_status fm_profile_enable(fm_map* m, int enable) {
   return m->be->profile_enable(enable != 0);
 }
 
+fm_status fm_profile_only(fm_map* m, const char* kernel_name) {
+  FM_LOCK(m);
+  return m->be->profile_only(kernel_name ? kernel_name : "");
+}
+
 fm_status fm_profile_read(fm_map* m, fm_kernel_stat* out, uint32_t max, uint32_t* count) {
   FM_LOCK(m);
   if (!out || !count) return bad_arg("fm_profile_read: null argument");

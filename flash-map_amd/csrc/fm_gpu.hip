@@ -347,6 +347,7 @@ class GpuBackend final : public Backend {
   }
   void serve_settle_remove(uint64_t removed) override { len_ -= std::min(len_, removed); }
   fm_status profile_enable(bool on) override;
+  fm_status profile_only(const char* name) override;
   fm_status profile_read(fm_kernel_stat* out, uint32_t max, uint32_t* count) override;
   fm_status get_tuning(fm_tuning* out) const override {
     *out = tune_;
@@ -524,10 +525,10 @@ class GpuBackend final : public Backend {
     GpuBackend* g;
     int kid;
     Prof(GpuBackend* g_, int k) : g(g_), kid(k) {
-      if (g->prof_ && !g->noalloc_) g->prof_mark(kid, true);
+      if (g->timed(kid)) g->prof_mark(kid, true);
     }
     ~Prof() {
-      if (g->prof_ && !g->noalloc_) g->prof_mark(kid, false);
+      if (g->timed(kid)) g->prof_mark(kid, false);
     }
   };
   static const char* kid_name(int k);
@@ -535,6 +536,8 @@ class GpuBackend final : public Backend {
   void prof_mark(int kid, bool begin);
   void harvest();
   bool prof_ = false;
+  int prof_only_ = -1;    // >= 0: time this kernel id only (fm_profile_only)
+  bool timed(int kid) const { return prof_ && !noalloc_ && (prof_only_ < 0 || prof_only_ == kid); }
   bool noalloc_ = false;  // inside the insert that allocates nothing: no timing events either
   hipError_t scratch_err_ = hipSuccess;  // hipGetSymbolAddress of `scratch` at map creation
   hipEvent_t prof_open_ = nullptr;
@@ -1253,6 +1256,19 @@ void GpuBackend::harvest() {
     ev_pool_.push_back(r.b);
   }
   pending_.clear();
+}
+
+fm_status GpuBackend::profile_only(const char* name) {
+  if (!name[0]) {
+    prof_only_ = -1;
+    return FM_OK;
+  }
+  for (int k = 0; k < kNumKid; ++k)
+    if (std::strcmp(kid_name(k), name) == 0) {
+      prof_only_ = k;
+      return FM_OK;
+    }
+  return fail(FM_ERR_INVALID_ARGUMENT, std::string("fm_profile_only: no kernel named ") + name);
 }
 
 fm_status GpuBackend::profile_enable(bool on) {

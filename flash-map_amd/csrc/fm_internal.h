@@ -69,6 +69,7 @@ class Backend {
   virtual fm_status serve_remove(const void*, uint64_t, uint8_t*) { return gpu_required(); }
   virtual void serve_settle_remove(uint64_t) {}
   virtual fm_status profile_enable(bool) { return gpu_required(); }
+  virtual fm_status profile_only(const char*) { return gpu_required(); }
   virtual fm_status profile_read(fm_kernel_stat*, uint32_t, uint32_t*) { return gpu_required(); }
   virtual fm_status get_tuning(fm_tuning*) const { return gpu_required(); }
   virtual fm_status set_tuning(const fm_tuning*) { return gpu_required(); }

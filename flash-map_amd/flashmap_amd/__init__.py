@@ -202,6 +202,7 @@ def lib() -> ctypes.CDLL:
                                                  ctypes.POINTER(GroupTuning), ctypes.POINTER(ctypes.c_double),
                                                  ctypes.POINTER(ctypes.c_uint32)]),
             "fm_profile_enable": (ctypes.c_int, [vp, ctypes.c_int]),
+            "fm_profile_only": (ctypes.c_int, [vp, ctypes.c_char_p]),
             "fm_profile_read": (ctypes.c_int, [vp, ctypes.POINTER(KernelStat), u32,
                                                ctypes.POINTER(ctypes.c_uint32)]),
             "fm_get_tuning": (ctypes.c_int, [vp, ctypes.POINTER(Tuning)]),
@@ -455,6 +456,11 @@ class FlashMap:
     def profile(self, enable: bool = True):
         """Turns per-kernel event timing on/off (resets the totals)."""
         _check(lib().fm_profile_enable(self._h, 1 if enable else 0))
+
+    def profile_only(self, kernel: str = ""):
+        """Limits event timing to one kernel's launches ("" : every kernel);
+        takes effect with the next profile()."""
+        _check(lib().fm_profile_only(self._h, kernel.encode()))
 
     def profile_read(self) -> dict:
         """{kernel name: (launches, total device ms)} since profile()."""
