@@ -458,8 +458,8 @@ class FlashMap:
         _check(lib().fm_profile_enable(self._h, 1 if enable else 0))
 
     def profile_only(self, kernel: str = ""):
-        """Limits event timing to one kernel's launches ("" : every kernel);
-        takes effect with the next profile()."""
+        """Limits event timing to one kernel's launches ("" : every kernel)
+        from this call on."""
         _check(lib().fm_profile_only(self._h, kernel.encode()))
 
     def profile_read(self) -> dict:

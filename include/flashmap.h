@@ -221,8 +221,8 @@ fm_status fm_profile_read(fm_map* m, fm_kernel_stat* out, uint32_t max,
 /* Limits the timing events to the launches of one kernel (its name as
  * fm_profile_read reports it; NULL or "" times every kernel again), so a
  * timed loop pays for two event records per launch of that kernel only.
- * Unknown names are FM_ERR_INVALID_ARGUMENT.  Takes effect with the next
- * fm_profile_enable. */
+ * Unknown names are FM_ERR_INVALID_ARGUMENT.  Applies to the launches
+ * made after the call. */
 fm_status fm_profile_only(fm_map* m, const char* kernel_name);
 
 /* ---- launch tuning (no reference counterpart) --------------------------

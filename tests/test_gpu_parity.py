@@ -656,3 +656,30 @@ def test_gpu_sparse_batches_that_queue_big_closures(K, V, next_bits, monkeypatch
     prof2 = m2.profile_read()
     assert prof2.get("k_ins_bits", (0, 0))[0] == (0 if next_bits else 1), prof2
     assert prof2.get("k_ins_big", (0, 0))[0] == 0, prof2
+
+
+def test_profile_only_times_one_kernel():
+    """fm_profile_only limits the timing events to one kernel's launches (the
+    benchmark's timed steps), "" times every kernel again, and an unknown
+    name is rejected; results are unaffected."""
+    say("profile_only")
+    rng = np.random.default_rng(77)
+    cap, K, V = 1 << 14, 32, 128
+    m, o = gpu_make(cap, K, V), OracleAdapter(cap, K, V)
+    keys = rng.integers(0, 256, (3000, K), dtype=np.uint8)
+    vals = rng.integers(0, 256, (3000, V), dtype=np.uint8)
+    apply_and_compare(m, o, [("ins", keys[:2000], vals[:2000])])
+    m.profile_only("k_ins_leader")
+    m.profile(True)
+    apply_and_compare(m, o, [("ins", keys[2000:2600], vals[2000:2600]), ("get", keys), ("rem", keys[:300])])
+    prof = m.profile_read()
+    assert set(prof) == {"k_ins_leader"} and prof["k_ins_leader"][0] == 1, prof
+    m.profile_only("")
+    m.profile(True)
+    apply_and_compare(m, o, [("ins", keys[2600:], vals[2600:]), ("get", keys)])
+    prof = m.profile_read()
+    assert {"k_ins_dry", "k_ins_leader", "k_get"} <= set(prof), prof
+    with pytest.raises(fm.FlashMapError):
+        m.profile_only("no_such_kernel")
+    m.profile(False)
+
