@@ -93,9 +93,7 @@ struct Ops {
   hipError_t (*rem_probe)(const Table&, const uint8_t*, uint64_t, unsigned long long*, uint32_t, uint32_t, hipStream_t);
   hipError_t (*rem_replay)(const Table&, const uint8_t*, uint64_t, unsigned long long*, const uint32_t*,
                            const uint32_t*, Counters*, bool, uint32_t, uint32_t, Counters*, uint32_t, hipStream_t);
-  hipError_t (*ins_fused)(const Table&, const dev::FusedInsert&, uint32_t, hipStream_t);
   hipError_t (*ins_dedup)(const Table&, const uint8_t*, uint64_t, uint32_t, hipStream_t);
-  hipError_t (*ins_fused_blocks)(int, uint32_t*);
 };
 
 inline uint32_t grid_for(uint64_t threads, uint32_t block) {
@@ -205,35 +203,6 @@ hipError_t launch_rem_replay(const Table& t, const uint8_t* q, uint64_t n, unsig
   return hipGetLastError();
 }
 
-// Cooperative launch: every workgroup is resident, so the kernel's grid-wide
-// barriers are safe; `max_blocks` is the co-resident limit.
-template <int KS, int VS>
-hipError_t launch_ins_fused(const Table& t, const dev::FusedInsert& a, uint32_t max_blocks, hipStream_t s) {
-  const uint64_t want = (a.n + dev::kFusedBlock - 1) / dev::kFusedBlock;
-  const uint32_t grid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(want, max_blocks)));
-  Table tt = t;
-  dev::FusedInsert aa = a;
-  void* args[] = {&tt, &aa};
-  return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&dev::k_ins_fused<KS, VS>), dim3(grid),
-                                    dim3(dev::kFusedBlock), args, 0, s);
-}
-
-template <int KS, int VS>
-hipError_t ins_fused_blocks(int device, uint32_t* out) {
-  int per_cu = 0, cus = 0, coop = 0;
-  *out = 0;
-  hipError_t e = hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, device);
-  if (e != hipSuccess || !coop) return e;
-  if ((e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&dev::k_ins_fused<KS, VS>),
-                                                        dev::kFusedBlock, 0)) != hipSuccess ||
-      (e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device)) != hipSuccess)
-    return e;
-  // Two per CU at most: far inside the LDS, wave and register limits, so the
-  // grid is resident even where the limits are not met exactly.
-  *out = static_cast<uint32_t>(std::min(std::max(0, per_cu), 2) * std::max(0, cus));
-  return hipSuccess;
-}
-
 template <int KS, int VS>
 Ops make_ops() {
   Ops o;
@@ -246,9 +215,7 @@ Ops make_ops() {
   o.ins_replay = &launch_ins_replay<KS, VS>;
   o.rem_probe = &launch_rem_probe<KS>;
   o.rem_replay = &launch_rem_replay<KS>;
-  o.ins_fused = &launch_ins_fused<KS, VS>;
   o.ins_dedup = &launch_ins_dedup<KS>;
-  o.ins_fused_blocks = &ins_fused_blocks<KS, VS>;
   return o;
 }
 
@@ -382,17 +349,6 @@ class GpuBackend final : public Backend {
       return fail(FM_ERR_KERNEL_LAUNCH, "insert: closures still queued for the big-closure replay");
     if (c.status & dev::kStatusBudget)
       return fail(FM_ERR_KERNEL_LAUNCH, "insert walk found no free slot in a full table (uncounted inserts overfilled it)");
-    if (c.status & dev::kStatusStall) {
-      // A grid barrier of the fused insert timed out: no workgroup reached the
-      // phase that writes the table.  Restore owner[] and Q, and use the
-      // multi-kernel path from now on.
-      q_synced_ = false;
-      fused_blocks_ = 0;
-      hipLaunchKernelGGL(dev::k_fill_u32, dim3(grid_for(t_.cap, 256)), dim3(256), 0, stream_, t_.owner, t_.cap,
-                         dev::kNoOwner);
-      (void)hipStreamSynchronize(stream_);
-      return fail(FM_ERR_KERNEL_LAUNCH, "fused insert: workgroups were not all resident; batch not applied");
-    }
     if ((c.status & dev::kStatusDryFail) && last_insert_ != InsertPath::kSort) {
       // Unreachable while len never undercounts occupied slots; the leader
       // kernels skipped the batch, so only owner[] (and Q) need restoring.
@@ -424,7 +380,7 @@ class GpuBackend final : public Backend {
     uint32_t* idx_s;
     void* sort_tmp;
   };
-  // fixed: use these buffers (no allocation, no deduplication, no fused launch).
+  // fixed: use these buffers (no allocation, no deduplication).
   fm_status run_insert(const uint8_t* d_keys, const uint8_t* d_vals, uint64_t n, uint8_t* outcome = nullptr,
                        bool defer_big = false, const InsertScratch* fixed = nullptr);
   // early_seq != 0: the leader hands the counters over with that sequence
@@ -457,12 +413,10 @@ class GpuBackend final : public Backend {
   uint32_t leader_max_ = dev::kLeaderMax;  // FM_LEADER_MAX: members a leader lane replays (more: k_ins_big)
   bool arcs_on_ = true;                    // FM_INSERT_ARCS=0: dense batches led by k_ins_leader instead
   bool next_bits_on_ = true;               // FM_INSERT_NEXT_BITS=0: sparse batches settled by k_ins_bits
-  enum class InsertPath { kSort, kLed, kFused };
+  enum class InsertPath { kSort, kLed };
   InsertPath last_insert_ = InsertPath::kSort;
-  uint32_t fused_blocks_ = 0;  // k_ins_fused workgroups per launch at most (0: no cooperative launch)
-  uint32_t fused_spin_ = 1u << 24;  // grid barrier polls (FM_FUSED_SPIN), ~tens of seconds
   // Stream-ordered facts about device state, kept by the host:
-  bool q_synced_ = false;      // Q_ == freebits (the fused insert keeps it so)
+  bool q_synced_ = false;      // Q_ == freebits
   bool q_pending_ = false;     // the last led insert restored Q_; valid once its status is clean
   bool ctr_zero_ = false;      // ctr_ holds zeros
   bool ctr_in_host_ = false;   // the last op left its final counters in h_ctr_
@@ -508,14 +462,13 @@ class GpuBackend final : public Backend {
   unsigned long long* Q_ = nullptr;
   unsigned long long* dirty_ = nullptr;  // remove conflicts: runs to replay, all zero between ops
   Counters* ctr_ = nullptr;
-  dev::GridBarrier* bar_ = nullptr;  // the fused insert's barrier words (zero between launches)
   Counters* h_ctr_ = nullptr;  // pinned
 
   // per-kernel event timing
   enum {
     kKidGet, kKidInsCopy, kKidInsDry, kKidInsClosure, kKidInsSort, kKidInsReplay, kKidRemProbe, kKidRemCheck,
     kKidRemApply, kKidRemRuns, kKidRemSort, kKidRemReplay, kKidCopyTable, kKidInsLeader, kKidInsBig, kKidInsBits,
-    kKidInsFused, kKidInsDedup, kKidInsArcs, kNumKid
+    kKidInsDedup, kKidInsArcs, kNumKid
   };
   struct PendingTiming {
     int kid;
@@ -603,7 +556,6 @@ void GpuBackend::release() {
   nfree_ = nempty_ = nq_ = Q_ = nullptr;
   dfree(dirty_);
   dfree(ctr_);
-  dfree(bar_);
   dfree(dd_first_);
   dfree(dd_last_);
   dfree(dd_bucket_);
@@ -680,8 +632,6 @@ fm_status GpuBackend::init(const fm_builder& b, std::string* err) {
       (e = hipMalloc(&dirty_, words * 8)) != hipSuccess ||
       (e = hipMemset(dirty_, 0, words * 8)) != hipSuccess ||
       (e = hipMalloc(&ctr_, sizeof(Counters))) != hipSuccess ||
-      (e = hipMalloc(&bar_, sizeof(dev::GridBarrier))) != hipSuccess ||
-      (e = hipMemset(bar_, 0, sizeof(dev::GridBarrier))) != hipSuccess ||
       (e = hipHostMalloc(&h_ctr_, sizeof(Counters), hipHostMallocDefault)) != hipSuccess) {
     *err = hip_msg("hipMalloc (table)", e);
     return FM_ERR_GPU_ALLOC;
@@ -692,8 +642,6 @@ fm_status GpuBackend::init(const fm_builder& b, std::string* err) {
   std::memset(h_ctr_, 0, sizeof(Counters));  // done == 0 matches no sequence number
   scratch_err_ = scratch_setup(dev_);  // checked by the insert that allocates nothing
   ops_ = select_ops(t_.K, t_.V);
-  if (ops_.ins_fused_blocks(dev_, &fused_blocks_) != hipSuccess) fused_blocks_ = 0;  // multi-kernel path instead
-  if (const char* env = std::getenv("FM_FUSED_SPIN")) fused_spin_ = static_cast<uint32_t>(std::strtoul(env, nullptr, 10));
   if (const char* env = std::getenv("FM_INSERT_SORT_PATH")) tune_sort_only_ = env[0] == '1';
   if (const char* env = std::getenv("FM_INSERT_ARCS")) arcs_on_ = std::strcmp(env, "0") != 0;
   if (const char* env = std::getenv("FM_INSERT_NEXT_BITS")) next_bits_on_ = std::strcmp(env, "0") != 0;
@@ -717,10 +665,10 @@ fm_status GpuBackend::set_tuning(const fm_tuning* t) {
   auto pow2 = [](uint32_t g) { return g >= 1 && g <= 16 && (g & (g - 1)) == 0; };
   auto blk = [](uint32_t b) { return b >= 64 && b <= 1024 && b % 64 == 0; };
   if (!t || !pow2(t->get_group) || !pow2(t->remove_group) || !blk(t->get_block) || !blk(t->insert_block) ||
-      t->insert_fused > 1 || t->insert_dedup > 1)
+      t->insert_fused != 0 || t->insert_dedup > 1)
     return fail(FM_ERR_INVALID_ARGUMENT,
-                "tuning: group must be 1,2,4,8,16; block a multiple of 64 in 64..1024; insert_fused and "
-                "insert_dedup 0 or 1");
+                "tuning: group must be 1,2,4,8,16; block a multiple of 64 in 64..1024; insert_fused 0 (the "
+                "cooperative insert was removed); insert_dedup 0 or 1");
   tune_ = *t;
   return FM_OK;
 }
@@ -1003,31 +951,6 @@ fm_status GpuBackend::run_insert(const uint8_t* d_keys, const uint8_t* d_vals, u
   // led without sorting.  Otherwise (the batch may fill the table) group by
   // closure id with a stable radix sort, which also covers "no barrier".
   const bool led = len_ + n < t_.cap && !tune_sort_only_;
-  if (led && tune_.insert_fused && fused_blocks_ && !dedup && !fixed) {
-    last_insert_ = InsertPath::kFused;
-    if (!q_synced_) {
-      Prof p(this, kKidInsCopy);
-      hipError_t e = hipMemcpyAsync(Q_, t_.freebits, words * 8, hipMemcpyDeviceToDevice, stream_);
-      if (e != hipSuccess) return fail(FM_ERR_TRANSFER, hip_msg("bitmap copy", e));
-    }
-    if (!ctr_zero_) {
-      hipError_t e = hipMemsetAsync(ctr_, 0, sizeof(Counters), stream_);
-      if (e != hipSuccess) return fail(FM_ERR_KERNEL_LAUNCH, hip_msg("hipMemsetAsync", e));
-    }
-    dev::FusedInsert a{d_keys, d_vals, n,      Q_,     dslot_, cid_,  ev_,   big_,
-                       cid_s_, idx_s_, ctr_,   bar_,   h_ctr_, tune_.insert_coop_arc, fused_spin_};
-    Prof p(this, kKidInsFused);
-    if ((st = check_launch(ops_.ins_fused(t, a, fused_blocks_, stream_), "k_ins_fused")) != FM_OK) {
-      q_synced_ = ctr_zero_ = ctr_in_host_ = false;
-      return st;
-    }
-    q_synced_ = false;
-    q_pending_ = true;  // Q_ == freebits once the status shows no failed dry pass or stall
-    ctr_zero_ = true;
-    ctr_in_host_ = true;
-    poll_seq_ = 0;
-    return FM_OK;
-  }
   if ((st = zero_counters()) != FM_OK) return st;
   // Q_ is this op's claim bitmap: a copy of freebits, unless the last led
   // insert (or a remove after it) left it equal to freebits already.
@@ -1214,7 +1137,7 @@ const char* GpuBackend::kid_name(int k) {
                                        "ins_radix_sort", "k_ins_replay",  "k_rem_probe",  "k_rem_check",
                                        "k_rem_apply",  "k_rem_runs",      "rem_radix_sort", "k_rem_replay",
                                        "copy_table",   "k_ins_leader",    "k_ins_big",    "k_ins_bits",
-                                       "k_ins_fused",  "k_ins_dedup",     "k_ins_arcs"};
+                                       "k_ins_dedup",  "k_ins_arcs"};
   return names[k];
 }
 
@@ -1520,7 +1443,7 @@ fm_status GpuBackend::bulk_insert_device(const void* d_keys, const void* d_vals,
 
 // fm_bulk_insert_device with every per-call buffer in this GPU's `scratch`
 // array: the same kernels and results, no allocation, no deduplication and
-// no fused launch (neither changes a result), no timing events.
+// no timing events (neither changes a result).
 fm_status GpuBackend::bulk_insert_device_noalloc(const void* d_keys, const void* d_vals, uint64_t n,
                                                  uint64_t* out_new) {
   if (n == 0) {

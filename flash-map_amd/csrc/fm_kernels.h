@@ -55,7 +55,6 @@ constexpr uint32_t kNoOwner = 0xFFFFFFFFu;
 constexpr uint32_t kStatusDryFail = 1u;   // dry pass found no free slot
 constexpr uint32_t kStatusConflict = 2u;  // remove batch needs exact replay
 constexpr uint32_t kStatusBudget = 4u;    // an insert walk exceeded cap+2 steps
-constexpr uint32_t kStatusStall = 8u;     // fused insert: a grid barrier timed out
 constexpr uint32_t kStatusBigPending = 16u;  // host copy only: k_ins_big and the settle still to run
 constexpr uint32_t kStatusConflictAll = 32u; // remove conflict with no EMPTY slot: the whole batch is one run
 
@@ -82,7 +81,7 @@ struct Table {
   // "claimed a free slot" and removes 1/0 for "tombstoned a slot".
   uint8_t* outcome;
   // Per-op: when set, a copy of freebits that removes keep equal to it (the
-  // fused insert's claim bitmap, so the next insert needs no copy).
+  // claim bitmap, so the next insert needs no copy).
   unsigned long long* freecopy;
   // Per-op insert deduplication (null when off): keys are bucketed by home;
   // dd_first[b] = smallest batch index in bucket b, dd_last[b] = largest
@@ -136,17 +135,8 @@ struct Counters {
   uint32_t status;
   uint32_t nbig;  // closures queued for the wave replay
   unsigned long long count;
-  uint32_t done;      // fused insert: workgroups finished
-  uint32_t nsmall;    // fused insert: closures queued for the register replay; k_ins_big: work ticket
-};
-
-// Grid barrier words of the fused insert, each on a cache line of its own
-// (every workgroup polls `gen` while the others arrive).
-struct GridBarrier {
-  uint32_t arrive;
-  uint32_t pad0[31];
-  uint32_t gen;
-  uint32_t pad1[31];
+  uint32_t done;      // completion tickets of a settle pass
+  uint32_t nsmall;    // k_ins_big: work ticket
 };
 
 __device__ __forceinline__ uint32_t make_flag(uint32_t state, uint64_t dist) {
@@ -2023,844 +2013,6 @@ k_ins_bits_rebuild(Table t, Counters* ctr, unsigned long long* Q, Counters* host
   __syncthreads();
   if (threadIdx.x == 0 && atomicAdd(&ctr->done, 1u) == gridDim.x - 1)
     hand_to_host(ctr, host_out, seq, __hip_atomic_load(&ctr->count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-}
-
-// ---------------------------------------------------------------------------
-// Fused insert: the leader path in one cooperative launch.  Three phases over
-// the same key-to-lane mapping, separated by grid-wide barriers:
-//   A  each key dry-claims a Q bit (Q == freebits on entry) and records the
-//      first event of its sequential insert on the pre-batch table;
-//   B  election: a key leads its closure when the nearest pre-batch free slot
-//      below its dry slot is a barrier; a leader counts its closure's members
-//      (dry-claimed slots up to the next barrier) and queues closures too
-//      large or too long for one wave;
-//   C  replay: a one-member closure applies its phase-A event directly (value
-//      update in place, claim, or a shift of the occupied run up to its dry
-//      slot); a closure of several members whose arc fits in 64 slots is
-//      replayed by its whole wave with the arc held in registers; queued
-//      closures are replayed by whole waves from the queue.  Each closure
-//      then settles its own dry claims in freebits/emptybits and in Q, so
-//      that Q == freebits again on exit.
-// Phase A's event is exact for a one-member closure: only its leader writes
-// inside the closure's arc, and only in phase C.  Data that crosses
-// workgroups inside the launch (Q, owner[], the queue) moves through
-// device-scope atomics only, so the barriers need no cache maintenance.  The
-// last workgroup to finish copies the counters to host memory and zeroes them
-// for the next launch.  256-thread workgroups, launched cooperatively (all
-// resident) and at most two per CU; the barriers' waits are bounded.
-// ---------------------------------------------------------------------------
-constexpr uint32_t kCodeBig = 0xFFFFFFFFu;  // election code: queued for a wave
-constexpr uint32_t kCodeQueued = 0xFFFFFFFEu;  // election code: queued for the register replay
-constexpr uint32_t kEvFree = 4u;            // insert event: EMPTY or TOMBSTONE (claim)
-constexpr uint64_t kProbeMax = 64;          // positions phase A decides an event within
-constexpr uint32_t kFusedBlock = 256;
-constexpr uint32_t kWaveLst = 1024;         // closure members one wave sorts at a time
-constexpr uint32_t kShiftMax = 16;          // longest run a one-member shift handles
-constexpr uint32_t kShiftChecks = 4;        // same-home key comparisons a shift plan makes
-
-struct FusedInsert {
-  const uint8_t* keys;
-  const uint8_t* vals;
-  uint64_t n;
-  unsigned long long* Q;  // claim bitmap, == freebits on entry and on exit
-  uint32_t* dslot;        // per key: dry slot
-  uint32_t* code;         // per key: election result (0 follower, m, kCodeQueued, kCodeBig)
-  unsigned long long* ev; // per key: phase-A event (kind << 32 | slot)
-  uint32_t* big;          // queued closures (dry slot of their leader)
-  uint32_t* sq_w;         // closures queued for the register replay: first slot of the arc
-  uint32_t* sq_a;         //   and its length
-  Counters* ctr;          // zero on entry, zero again on exit
-  GridBarrier* bar;       // zero on entry, zero again on exit
-  Counters* host_out;     // host-visible copy of the final counters
-  uint32_t coop_arc;
-  uint32_t spin;          // polls a grid barrier waits before giving up
-};
-
-// Grid-wide barrier number `k` (1, 2, ...) of the cooperative launch, with a
-// bounded wait.  Arrivals accumulate in one counter; the workgroup that
-// brings it to k * gridDim.x opens the next generation.  A workgroup that has
-// polled `spin` times without seeing it flags kStatusStall and returns false,
-// so a launch whose workgroups are not all resident ends instead of hanging.
-// Every thread first waits for its own memory operations to complete; the
-// data that other workgroups read after the barrier was written with
-// device-scope atomics, which are complete at that point.
-__device__ inline bool grid_barrier(GridBarrier* bar, Counters* ctr, uint32_t k, uint32_t spin) {
-  __shared__ uint32_t passed;
-  __builtin_amdgcn_s_waitcnt(0);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    uint32_t ok = 1;
-    const uint32_t gen = ld_agent(&bar->gen);
-    if (atomicAdd(&bar->arrive, 1u) == k * gridDim.x - 1) {
-      __hip_atomic_fetch_add(&bar->gen, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    } else {
-      uint32_t polls = 0;
-      while (ld_agent(&bar->gen) == gen) {
-        if (++polls > spin) {
-          atomicOr(&ctr->status, kStatusStall);
-          ok = 0;
-          break;
-        }
-        __builtin_amdgcn_s_sleep(4);
-      }
-    }
-    passed = ok;
-  }
-  __syncthreads();
-  return passed != 0;
-}
-
-__device__ __forceinline__ unsigned long long pack_ev(uint32_t kind, uint64_t slot) {
-  return (static_cast<unsigned long long>(kind) << 32) | slot;
-}
-
-// First event of the sequential insert of key `k` (rayon_cpu.rs:157-262) on
-// the table as it is, decided within kProbeMax positions of `home`: kEvMatch
-// (equal key at position == dist), kEvTermOcc (OCCUPIED with position > dist,
-// the eviction point), kEvFree (EMPTY or TOMBSTONE, the claim), kEvNone (not
-// decided).  Flags are read four at a time.
-template <int KS>
-__device__ inline unsigned long long insert_event(const Table& t, const Blob<KS>& k, uint32_t K, uint64_t home,
-                                                  uint4 first, uint64_t cslot, const Blob<KS>& ckey) {
-  uint64_t base = home & ~3ull;
-  uint32_t j0 = static_cast<uint32_t>(home & 3);
-  uint64_t p = 0;
-  while (p < kProbeMax) {
-    const uint4 f4 = p == 0 ? first : *reinterpret_cast<const uint4*>(t.flags + base);
-    const uint32_t fs[4] = {f4.x, f4.y, f4.z, f4.w};
-#pragma unroll
-    for (uint32_t j = 0; j < 4; ++j) {
-      if (j < j0) continue;
-      if (p > t.mask) return pack_ev(kEvNone, 0);
-      const uint32_t f = fs[j];
-      const uint64_t idx = base + j;
-      if ((f & 0xFu) != kOccupied) return pack_ev(kEvFree, idx);
-      const uint64_t d = f >> 4;
-      if (p > d) return pack_ev(kEvTermOcc, idx);
-      if (d == p && (idx == cslot ? k.eqb(ckey) : k.eq(t.keys + idx * K, K))) return pack_ev(kEvMatch, idx);
-      ++p;
-    }
-    j0 = 0;
-    base = (base + 4) & t.mask;
-  }
-  return pack_ev(kEvNone, 0);
-}
-
-// Members of the closure whose first dry slot is s: dry-claimed slots
-// (P & ~Q) from s up to the barrier, the first slot at/after s set in both.
-__device__ inline uint32_t closure_members(const unsigned long long* P, const unsigned long long* Q, uint64_t s,
-                                           uint64_t cap, unsigned long long pw0, unsigned long long qw0,
-                                           uint64_t* barrier) {
-  const uint64_t words = (cap + 63) >> 6;
-  const uint64_t w0 = s >> 6;
-  uint64_t w = w0;
-  unsigned long long msk = ~0ull << (s & 63);
-  uint32_t m = 0;
-  for (uint64_t it = 0; it <= words; ++it) {
-    if (it == words) msk = (s & 63) ? (~0ull >> (64 - (s & 63))) : 0ull;  // back at w0: bits below s
-    const unsigned long long pw = it == 0 ? pw0 : P[w];
-    const unsigned long long qw = it == 0 ? qw0 : load_word(&Q[w]);
-    const unsigned long long bar = pw & qw & msk;
-    const unsigned long long mem = pw & ~qw & msk;
-    if (bar) {
-      const int b = __ffsll(bar) - 1;
-      m += static_cast<uint32_t>(__popcll(mem & ((1ull << b) - 1ull)));
-      *barrier = (w << 6) + static_cast<uint64_t>(b);
-      return m;
-    }
-    m += static_cast<uint32_t>(__popcll(mem));
-    msk = ~0ull;
-    w = (w + 1 == words) ? 0 : w + 1;
-  }
-  *barrier = cap;
-  return m;
-}
-
-// Settles one dry claim s of a replayed closure: a claimed slot leaves both
-// bitmaps, an unclaimed dry slot gets its Q bit back.
-__device__ inline void settle_slot(const Table& t, unsigned long long* Q, uint64_t s, bool taken) {
-  const unsigned long long bit = 1ull << (s & 63);
-  if (taken) {
-    atomicAnd(&t.freebits[s >> 6], ~bit);
-    atomicAnd(&t.emptybits[s >> 6], ~bit);
-  } else {
-    atomicOr(&Q[s >> 6], bit);
-  }
-  st_agent(&t.owner[s], kNoOwner);
-}
-
-// One-member closure whose key's first event is OCCUPIED with position >
-// dist at slot e: the sequential insert evicts there and carries residents
-// along the occupied run e..s-1 up to the free slot s (its dry slot).  The
-// eviction points follow from the run's flags alone (a carried resident
-// evicts where its position exceeds the slot's dist), and the moves are done
-// from the last to the first, so each destination's resident has moved on
-// already.  Returns false (nothing written) for a run longer than kShiftMax,
-// a run that is not all OCCUPIED, or a carried resident meeting a slot of
-// equal dist (an equal key there is possible only with duplicates in the
-// table); the caller then walks.
-// Plans the shift: reads the run's flags (fl[j] = slot e+j, in aligned
-// 16-byte chunks; cap is a multiple of 4) and returns the eviction points
-// (bit j: slot e+j), or 0 when the run cannot be shifted this way.  Where a
-// carried resident reaches a slot whose dist equals its position (a resident
-// with the same home) the walk compares keys.  Up to kShiftChecks such
-// comparisons are made on the keys' first 8 bytes (or the whole key when it
-// is shorter), loaded together after the flags; equal prefixes, or more
-// comparisons, return 0 and leave the exact comparison to the walk.  (Equal
-// keys there are possible only with duplicate keys in the table.)
-template <int KS>
-__device__ __forceinline__ uint32_t shift_plan(const Table& t, uint64_t e, uint64_t L, uint32_t (&fl)[kShiftMax]) {
-  if (L == 0 || L > kShiftMax || t.cap < 8 * kShiftMax) return 0;
-  const uint64_t base = e & ~3ull;
-  const uint32_t o = static_cast<uint32_t>(e & 3);
-  const uint64_t nch = (o + L + 3) >> 2;
-  uint32_t fs[kShiftMax + 4];
-#pragma unroll
-  for (uint32_t c = 0; c < (kShiftMax + 4) / 4; ++c) {
-    uint4 q = make_uint4(0, 0, 0, 0);
-    if (c < nch) q = *reinterpret_cast<const uint4*>(t.flags + ((base + 4 * c) & t.mask));
-    fs[4 * c] = q.x;
-    fs[4 * c + 1] = q.y;
-    fs[4 * c + 2] = q.z;
-    fs[4 * c + 3] = q.w;
-  }
-#pragma unroll
-  for (uint32_t j = 0; j < kShiftMax; ++j)
-    fl[j] = o == 0 ? fs[j] : o == 1 ? fs[j + 1] : o == 2 ? fs[j + 2] : fs[j + 3];
-  uint32_t evm = 1u;  // the key itself evicts at e
-  bool ok = (fl[0] & 0xFu) == kOccupied;
-  uint64_t p = (fl[0] >> 4) + 1;  // position of the resident carried from e
-  uint32_t carrier = 0;           // slot (relative) the carried resident came from
-  uint32_t chk = 0, nchk = 0;     // comparisons: byte c = carrier | checked slot << 4
-#pragma unroll
-  for (uint32_t j = 1; j < kShiftMax; ++j) {
-    if (j < L) {
-      const uint64_t d = fl[j] >> 4;
-      ok = ok && (fl[j] & 0xFu) == kOccupied;
-      if (p == d) {
-        if (nchk < kShiftChecks) chk |= (carrier | (j << 4)) << (8 * nchk);
-        ++nchk;
-      }
-      if (p > d) {
-        evm |= 1u << j;
-        p = d + 1;
-        carrier = j;
-      } else {
-        ++p;
-      }
-    }
-  }
-  if (!ok || nchk > kShiftChecks) return 0u;
-  if constexpr (KS >= 8 && KS % 8 == 0) {
-    uint2 x[kShiftChecks], y[kShiftChecks];
-#pragma unroll
-    for (uint32_t c = 0; c < kShiftChecks; ++c) {
-      if (c < nchk) {
-        const uint32_t b = (chk >> (8 * c)) & 0xFFu;
-        x[c] = *reinterpret_cast<const uint2*>(t.keys + ((e + (b & 0xFu)) & t.mask) * KS);
-        y[c] = *reinterpret_cast<const uint2*>(t.keys + ((e + (b >> 4)) & t.mask) * KS);
-      }
-    }
-    bool same = false;
-#pragma unroll
-    for (uint32_t c = 0; c < kShiftChecks; ++c)
-      if (c < nchk) same = same || (x[c].x == y[c].x && x[c].y == y[c].y);
-    if (same) return 0u;
-  } else {
-    if (nchk) return 0u;
-  }
-  return evm;
-}
-
-template <int KS, int VS>
-__device__ inline bool shift_insert(const Table& t, const Blob<KS>& k, const Blob<VS>& v, uint64_t h, uint64_t e,
-                                    uint64_t s, uint32_t K, uint32_t V) {
-  const uint64_t L = (s - e) & t.mask;
-  uint32_t fl[kShiftMax];
-  const uint32_t evm = shift_plan<KS>(t, e, L, fl);
-  if (!evm) return false;
-  uint64_t dest = L;  // relative slot the next resident (downwards) moves to
-#pragma unroll
-  for (int j = kShiftMax - 1; j >= 0; --j) {
-    if (static_cast<uint64_t>(j) < L && ((evm >> j) & 1u)) {
-      const uint64_t src = (e + j) & t.mask;
-      const uint64_t dst = (e + dest) & t.mask;
-      Blob<KS> rk;
-      Blob<VS> rv;
-      rk.load(t.keys + src * K, K);
-      rv.load(t.vals + src * V, V);
-      rk.store(t.keys + dst * K, K);
-      rv.store(t.vals + dst * V, V);
-      t.flags[dst] = make_flag(kOccupied, (fl[j] >> 4) + (dest - j));
-      dest = j;
-    }
-  }
-  k.store(t.keys + e * K, K);
-  v.store(t.vals + e * V, V);
-  t.flags[e] = make_flag(kOccupied, (e - h) & t.mask);
-  return true;
-}
-
-// Whether the one-member closure of a key with phase-A event `ev` and dry
-// slot s is applied directly (update in place, claim of s, or a shift of at
-// most kShiftMax slots) rather than walked.
-__device__ __forceinline__ bool single_is_direct(const Table& t, unsigned long long ev, uint64_t s) {
-  const uint32_t kind = static_cast<uint32_t>(ev >> 32);
-  const uint64_t j = ev & 0xFFFFFFFFull;
-  if (kind == kEvMatch) return true;
-  if (kind == kEvFree) return j == s;
-  return kind == kEvTermOcc && ((s - j) & t.mask) <= kShiftMax && t.cap >= 8 * kShiftMax;
-}
-
-// Phase C, one-member closure of key i.
-template <int KS, int VS>
-__device__ inline uint32_t fused_single(const Table& t, const FusedInsert& a, uint64_t i, uint32_t K, uint32_t V) {
-  const uint64_t s = a.dslot[i];
-  const unsigned long long e = a.ev[i];
-  const uint32_t kind = static_cast<uint32_t>(e >> 32);
-  const uint64_t j = e & 0xFFFFFFFFull;
-  Blob<VS> v;
-  v.load(a.vals + i * V, V);
-  uint32_t c1 = 0;
-  uint64_t at = s;
-  if (kind == kEvMatch) {
-    v.store(t.vals + j * V, V);  // update in place, not new
-  } else {
-    Blob<KS> k;
-    k.load(a.keys + i * K, K);
-    const uint64_t h = k.hash(K, t.mode) & t.mask;
-    if (kind == kEvFree && j == s) {
-      k.store(t.keys + s * K, K);
-      v.store(t.vals + s * V, V);
-      t.flags[s] = make_flag(kOccupied, (s - h) & t.mask);
-      c1 = 1;
-    } else if (kind == kEvTermOcc && shift_insert<KS, VS>(t, k, v, h, j, s, K, V)) {
-      c1 = 1;
-    } else {
-      const uint64_t p0 = kind == kEvTermOcc ? ((j - h) & t.mask) : 0;
-      c1 = insert_walk<KS, VS, false>(t, k, v, h, p0, K, V, a.ctr, &at);
-    }
-  }
-  if (t.outcome) t.outcome[i] = static_cast<uint8_t>(c1);
-  if (c1 && at != s) {  // a one-member closure claims its own dry slot; kept exact regardless
-    settle_slot(t, a.Q, at, true);
-    atomicAnd(&a.Q[at >> 6], ~(1ull << (at & 63)));
-    settle_slot(t, a.Q, s, false);
-  } else {
-    settle_slot(t, a.Q, s, c1 != 0);
-  }
-  return c1;
-}
-
-constexpr uint32_t kFromBatch = 0x80000000u;  // move source: a batch pair, not a slot
-constexpr uint32_t kWaveMoves = 2048 / 3;      // moves one wave's share of lst holds
-
-// Phase C, the one-member closures of one wave (static sizes; every lane
-// calls, `mine` marks the lanes that hold one, key i).  Updates in place and
-// claims of the dry slot are written directly.  The eviction shifts of all
-// lanes go into one move list in the wave's LDS share `ml`: each lane lists
-// its run's moves from the last eviction point down (the resident carried
-// furthest first, the batch pair last), so a move's destination has always
-// been read by an earlier move of the list; the list is then applied 64
-// moves at a time, every lane loading its pair before any lane stores.  A
-// lane whose moves do not fit shifts on its own; an event that needs a walk
-// is walked.  Returns the lane's claim.
-template <int KS, int VS>
-__device__ inline uint32_t fused_singles_wave(const Table& t, const FusedInsert& a, uint64_t i, uint32_t* ml) {
-  const uint32_t lane = threadIdx.x & 63u;
-  uint64_t s = 0, j = 0, h = 0, L = 0;
-  uint32_t kind = kEvNone, c1 = 0, evm = 0;
-  Blob<KS> k;
-  Blob<VS> v;
-  uint32_t fl[kShiftMax];
-  // Everything a one-member closure may need is loaded at once, for every
-  // key of the wave (the election code decides afterwards).
-  uint32_t code = 0;
-  unsigned long long e = 0;
-  if (i < a.n) {
-    code = a.code[i];
-    s = a.dslot[i];
-    e = a.ev[i];
-    v.load(a.vals + i * VS, VS);
-    k.load(a.keys + i * KS, KS);
-  }
-  const bool mine = i < a.n && code < kCodeQueued && (code & 0xFFu) == 1u;
-  bool done = !mine;
-  if (mine) {
-    kind = static_cast<uint32_t>(e >> 32);
-    j = e & 0xFFFFFFFFull;
-    if (kind == kEvMatch) {
-      v.store(t.vals + j * VS, VS);  // update in place, not new
-      done = true;
-    } else {
-      h = k.hash(KS, t.mode) & t.mask;
-      if (kind == kEvFree && j == s) {
-        k.store(t.keys + s * KS, KS);
-        v.store(t.vals + s * VS, VS);
-        t.flags[s] = make_flag(kOccupied, (s - h) & t.mask);
-        c1 = 1;
-        done = true;
-      } else if (kind == kEvTermOcc) {
-        L = (s - j) & t.mask;
-        evm = shift_plan<KS>(t, j, L, fl);
-      }
-    }
-  }
-  // moves of this lane: one per eviction point, plus the batch pair
-  const uint32_t nmov = (!done && evm) ? static_cast<uint32_t>(__popc(evm)) + 1u : 0u;
-  uint32_t incl = nmov;
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    const uint32_t y = static_cast<uint32_t>(__shfl_up(static_cast<int>(incl), off, 64));
-    if (lane >= static_cast<uint32_t>(off)) incl += y;
-  }
-  const uint32_t first = incl - nmov;
-  const bool listed = nmov != 0 && incl <= kWaveMoves;
-  if (listed) {
-    uint64_t dest = L;
-    uint32_t w = first;
-#pragma unroll
-    for (int jj = kShiftMax - 1; jj >= 0; --jj) {
-      if (static_cast<uint64_t>(jj) < L && ((evm >> jj) & 1u)) {
-        ml[3 * w] = static_cast<uint32_t>((j + jj) & t.mask);
-        ml[3 * w + 1] = static_cast<uint32_t>((j + dest) & t.mask);
-        ml[3 * w + 2] = make_flag(kOccupied, (fl[jj] >> 4) + (dest - jj));
-        ++w;
-        dest = jj;
-      }
-    }
-    ml[3 * w] = kFromBatch | static_cast<uint32_t>(i);
-    ml[3 * w + 1] = static_cast<uint32_t>(j);
-    ml[3 * w + 2] = make_flag(kOccupied, (j - h) & t.mask);
-    c1 = 1;
-    done = true;
-  }
-  uint32_t used = listed ? incl : 0u;  // listed lanes are a prefix of the wave
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) used = max(used, static_cast<uint32_t>(__shfl_xor(static_cast<int>(used), off, 64)));
-  wave_sync();
-  for (uint32_t base = 0; base < used; base += 64) {
-    const uint32_t e = base + lane;
-    if (e < used) {
-      const uint32_t src = ml[3 * e], dst = ml[3 * e + 1], fg = ml[3 * e + 2];
-      Blob<KS> mk;
-      Blob<VS> mv;
-      if (src & kFromBatch) {
-        const uint64_t bi = src & ~kFromBatch;
-        mk.load(a.keys + bi * KS, KS);
-        mv.load(a.vals + bi * VS, VS);
-      } else {
-        mk.load(t.keys + static_cast<uint64_t>(src) * KS, KS);
-        mv.load(t.vals + static_cast<uint64_t>(src) * VS, VS);
-      }
-      mk.store(t.keys + static_cast<uint64_t>(dst) * KS, KS);
-      mv.store(t.vals + static_cast<uint64_t>(dst) * VS, VS);
-      t.flags[dst] = fg;
-    }
-  }
-  uint64_t at = s;
-  if (!done) {
-    if (evm && shift_insert<KS, VS>(t, k, v, h, j, s, KS, VS)) {
-      c1 = 1;
-    } else {
-      const uint64_t p0 = kind == kEvTermOcc ? ((j - h) & t.mask) : 0;
-      c1 = insert_walk<KS, VS, false>(t, k, v, h, p0, KS, VS, a.ctr, &at);
-    }
-  }
-  if (mine) {
-    if (t.outcome) t.outcome[i] = static_cast<uint8_t>(c1);
-    if (c1 && at != s) {  // a one-member closure claims its own dry slot; kept exact regardless
-      settle_slot(t, a.Q, at, true);
-      atomicAnd(&a.Q[at >> 6], ~(1ull << (at & 63)));
-      settle_slot(t, a.Q, s, false);
-    } else {
-      settle_slot(t, a.Q, s, c1 != 0);
-    }
-  }
-  return c1;
-}
-
-// Phase C, closure of 2..kLeaderMax members led by key i, on one lane (`my`
-// is the lane's LDS column).
-template <int KS, int VS>
-__device__ inline uint32_t fused_small(const Table& t, const FusedInsert& a, uint64_t i, uint32_t m, uint32_t* my,
-                                       uint32_t K, uint32_t V) {
-  const unsigned long long* P = t.freebits;
-  const uint64_t s = a.dslot[i];
-  uint64_t pos = s;
-  for (uint32_t j = 0; j < m; ++j) {
-    const uint64_t x = bm_next_set(P, pos, t.cap);
-    my[j * kFusedBlock] = (x == s) ? static_cast<uint32_t>(i) : ld_agent(&t.owner[x]);
-    pos = (x + 1 == t.cap) ? 0 : x + 1;
-  }
-  for (uint32_t u = 1; u < m; ++u) {  // insertion sort by batch index
-    const uint32_t val = my[u * kFusedBlock];
-    uint32_t c = u;
-    while (c > 0 && my[(c - 1) * kFusedBlock] > val) {
-      my[c * kFusedBlock] = my[(c - 1) * kFusedBlock];
-      --c;
-    }
-    my[c * kFusedBlock] = val;
-  }
-  uint32_t claimed = 0;
-  for (uint32_t j = 0; j < m; ++j) {
-    const uint64_t k = my[j * kFusedBlock];
-    const uint32_t c1 = insert_one<KS, VS, false>(t, a.keys + k * K, a.vals + k * V, K, V, a.ctr);
-    claimed += c1;
-    if (t.outcome) t.outcome[k] = static_cast<uint8_t>(c1);
-  }
-  pos = s;
-  for (uint32_t j = 0; j < m; ++j) {  // the closure's dry slots, in slot order
-    const uint64_t x = bm_next_set(P, pos, t.cap);
-    settle_slot(t, a.Q, x, (t.flags[x] & 0xFu) == kOccupied);
-    pos = (x + 1 == t.cap) ? 0 : x + 1;
-  }
-  return claimed;
-}
-
-// Phase C, a closure of several members whose arc (slots w0 .. w0+A-1,
-// A <= 64) is replayed by a whole wave with the arc in registers: lane j
-// holds slot w0+j (flag, key, value) and lane r the r-th member's pair in
-// batch order.  Every member's sequential walk is decided on wave-uniform
-// state (readlane of the slot it reaches) and applied to the lanes' copies;
-// the changed slots are written back once, then the arc's dry claims are
-// settled.  Within the arc every pre-batch free slot is a dry claim (the arc
-// lies between two consecutive barriers), and no walk leaves it.  Called by
-// all 64 lanes with uniform arguments; returns the claims (in every lane).
-template <int KS, int VS>
-__device__ uint32_t wave_replay_arc(const Table& t, const FusedInsert& a, uint64_t w0, uint32_t A) {
-  static_assert(KS % 4 == 0 && VS % 4 == 0 && KS > 0 && VS > 0, "static word sizes");
-  constexpr int KW = KS / 4, VW = VS / 4;
-  const uint32_t lane = threadIdx.x & 63u;
-  const bool in = lane < A;
-  const uint64_t slot = (w0 + lane) & t.mask;
-  uint32_t f = 0, own = kNoOwner;
-  bool member = false;
-  Blob<KS> sk;
-  Blob<VS> sv;
-#pragma unroll
-  for (int w = 0; w < KW; ++w) sk.w[w] = 0;
-#pragma unroll
-  for (int w = 0; w < VW; ++w) sv.w[w] = 0;
-  if (in) {
-    f = t.flags[slot];
-    sk.load(t.keys + slot * KS, KS);
-    sv.load(t.vals + slot * VS, VS);
-    member = ((t.freebits[slot >> 6] >> (slot & 63)) & 1ull) != 0;
-    own = ld_agent(&t.owner[slot]);
-  }
-  const unsigned long long mm = __ballot(member);
-  const uint32_t m = static_cast<uint32_t>(__popcll(mm));
-  // rank of each member by batch index; lane r then fetches member r's pair
-  uint32_t rank = 0;
-  for (unsigned long long rest = mm; rest; rest &= rest - 1) {
-    const int l = __ffsll(rest) - 1;
-    const uint32_t o = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(own), l));
-    rank += (o < own) ? 1u : 0u;
-  }
-  uint32_t myidx = 0;
-  for (unsigned long long rest = mm; rest; rest &= rest - 1) {
-    const int l = __ffsll(rest) - 1;
-    const uint32_t r = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(rank), l));
-    const uint32_t o = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(own), l));
-    if (lane == r) myidx = o;
-  }
-  Blob<KS> mk;
-  Blob<VS> mv;
-#pragma unroll
-  for (int w = 0; w < KW; ++w) mk.w[w] = 0;
-#pragma unroll
-  for (int w = 0; w < VW; ++w) mv.w[w] = 0;
-  if (lane < m) {
-    mk.load(a.keys + static_cast<uint64_t>(myidx) * KS, KS);
-    mv.load(a.vals + static_cast<uint64_t>(myidx) * VS, VS);
-  }
-  const uint32_t mh = lane < m ? static_cast<uint32_t>(mk.hash(KS, t.mode) & t.mask) : 0u;
-  uint32_t sh = in ? static_cast<uint32_t>(sk.hash(KS, t.mode) & t.mask) : 0u;  // slot key's home
-  bool dirty = false, vdirty = false;
-  uint32_t claimed = 0;
-  for (uint32_t r = 0; r < m; ++r) {
-    Blob<KS> ck;
-    Blob<VS> cv;
-#pragma unroll
-    for (int w = 0; w < KW; ++w) ck.w[w] = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(mk.w[w]), r));
-#pragma unroll
-    for (int w = 0; w < VW; ++w) cv.w[w] = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(mv.w[w]), r));
-    uint32_t ch = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(mh), r));
-    auto eq_ballot = [&]() {
-      uint32_t diff = (in && (f & 0xFu) == kOccupied) ? 0u : 1u;
-#pragma unroll
-      for (int w = 0; w < KW; ++w) diff |= sk.w[w] ^ ck.w[w];
-      return static_cast<unsigned long long>(__ballot(diff == 0));
-    };
-    unsigned long long eqm = eq_ballot();
-    uint64_t p = 0;
-    uint32_t j = static_cast<uint32_t>((ch - w0) & t.mask);
-    uint32_t c1 = 0;
-    for (;;) {
-      // Jump to the next slot where something happens: the carried pair is at
-      // position p + (l - j) when it reaches lane l; it stops at a free slot
-      // or where dist <= that position (eviction, or an equal-dist resident
-      // whose key is compared).  Slots before it are passed over.
-      const bool stop = in && lane >= j && ((f & 0xFu) != kOccupied || (f >> 4) <= p + (lane - j));
-      const unsigned long long sm = __ballot(stop);
-      if (!sm) {  // no walk leaves the arc; reaching its end means the state is not what was elected
-        if (lane == 0) atomicOr(&a.ctr->status, kStatusBudget);
-        break;
-      }
-      const uint32_t nj = static_cast<uint32_t>(__ffsll(sm) - 1);
-      p += nj - j;
-      j = nj;
-      const uint32_t fj = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(f), j));
-      const uint64_t d = fj >> 4;
-      if ((fj & 0xFu) == kOccupied) {
-        if (d == p && ((eqm >> j) & 1ull)) {  // equal key: update its value in place
-          if (lane == j) {
-            sv = cv;
-            vdirty = true;
-          }
-          break;
-        }
-        if (p > d) {  // Robin Hood eviction: the carried pair in, the resident carried on
-          Blob<KS> nk;
-          Blob<VS> nv;
-#pragma unroll
-          for (int w = 0; w < KW; ++w)
-            nk.w[w] = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(sk.w[w]), j));
-#pragma unroll
-          for (int w = 0; w < VW; ++w)
-            nv.w[w] = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(sv.w[w]), j));
-          const uint32_t nh = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(sh), j));
-          if (lane == j) {
-            sk = ck;
-            sv = cv;
-            sh = ch;
-            f = make_flag(kOccupied, p);
-            dirty = true;
-          }
-          ck = nk;
-          cv = nv;
-          ch = nh;
-          p = d + 1;
-          ++j;
-          eqm = eq_ballot();
-          continue;
-        }
-        ++p;
-        ++j;
-        continue;
-      }
-      if (lane == j) {  // EMPTY or TOMBSTONE: claim
-        sk = ck;
-        sv = cv;
-        sh = ch;
-        f = make_flag(kOccupied, p);
-        dirty = true;
-      }
-      c1 = 1;
-      break;
-    }
-    claimed += c1;
-    if (lane == r && t.outcome) t.outcome[myidx] = static_cast<uint8_t>(c1);
-  }
-  if (dirty) {
-    sk.store(t.keys + slot * KS, KS);
-    sv.store(t.vals + slot * VS, VS);
-    t.flags[slot] = f;
-  } else if (vdirty) {
-    sv.store(t.vals + slot * VS, VS);
-  }
-  if (member) settle_slot(t, a.Q, slot, (f & 0xFu) == kOccupied);
-  return claimed;
-}
-
-// Phase C of the fused insert (whole workgroup; `lst` is its LDS list): the
-// workgroup's own keys, then its share of the two queues (entry c goes to
-// wave c mod waves-in-grid).  Returns this thread's share of the claims.
-template <int KS, int VS>
-__device__ inline unsigned long long fused_replay(const Table& t, const FusedInsert& a, uint32_t* lst, uint32_t K,
-                                                  uint32_t V) {
-  const uint64_t n = a.n;
-  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
-  const uint32_t lane = threadIdx.x & 63u;
-  const uint32_t gw = blockIdx.x * (kFusedBlock / 64) + (threadIdx.x >> 6);  // wave id in the grid
-  const uint32_t nw = gridDim.x * (kFusedBlock / 64);
-  uint32_t* my = lst + threadIdx.x;
-  unsigned long long claimed = 0;
-  // this wave's first register-replay entry, fetched ahead of the lane work
-  const uint32_t nsmall = ld_agent(&a.ctr->nsmall);
-  uint32_t w_first = 0, a_first = 0;
-  if (gw < nsmall) {
-    w_first = ld_agent(&a.sq_w[gw]);
-    a_first = ld_agent(&a.sq_a[gw]);
-  }
-  for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * blockDim.x; base < n; base += stride) {
-    const uint64_t i = base + threadIdx.x;
-    if constexpr (KS > 0 && VS > 0 && KS % 4 == 0 && VS % 4 == 0) {
-      // m >= 2 closures were all queued in phase B
-      claimed += fused_singles_wave<KS, VS>(t, a, i,
-                                            lst + (threadIdx.x >> 6) * (kLeaderMax * kFusedBlock / (kFusedBlock / 64)));
-    } else {
-      const uint32_t code = i < n ? a.code[i] : 0u;
-      const uint32_t m = code & 0xFFu;
-      if (i < n && code < kCodeQueued && m == 1) claimed += fused_single<KS, VS>(t, a, i, K, V);
-      else if (i < n && code < kCodeQueued && m >= 2) claimed += fused_small<KS, VS>(t, a, i, m, my, K, V);
-    }
-  }
-  if constexpr (KS > 0 && VS > 0 && KS % 4 == 0 && VS % 4 == 0) {
-    uint32_t got = 0;
-    for (uint32_t c = gw; c < nsmall; c += nw)
-      got += wave_replay_arc<KS, VS>(t, a, c == gw ? w_first : ld_agent(&a.sq_w[c]),
-                                     c == gw ? a_first : ld_agent(&a.sq_a[c]));
-    if (lane == 0) claimed += got;
-  }
-  const uint32_t nbig = ld_agent(&a.ctr->nbig);
-  if (nbig) {
-    // Queued large closures, one wave each; lst is split into 2048 words per wave.
-    __syncthreads();
-    uint32_t* region = lst + (threadIdx.x >> 6) * (kLeaderMax * kFusedBlock / (kFusedBlock / 64));
-    uint32_t* carry = region;     // 2 * kMaxGenericSize / 4 = 128 words
-    uint32_t* cnt = region + 128;
-    uint32_t* wl = region + 256;  // kWaveLst words
-    for (uint32_t c = gw; c < nbig; c += nw) {
-      const uint32_t got =
-          replay_arc<KS, VS, true>(t, a.keys, a.vals, n, a.Q, ld_agent(&a.big[c]), wl, kWaveLst, carry, cnt, a.ctr);
-      if (lane == 0) claimed += got;
-    }
-  }
-  return claimed;
-}
-
-template <int KS, int VS>
-__global__ void __launch_bounds__(kFusedBlock)
-k_ins_fused(Table t, FusedInsert a) {
-  __shared__ uint32_t lst[kLeaderMax * kFusedBlock];
-  const uint32_t K = KS ? KS : t.K;
-  const uint32_t V = VS ? VS : t.V;
-  const uint64_t n = a.n;
-  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
-  const uint64_t tid = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  const unsigned long long* P = t.freebits;
-  const uint64_t budget = ((t.cap + 63) >> 6) + n + 2;
-
-  // A: dry claim and pre-batch event.
-  for (uint64_t i = tid; i < n; i += stride) {
-    Blob<KS> k;
-    k.load(a.keys + i * K, K);
-    const uint64_t h = k.hash(K, t.mode) & t.mask;
-    const uint4 f0 = *reinterpret_cast<const uint4*>(t.flags + (h & ~3ull));  // overlaps the claim
-    // The first slot of that chunk whose dist equals its probe position is the
-    // likely match: its key is loaded before the claim too.
-    uint64_t cslot = ~0ull;
-    Blob<KS> ckey;
-    if constexpr (KS > 0) {
-      const uint32_t fs0[4] = {f0.x, f0.y, f0.z, f0.w};
-      const uint32_t j0 = static_cast<uint32_t>(h & 3);
-      bool stop = false;
-#pragma unroll
-      for (uint32_t jj = 0; jj < 4; ++jj) {
-        if (jj >= j0 && !stop) {
-          const uint32_t f = fs0[jj];
-          const uint32_t pp = jj - j0;
-          if ((f & 0xFu) != kOccupied || pp > (f >> 4)) {
-            stop = true;
-          } else if ((f >> 4) == pp) {
-            cslot = (h & ~3ull) + jj;
-            stop = true;
-          }
-        }
-      }
-      if (cslot != ~0ull) ckey.load(t.keys + cslot * K, K);
-    }
-    uint64_t s;
-    if (bm_claim(a.Q, h, t.cap, budget, &s)) {
-      a.dslot[i] = static_cast<uint32_t>(s);
-      st_agent(&t.owner[s], static_cast<uint32_t>(i));
-    } else {
-      atomicOr(&a.ctr->status, kStatusDryFail);
-      a.dslot[i] = kNoSlot;
-    }
-    a.ev[i] = insert_event<KS>(t, k, K, h, f0, cslot, ckey);
-  }
-
-  // No workgroup reaches phase C, the only one that writes the table, unless
-  // every workgroup has passed both barriers.
-  if (!grid_barrier(a.bar, a.ctr, 1, a.spin)) {
-    // stalled: nothing applied (the host restores owner[] and Q)
-  } else if (ld_agent(&a.ctr->status) & kStatusDryFail) {
-    // nothing applied; undo the owner[] entries
-    for (uint64_t i = tid; i < n; i += stride) {
-      const uint32_t s = a.dslot[i];
-      if (s != kNoSlot) st_agent(&t.owner[s], kNoOwner);
-    }
-  } else {
-    // B: election.
-    // Queue slots are reserved per workgroup (one atomic each per round).
-    constexpr bool kRegs = KS > 0 && VS > 0 && KS % 4 == 0 && VS % 4 == 0;
-    __shared__ uint32_t q_cnt[2], q_base[2];
-    for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * blockDim.x; base < n; base += stride) {
-      const uint64_t i = base + threadIdx.x;
-      if (threadIdx.x < 2) q_cnt[threadIdx.x] = 0;
-      __syncthreads();
-      uint32_t code = 0, want = 0, local = 0;  // want: 1 register replay, 2 wave replay
-      uint64_t b = 0, arc = 0, s = 0;
-      if (i < n) {
-        s = a.dslot[i];
-        const unsigned long long pw = P[s >> 6], qw = load_word(&a.Q[s >> 6]);
-        const unsigned long long below = pw & ((1ull << (s & 63)) - 1ull);
-        b = below ? ((s & ~63ull) + static_cast<uint64_t>(63 - __clzll(below))) : bm_prev_set(P, s, t.cap);
-        const unsigned long long qb = (b >> 6) == (s >> 6) ? qw : load_word(&a.Q[b >> 6]);
-        if (b != s && ((qb >> (b & 63)) & 1ull)) {
-          uint64_t x;
-          const uint32_t m = closure_members(P, a.Q, s, t.cap, pw, qw, &x);
-          arc = (x - b - 1) & t.mask;
-          // Static sizes: anything a lane cannot apply directly goes to a
-          // wave (register replay for arcs of up to 64 slots, else the queue
-          // of large closures); a walk is left on a lane only for small
-          // tables or one-member arcs longer than 64 slots.
-          const bool direct = m == 1 && single_is_direct(t, a.ev[i], s);
-          const bool regs_ok = kRegs && arc <= 64 && t.cap > 128;
-          if (m > kLeaderMax || arc > a.coop_arc || (kRegs && m >= 2 && !regs_ok)) {
-            want = 2;
-            code = kCodeBig;
-          } else if (!direct && regs_ok) {
-            want = 1;
-            code = kCodeQueued;
-          } else {
-            code = m;
-          }
-        }
-        a.code[i] = code;
-        if (want) local = atomicAdd(&q_cnt[want - 1], 1u);
-      }
-      __syncthreads();
-      if (threadIdx.x < 2 && q_cnt[threadIdx.x])
-        q_base[threadIdx.x] = atomicAdd(threadIdx.x == 0 ? &a.ctr->nsmall : &a.ctr->nbig, q_cnt[threadIdx.x]);
-      __syncthreads();
-      if (want == 1) {
-        st_agent(&a.sq_w[q_base[0] + local], static_cast<uint32_t>((b + 1) & t.mask));
-        st_agent(&a.sq_a[q_base[0] + local], static_cast<uint32_t>(arc));
-      } else if (want == 2) {
-        st_agent(&a.big[q_base[1] + local], static_cast<uint32_t>(s));
-      }
-    }
-    if (grid_barrier(a.bar, a.ctr, 2, a.spin)) block_add(&a.ctr->count, fused_replay<KS, VS>(t, a, lst, K, V));
-  }
-
-  // The last workgroup out publishes the counters and zeroes them.
-  __builtin_amdgcn_s_waitcnt(0);
-  __syncthreads();
-  if (threadIdx.x == 0 && atomicAdd(&a.ctr->done, 1u) == gridDim.x - 1) {
-    Counters out{};
-    out.status = ld_agent(&a.ctr->status);
-    out.nbig = ld_agent(&a.ctr->nbig);
-    out.count = __hip_atomic_load(&a.ctr->count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    *a.host_out = out;
-    *a.ctr = Counters{};
-    a.bar->arrive = 0;
-    a.bar->gen = 0;
-    __threadfence_system();
-  }
 }
 
 // ---------------------------------------------------------------------------

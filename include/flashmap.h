@@ -235,9 +235,11 @@ typedef struct fm_tuning {
   uint32_t insert_block;
   uint32_t insert_coop_arc; /* closures whose barrier-to-barrier arc is longer
                                are walked by a whole wave (0: all of them) */
-  uint32_t insert_fused;    /* 1: batches that cannot fill the table run as one
-                               cooperative launch with grid-wide barriers;
-                               0 (default): as a sequence of kernels */
+  uint32_t insert_fused;    /* must be 0: kept for the layout of the struct
+                               (a cooperative single-launch insert measured
+                               slower than the kernel sequence and was
+                               removed); anything else is
+                               FM_ERR_INVALID_ARGUMENT */
   uint32_t insert_dedup;    /* 1: fold later copies of a key within an insert
                                batch into its first copy before claiming slots
                                (exact; pays off for batches with hot keys;

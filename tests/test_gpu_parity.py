@@ -215,49 +215,21 @@ def test_config2_reduced_bit_exact():
     apply_and_compare(m, o, ops)
 
 
-@pytest.mark.parametrize("seed", range(12))
-def test_gpu_fused_insert_path(seed):
-    """insert_fused = 1: the leader path as one cooperative launch (dry
-    claims, election and replay separated by grid barriers)."""
-    rng = np.random.default_rng(4000 + seed)
-    cap = [64, 256, 1024, 4096, 1 << 14, 1 << 16][seed % 6]
-    K, V = [(8, 8), (32, 128), (3, 5), (32, 8)][seed % 4]
-    say(f"fused insert seed={seed} cap={cap} K={K} V={V}")
-    ops = random_history(rng, cap, K, V, 20, max(1, cap // 2), cap * 2)
-    m = gpu_make(cap, K, V, seed % 2)
-    m.set_tuning(insert_fused=1, insert_coop_arc=[0, 64, 16][seed % 3])
-    apply_and_compare(m, OracleAdapter(cap, K, V, seed % 2), ops)
-
-
-def test_gpu_fused_insert_large_batches():
-    """Batches larger than one pass of the cooperative grid (every lane takes
-    several keys per phase), config 1 shape, through the fused path."""
-    from flashmap_amd.workloads import config1
-
-    say("fused insert large batches")
-    c = config1(n_insert=400_000, n_query=100_000, n_remove=50_000, capacity=1 << 20)
-    m = gpu_make(c.capacity, 8, 8)
-    m.set_tuning(insert_fused=1)
-    apply_and_compare(m, OracleAdapter(c.capacity, 8, 8),
-                      [("ins", c.insert_keys[:300_000], c.insert_vals[:300_000]), ("rem", c.remove_keys),
-                       ("ins", c.insert_keys[200_000:], c.insert_vals[200_000:]), ("get", c.queries)])
-
-
-def test_gpu_fused_and_multikernel_agree_config2():
-    """Both insert paths on the config 2 shape, from the same preload."""
+def test_gpu_config2_shape_history():
+    """The config 2 shape at reduced size: preload, sparse batches, removes,
+    re-inserts of removed keys, gets.  The retired cooperative insert is
+    refused by the tuning call."""
     from flashmap_amd.workloads import config2
 
-    say("fused vs multi-kernel config2 shape")
+    say("config2 shape history")
     c = config2(n_preload=200_000, n_batches=3, batch=5_000, n_get=20_000, n_remove=5_000, capacity=400_000)
     o = OracleAdapter(c.capacity, 32, 128)
     ops = [("ins", c.preload_keys, c.preload_vals)] + [("ins", k, v) for k, v in c.batches] + \
           [("rem", c.remove_keys), ("ins", c.remove_keys[:2000], c.batches[0][1][:2000]), ("get", c.get_keys)]
     m = gpu_make(c.capacity, 32, 128)
-    m.set_tuning(insert_fused=0)
+    with pytest.raises(fm.FlashMapError):
+        m.set_tuning(insert_fused=1)
     apply_and_compare(m, o, ops)
-    m2 = gpu_make(c.capacity, 32, 128)
-    m2.set_tuning(insert_fused=1)
-    apply_and_compare(m2, OracleAdapter(c.capacity, 32, 128), ops)
 
 
 @pytest.mark.parametrize("structure_only", [False, True], ids=["all", "structure"])
@@ -272,49 +244,18 @@ def test_gpu_copy_then_ops(structure_only):
     src, o = gpu_make(cap, K, V), OracleAdapter(cap, K, V)
     apply_and_compare(src, o, ops1)
     dst = gpu_make(cap, K, V)
-    dst.set_tuning(insert_fused=1)
     if structure_only:  # values are not copied: dst first holds the same values
         apply_and_compare(dst, OracleAdapter(cap, K, V), ops1)
     dst.copy_from(src, structure_only=structure_only)
     apply_and_compare(dst, o, ops2)
 
 
-def test_gpu_fused_barrier_timeout_leaves_table(monkeypatch):
-    """FM_FUSED_SPIN=0 makes the fused insert's grid barriers give up at once.
-    A batch whose workgroups do not all meet must fail without touching the
-    table (the phase that writes it runs only after both barriers), and the
-    map must then go on exactly through the multi-kernel path."""
-    monkeypatch.setenv("FM_FUSED_SPIN", "0")
-    cap = 1 << 14
-    m = gpu_make(cap, 8, 8)
-    m.set_tuning(insert_fused=1)
-    monkeypatch.delenv("FM_FUSED_SPIN")
-    say("fused barrier timeout")
-    rng = np.random.default_rng(17)
-    rows = np.unique(rng.integers(1, 1 << 62, size=9000, dtype=np.int64)).astype(np.uint64)[:8000]
-    rows = rows.view(np.uint8).reshape(-1, 8)
-    o = OracleAdapter(cap, 8, 8)
-    try:
-        got = m.bulk_insert(rows[:4000], rows[:4000])
-    except fm.FlashMapError:
-        got = None
-    if got is None:
-        assert len(m) == 0
-        assert_same_slots(m, o, "after a timed-out fused insert")
-    else:
-        assert got == o.bulk_insert(rows[:4000], rows[:4000])
-        assert_same_slots(m, o, "fused insert whose barriers all met")
-    apply_and_compare(m, o, [("ins", rows[2000:6000], rows[2000:6000]), ("rem", rows[:1000]),
-                             ("ins", rows[5000:], rows[:3000]), ("get", rows)])
-
-
-@pytest.mark.parametrize("fused", [1, 0])
-def test_gpu_shift_past_duplicate_key(fused):
+def test_gpu_shift_past_duplicate_key():
     """A shift whose carried resident meets its own duplicate (left by the
     tombstone anomaly): the sequential insert updates the duplicate's value
     and claims nothing.  Identity hash, home = key mod 1024."""
     cap, h = 1024, 500
-    say(f"shift past duplicate fused={fused}")
+    say("shift past duplicate")
 
     def key(home, tag):
         return np.array([home + cap * tag], dtype=np.uint64).view(np.uint8).reshape(1, 8)
@@ -327,7 +268,6 @@ def test_gpu_shift_past_duplicate_key(fused):
            ("ins", Z, v(5)),                # evicts Y at h; the carried Y meets its duplicate
            ("get", np.concatenate([W, X, Y, Z]))]
     m = gpu_make(cap, 8, 8)
-    m.set_tuning(insert_fused=fused)
     apply_and_compare(m, OracleAdapter(cap, 8, 8), ops)
 
 
