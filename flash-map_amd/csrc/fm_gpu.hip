@@ -645,6 +645,11 @@ fm_status GpuBackend::init(const fm_builder& b, std::string* err) {
   if (const char* env = std::getenv("FM_INSERT_SORT_PATH")) tune_sort_only_ = env[0] == '1';
   if (const char* env = std::getenv("FM_INSERT_ARCS")) arcs_on_ = std::strcmp(env, "0") != 0;
   if (const char* env = std::getenv("FM_INSERT_NEXT_BITS")) next_bits_on_ = std::strcmp(env, "0") != 0;
+  // Arcs of at least this many members replayed from their groups when
+  // canonical (0: never).
+  t_.canon_min = 32;
+  if (const char* env = std::getenv("FM_INSERT_CANON_MIN"))
+    t_.canon_min = static_cast<uint32_t>(std::strtoul(env, nullptr, 10));
   if (const char* env = std::getenv("FM_LEADER_MAX"))
     leader_max_ = static_cast<uint32_t>(std::max(1ul, std::min<unsigned long>(std::strtoul(env, nullptr, 10), dev::kLeaderMax)));
   sort_end_bit_ = 1;

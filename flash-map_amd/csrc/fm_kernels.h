@@ -9,7 +9,9 @@
 //   emptybits[cap/64] u64  bit set <=> slot is EMPTY
 //   owner[cap]      u32   per-slot scratch of inserts: the batch index of
 //                         the key that dry-claimed the slot (read only at
-//                         slots dry-claimed by the same batch, never reset)
+//                         slots dry-claimed by the same batch, never reset;
+//                         replay_arc_canon also keeps its final records in
+//                         the entries of the arc it replays)
 //   rowner[cap]     u32   per-slot scratch of removes (conflict detection),
 //                         0xFFFFFFFF between ops
 //
@@ -91,6 +93,10 @@ struct Table {
   uint32_t* dd_last;
   uint32_t* dd_bucket;
   uint64_t dd_smask;
+  // k_ins_big replays an arc with at least this many members from its
+  // groups of equal homes (replay_arc_canon) when the arc is canonical; 0:
+  // never.
+  uint32_t canon_min;
   // Per-op insert, when set: the bitmaps the map holds after this batch.
   // The dry pass copies freebits into nfree and nq and emptybits into
   // nempty; every real claim then clears its bit in all three (claim_next)
@@ -860,6 +866,7 @@ __device__ uint32_t insert_one_wave(const Table& t, const uint8_t* __restrict__ 
 // every source before it is overwritten.
 // ---------------------------------------------------------------------------
 constexpr uint32_t kArcLdsWords = 28672;       // 112 KiB: flags, value sources and keys of one arc
+constexpr uint32_t kBigPoolWords = 38912;      // 152 KiB: k_ins_big's member list and replay arrays
 constexpr uint32_t kArcBatch = 0x80000000u;    // value source: a batch value, not an arc slot
 
 template <int KS>
@@ -1045,6 +1052,426 @@ __device__ uint32_t replay_arc_lds(const Table& t, const uint8_t* __restrict__ i
     }
   }
   return claimed;
+}
+
+// ---------------------------------------------------------------------------
+// Canonical arcs: an arc's whole batch replay computed from its groups of
+// equal homes.  An arc is canonical when its pre-batch residents lie in
+// packed runs sorted by home: every OCCUPIED slot has dist 0, or follows an
+// OCCUPIED slot whose dist is at least its own minus 1 (insert-only
+// histories leave every arc this way; a tombstone inside a run does not).
+// In a canonical run a sequential insert of key y with home h
+// (rayon_cpu.rs:157-262) walks past the residents of homes <= h (comparing
+// keys where the home is h), takes the place of the first resident of a
+// larger home, and every later group of equal homes up to the run's end
+// moves one slot on with its first member carried to its back (the carried
+// pair passes the rest of its group, whose dists equal its position, and
+// evicts the first resident of the next larger home); the walk ends at F_y,
+// the first free slot at/after h, and the run stays canonical.  So after
+// the batch:
+//  * the arc holds its residents and the first copy of every new key,
+//    sorted by home, at the positions parking gives them
+//    (pos = max(home, previous pos + 1));
+//  * a later copy of a key (of a resident or of an earlier batch key) only
+//    replaces that element's value: the last copy's value stays;
+//  * a group of equal homes h is its residents in slot order, then its new
+//    keys appended in batch order, rotated by one (front to back) for every
+//    insert y with home_y < h < F_y made after it gained its first element
+//    (the F_y are a parking of the first copies in batch order).
+// The wave computes F_y for every first copy in batch order (the arc's
+// occupancy in registers; rotations counted per home as range additions:
+// per slot inside a 64-slot block, per block across blocks), then each
+// group's final order from its rotation counts, and writes every changed
+// slot from the end of the arc down (every element moves towards the end,
+// so a slot's old pair is read before it is overwritten).  The table is not
+// written before the replay is known to be consistent; otherwise the
+// function returns kNotCanon with only owner[] scratch written, and the
+// caller replays the walks.  Called by all 64 lanes of a single-wave
+// workgroup with lst[0..m) the members' batch indices in batch order.
+// LDS after lst: rec[m], fp[max(m, 128)], hb[L], hg[L], srt[m] (16-bit).
+// ---------------------------------------------------------------------------
+constexpr uint32_t kNotCanon = 0xFFFFFFFFu;
+constexpr uint32_t kCanonMaxL = 8192;         // the arc's occupancy: two 64-bit words per lane
+constexpr uint32_t kRecLater = 0x80000000u;   // rec: a later copy of a key
+constexpr uint32_t kRecFlag = 0x8000u;        // rec: first copy: its group needs its rotation count;
+                                              //      later copy: its target is a resident
+__host__ __device__ constexpr uint32_t canon_words(uint32_t m, uint32_t L) {
+  return m + m + (m > 128 ? m : 128) + 2 * L + (m + 1) / 2;
+}
+
+__device__ __forceinline__ uint32_t rd_lane(uint32_t v, uint32_t l) {
+  return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), static_cast<int>(l)));
+}
+__device__ __forceinline__ unsigned long long rd_lane64(unsigned long long v, uint32_t l) {
+  return static_cast<unsigned long long>(rd_lane(static_cast<uint32_t>(v), l)) |
+         (static_cast<unsigned long long>(rd_lane(static_cast<uint32_t>(v >> 32), l)) << 32);
+}
+__device__ __forceinline__ uint32_t wave_incl_add(uint32_t v) {
+  const uint32_t lane = threadIdx.x & 63u;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t y = static_cast<uint32_t>(__shfl_up(static_cast<int>(v), off, 64));
+    if (lane >= static_cast<uint32_t>(off)) v += y;
+  }
+  return v;
+}
+__device__ __forceinline__ int wave_incl_max(int v) {
+  const uint32_t lane = threadIdx.x & 63u;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int y = __shfl_up(v, off, 64);
+    if (lane >= static_cast<uint32_t>(off)) v = max(v, y);
+  }
+  return v;
+}
+
+// Position of the first of `n` elements of home h (lane h of a 64-home
+// chunk) when the elements of every home are parked in home order:
+// elements before home h number s (exclusive sum), and the first one sits
+// at s + max over nonempty homes h' <= h of (h' - s(h')).  `sum` and `mx`
+// carry the chunk scans over.
+__device__ __forceinline__ uint32_t park_first(uint32_t h, uint32_t n, uint32_t& sum, int& mx) {
+  const uint32_t incl = wave_incl_add(n);
+  const uint32_t s = sum + incl - n;
+  const int m = max(wave_incl_max(n ? static_cast<int>(h) - static_cast<int>(s) : INT_MIN), mx);
+  sum += rd_lane(incl, 63);
+  mx = rd_lane(static_cast<uint32_t>(m), 63);
+  return n ? s + static_cast<uint32_t>(m) : 0u;
+}
+
+template <int KS>
+__device__ __forceinline__ uint32_t key_fp(const Blob<KS>& k) {
+  uint32_t f = 0x9E3779B9u;
+#pragma unroll
+  for (int w = 0; w < KS / 4; ++w) {
+    f = (f ^ k.w[w]) * 0x85EBCA6Bu;
+    f ^= f >> 13;
+  }
+  return f;
+}
+
+// Rotations of a deque of `size` elements by r (front to back): index i -> (i - r) mod size.
+__device__ __forceinline__ uint32_t rot_index(uint32_t i, uint32_t r, uint32_t size) {
+  const uint32_t k = r % size;
+  return i >= k ? i - k : i + size - k;
+}
+
+template <int KS, int VS>
+__device__ uint32_t replay_arc_canon(const Table& t, const uint8_t* __restrict__ in_keys,
+                                     const uint8_t* __restrict__ in_vals, uint64_t r0, uint32_t L,
+                                     const uint32_t* lst, uint32_t m, uint32_t* pool) {
+  static_assert(KS % 4 == 0 && VS % 4 == 0 && KS > 0 && VS > 0, "static word sizes");
+  const uint32_t lane = threadIdx.x & 63u;
+  uint32_t* rec = pool + m;
+  uint32_t* fp = rec + m;
+  uint32_t* hb = fp + (m > 128 ? m : 128);  // residents (high 16) | members' cursor (low 16)
+  uint32_t* hg = hb + L;                    // elements, then first position (high 16) | rotations (low 16)
+  uint16_t* srt = reinterpret_cast<uint16_t*>(hg + L);
+  uint16_t* rec16 = reinterpret_cast<uint16_t*>(rec);
+  const uint32_t nch = (L + 63) >> 6;
+  for (uint32_t h = lane; h < L; h += 64) {
+    hb[h] = 0;
+    hg[h] = 0;
+  }
+  wave_sync();
+  // 1. The arc's flags: canonical check, residents per home, occupancy.
+  unsigned long long occ0 = ~0ull, occ1 = ~0ull;  // lane l: words l and l + 64 (bit set: occupied or past the arc)
+  bool bad = false;
+  uint32_t prevf = 0;  // the flag before the chunk (the barrier before the arc: free)
+  for (uint32_t c = 0; c < nch; ++c) {
+    const uint32_t q = (c << 6) + lane;
+    const bool in = q < L;
+    const uint32_t f = in ? t.flags[(r0 + q) & t.mask] : 0u;
+    uint32_t pf = static_cast<uint32_t>(__shfl_up(static_cast<int>(f), 1, 64));
+    if (lane == 0) pf = prevf;
+    const bool occ = in && (f & 0xFu) == kOccupied;
+    const uint32_t d = f >> 4;
+    if (occ) {
+      const bool pocc = (pf & 0xFu) == kOccupied;
+      if (d > q || (d > 0 && !(pocc && (pf >> 4) + 1 >= d))) bad = true;
+      else atomicAdd(&hb[q - d], 0x10000u);
+    }
+    const unsigned long long word = __ballot(occ || !in);
+    if (c < 64) {
+      if (lane == c) occ0 = word;
+    } else if (lane == c - 64) {
+      occ1 = word;
+    }
+    prevf = rd_lane(f, 63);
+  }
+  if (__ballot(bad)) return kNotCanon;
+  // 2. Members: homes (relative to the arc), fingerprints, counts per home.
+  for (uint32_t j = lane; j < m; j += 64) {
+    Blob<KS> k;
+    k.load(in_keys + static_cast<uint64_t>(lst[j]) * KS, KS);
+    const uint64_t h = ((k.hash(KS, t.mode) & t.mask) - r0) & t.mask;
+    if (h >= L) {
+      bad = true;
+    } else {
+      rec[j] = static_cast<uint32_t>(h);
+      fp[j] = key_fp(k);
+      atomicAdd(&hb[h], 1u);
+    }
+  }
+  if (__ballot(bad)) return kNotCanon;
+  wave_sync();
+  // 3. Members sorted by home, batch order kept (srt: batch ranks); after
+  //    it hb's low half is the end of each home's range.
+  {
+    uint32_t sum = 0;
+    for (uint32_t c = 0; c < nch; ++c) {
+      const uint32_t h = (c << 6) + lane;
+      const uint32_t w = h < L ? hb[h] : 0u;
+      const uint32_t n = w & 0xFFFFu;
+      const uint32_t incl = wave_incl_add(n);
+      if (h < L) hb[h] = (w & 0xFFFF0000u) | (sum + incl - n);
+      sum += rd_lane(incl, 63);
+    }
+  }
+  wave_sync();
+  for (uint32_t base = 0; base < m; base += 64) {
+    const uint32_t j = base + lane;
+    const bool in = j < m;
+    const uint32_t h = in ? (rec[j] & 0x7FFFu) : 0xFFFFu;
+    const uint32_t cur = in ? (hb[h] & 0xFFFFu) : 0u;
+    uint32_t lower = 0;
+    for (uint32_t o = 0; o < 63; ++o) lower += (o < lane && rd_lane(h, o) == h) ? 1u : 0u;
+    wave_sync();
+    if (in) {
+      srt[cur + lower] = static_cast<uint16_t>(j);
+      atomicAdd(&hb[h], 1u);
+    }
+    wave_sync();
+  }
+  // 4. Later copies (one lane per home): each member's key against the
+  //    group's residents and earlier first copies; the group's element
+  //    count; first copies of groups of two or more elements need their
+  //    rotation counts.
+  {
+    uint32_t rsum = 0;
+    int rmx = INT_MIN;
+    for (uint32_t c = 0; c < nch; ++c) {
+      const uint32_t h = (c << 6) + lane;
+      const bool in = h < L;
+      const uint32_t w = in ? hb[h] : 0u;
+      const uint32_t rc = w >> 16;
+      const uint32_t rf = park_first(h, rc, rsum, rmx);
+      const uint32_t mend = w & 0xFFFFu, mbeg = (in && h > 0) ? (hb[h - 1] & 0xFFFFu) : 0u;
+      uint32_t firsts = 0;
+      for (uint32_t y = mbeg; y < mend; ++y) {
+        const uint32_t j = srt[y];
+        const uint32_t fj = fp[j];
+        uint32_t tgt = kNoOwner;
+        bool tres = false;
+        if (rc > 0) {
+          Blob<KS> kj;
+          kj.load(in_keys + static_cast<uint64_t>(lst[j]) * KS, KS);
+          for (uint32_t i = 0; i < rc && tgt == kNoOwner; ++i)
+            if (kj.eq(t.keys + ((r0 + rf + i) & t.mask) * KS, KS)) {
+              tgt = rf + i;
+              tres = true;
+            }
+        }
+        for (uint32_t y2 = mbeg; y2 < y && tgt == kNoOwner; ++y2) {
+          const uint32_t j2 = srt[y2];
+          if ((rec[j2] & kRecLater) || fp[j2] != fj) continue;
+          Blob<KS> kj;
+          kj.load(in_keys + static_cast<uint64_t>(lst[j]) * KS, KS);
+          if (kj.eq(in_keys + static_cast<uint64_t>(lst[j2]) * KS, KS)) tgt = j2;
+        }
+        if (tgt != kNoOwner) rec[j] |= kRecLater | (tres ? kRecFlag : 0u) | (tgt << 16);
+        else ++firsts;
+      }
+      if (in) hg[h] = (rc + firsts) << 16;
+      if (rc + firsts >= 2)
+        for (uint32_t y = mbeg; y < mend; ++y) {
+          const uint32_t j = srt[y];
+          if (!(rec[j] & kRecLater)) rec[j] |= kRecFlag;
+        }
+    }
+  }
+  wave_sync();
+  // 5. First copies in batch order: F (parking), rotation counts.
+  uint32_t blk0 = 0, blk1 = 0;  // rotations of 64-slot blocks lane and lane + 64
+  for (uint32_t base = 0; base < m && !bad; base += 64) {
+    const uint32_t jl = base + lane;
+    const uint32_t recl = jl < m ? rec[jl] : kRecLater;
+    const uint32_t cnt = min(64u, m - base);
+    for (uint32_t u = 0; u < cnt; ++u) {
+      const uint32_t r = rd_lane(recl, u);
+      if (r & kRecLater) continue;
+      const uint32_t h = r & 0x7FFFu;
+      const uint32_t w = h >> 6;
+      const unsigned long long word = w < 64 ? rd_lane64(occ0, w) : rd_lane64(occ1, w - 64);
+      const unsigned long long fr = ~word & (~0ull << (h & 63));
+      uint32_t F;
+      if (fr) {
+        F = (w << 6) + static_cast<uint32_t>(__ffsll(fr) - 1);
+      } else {
+        const unsigned long long f0 = lane > w ? ~occ0 : 0ull;
+        const unsigned long long f1 = lane + 64 > w ? ~occ1 : 0ull;
+        const unsigned long long b0 = __ballot(f0 != 0);
+        if (b0) {
+          const uint32_t l = static_cast<uint32_t>(__ffsll(b0) - 1);
+          F = (l << 6) + static_cast<uint32_t>(__ffsll(rd_lane64(f0, l)) - 1);
+        } else {
+          const unsigned long long b1 = __ballot(f1 != 0);
+          if (!b1) {
+            bad = true;
+            break;
+          }
+          const uint32_t l = static_cast<uint32_t>(__ffsll(b1) - 1);
+          F = ((l + 64) << 6) + static_cast<uint32_t>(__ffsll(rd_lane64(f1, l)) - 1);
+        }
+      }
+      const uint32_t fw = F >> 6;
+      if (lane == (fw & 63u)) {
+        if (fw < 64) occ0 |= 1ull << (F & 63);
+        else occ1 |= 1ull << (F & 63);
+      }
+      const uint32_t lo = h + 1, hi = F;  // groups this insert rotates: homes lo .. hi-1
+      if (lo < hi) {
+        const uint32_t bl = lo >> 6, bh = hi >> 6;
+        const uint32_t x = (bl << 6) + lane;
+        if (bl == bh) {
+          if (x >= lo && x < hi) atomicAdd(&hg[x], 1u);
+        } else {
+          if (x >= lo) atomicAdd(&hg[x], 1u);
+          const uint32_t x2 = (bh << 6) + lane;
+          if (x2 < hi) atomicAdd(&hg[x2], 1u);
+          blk0 += (lane > bl && lane < bh) ? 1u : 0u;
+          blk1 += (lane + 64 > bl && lane + 64 < bh) ? 1u : 0u;
+        }
+      }
+      if (r & kRecFlag) {  // the rotations its group had before it
+        const uint32_t b = h >> 6;
+        const uint32_t cnt_h = (hg[h] & 0xFFFFu) + (b < 64 ? rd_lane(blk0, b) : rd_lane(blk1, b - 64));
+        if (lane == 0) rec16[2 * (base + u) + 1] = static_cast<uint16_t>(cnt_h);
+      }
+    }
+  }
+  if (bad) return kNotCanon;  // (wave-uniform)
+  fp[lane] = blk0;  // the fingerprints are done with
+  fp[lane + 64] = blk1;
+  wave_sync();
+  // 6. Each home's first final position (parking of the final groups).
+  {
+    uint32_t sum = 0;
+    int mx = INT_MIN;
+    for (uint32_t c = 0; c < nch; ++c) {
+      const uint32_t h = (c << 6) + lane;
+      const uint32_t w = h < L ? hg[h] : 0u;
+      const uint32_t p = park_first(h, w >> 16, sum, mx);
+      if (h < L) hg[h] = (p << 16) | (w & 0xFFFFu);
+    }
+  }
+  for (uint32_t q = lane; q < L; q += 64) t.owner[(r0 + q) & t.mask] = kNoOwner;
+  __threadfence_block();
+  wave_sync();
+  // 7. Each group's final order (one lane per home): a record per element
+  //    at its final slot in owner[]: bit 31 set: key and value of batch rank
+  //    (bits 0-14, the element's last copy), else the resident at slot
+  //    (bits 0-14) unchanged; bits 16-30 the home.
+  uint32_t claimed = 0;
+  {
+    uint32_t rsum = 0;
+    int rmx = INT_MIN;
+    for (uint32_t c = 0; c < nch; ++c) {
+      const uint32_t h = (c << 6) + lane;
+      const bool in = h < L;
+      const uint32_t w = in ? hb[h] : 0u;
+      const uint32_t rc = w >> 16;
+      const uint32_t rf = park_first(h, rc, rsum, rmx);
+      const uint32_t mend = w & 0xFFFFu, mbeg = (in && h > 0) ? (hb[h - 1] & 0xFFFFu) : 0u;
+      if (!in) continue;
+      const uint32_t g = hg[h];
+      const uint32_t gpos = g >> 16;
+      const uint32_t cfin = (g & 0xFFFFu) + fp[h >> 6];
+      uint32_t nf = 0;
+      for (uint32_t y = mbeg; y < mend; ++y) nf += (rec[srt[y]] & kRecLater) ? 0u : 1u;
+      claimed += nf;
+      for (uint32_t e = 0; e < rc + nf; ++e) {
+        // element e: resident e (e < rc) or the (e - rc)-th first copy
+        uint32_t idx, size, cprev = 0, ystart = mbeg, id = 0;
+        if (e < rc) {
+          idx = e;
+          size = rc;
+          id = rf + e;
+        } else {
+          uint32_t k = e - rc, y = mbeg;
+          for (;; ++y) {
+            if (rec[srt[y]] & kRecLater) continue;
+            if (k == 0) break;
+            --k;
+          }
+          id = srt[y];
+          idx = e;
+          size = e + 1;
+          cprev = (rec[id] >> 16) & 0x7FFFu;
+          ystart = y + 1;
+        }
+        uint32_t vs = kNoOwner;
+        for (uint32_t y = mbeg; y < mend; ++y) {
+          const uint32_t j = srt[y];
+          const uint32_t r = rec[j];
+          if (r & kRecLater) {
+            if (((r >> 16) & 0x7FFFu) == id && ((r & kRecFlag) != 0) == (e < rc)) vs = j;
+          } else if (y >= ystart) {
+            const uint32_t cj = (r >> 16) & 0x7FFFu;
+            idx = rot_index(idx, cj - cprev, size);
+            cprev = cj;
+            ++size;
+          }
+        }
+        idx = rot_index(idx, cfin - cprev, size);
+        const uint32_t pos = gpos + idx;
+        if (pos >= L) {
+          bad = true;
+          break;
+        }
+        const uint32_t recd = (vs != kNoOwner) ? (0x80000000u | vs) : (e < rc ? id : (0x80000000u | id));
+        t.owner[(r0 + pos) & t.mask] = recd | (h << 16);
+      }
+    }
+  }
+  if (__ballot(bad)) return kNotCanon;
+  __threadfence_block();  // the records, read back by other lanes
+  wave_sync();
+  // 8. Write-back from the end of the arc down, 64 slots at a time.
+  for (int64_t base = static_cast<int64_t>((L - 1) & ~63u); base >= 0; base -= 64) {
+    const uint32_t q = static_cast<uint32_t>(base) + lane;
+    const uint64_t slot = (r0 + q) & t.mask;
+    const uint32_t rd = q < L ? t.owner[slot] : kNoOwner;
+    const uint32_t id = rd & 0x7FFFu;
+    const bool batch = (rd & 0x80000000u) != 0;
+    const bool put = rd != kNoOwner && (batch || id != q);
+    Blob<KS> k;
+    Blob<VS> v;
+    if (put) {
+      if (batch) {
+        const uint64_t bi = lst[id];
+        k.load(in_keys + bi * KS, KS);
+        v.load(in_vals + dd_value_index(t, bi) * VS, VS);
+      } else {
+        const uint64_t os = (r0 + id) & t.mask;
+        k.load(t.keys + os * KS, KS);
+        v.load(t.vals + os * VS, VS);
+      }
+    }
+    wave_sync();  // every lane's source read before any write of this chunk
+    if (put) {
+      t.flags[slot] = make_flag(kOccupied, q - ((rd >> 16) & 0x7FFFu));
+      k.store(t.keys + slot * KS, KS);
+      v.store(t.vals + slot * VS, VS);
+    }
+  }
+  if (t.outcome)
+    for (uint32_t j = lane; j < m; j += 64) t.outcome[lst[j]] = (rec[j] & kRecLater) ? 0 : 1;
+  // (the sum over lanes)
+  uint32_t tot = claimed;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) tot += static_cast<uint32_t>(__shfl_xor(static_cast<int>(tot), off, 64));
+  return tot;
 }
 
 // A one-member closure's insert decided from its flags.  Key i alone in its
@@ -1729,15 +2156,19 @@ k_ins_leader(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __rest
 // (the next bit still set in Q).  The members' batch indices are gathered
 // from owner[] at the arc's dry-claimed slots (P & ~Q), sorted (bitonic, in
 // windows of `lcap` indices when there are more), and inserted in batch
-// order, each walked by the whole wave (static sizes) or by lane 0.  With
-// kPublish the arc's dry claims are settled afterwards: a slot that ended
-// OCCUPIED leaves both bitmaps, any other gets its Q bit back, and its
-// owner[] entry is reset.  Returns the free slots claimed (in every lane).
-template <int KS, int VS, bool kPublish>
+// order: a canonical arc of at least t.canon_min members from its groups
+// (replay_arc_canon), an arc that fits in LDS by wave scans
+// (replay_arc_lds), any other by walks of the whole wave (static sizes) or
+// of lane 0.  With static sizes `lst` starts a pool of `pool_words` LDS
+// words that the first two use.  The arc's real claims are cleared in the
+// next bitmaps (Table::nfree) when the map keeps them.  Returns the free
+// slots claimed (in every lane).
+template <int KS, int VS>
 __device__ uint32_t replay_arc(const Table& t, const uint8_t* __restrict__ in_keys,
                                const uint8_t* __restrict__ in_vals, uint64_t n, unsigned long long* Q, uint64_t s,
                                uint32_t* lst, uint32_t lcap, uint32_t* carry, uint32_t* cnt, Counters* ctr,
-                               uint32_t* arc_lds = nullptr) {
+                               uint32_t pool_words) {
+  constexpr bool kStatic = KS > 0 && VS > 0 && KS % 4 == 0 && VS % 4 == 0;
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t K = KS ? KS : t.K;
   const uint32_t V = VS ? VS : t.V;
@@ -1749,6 +2180,18 @@ __device__ uint32_t replay_arc(const Table& t, const uint8_t* __restrict__ in_ke
   const uint64_t arc = (x + t.cap - s) & t.mask;  // slots s .. x-1
   const uint64_t w0 = s >> 6;
   const uint64_t nwords = min(((s & 63) + arc + 63) >> 6, words);
+  // The whole arc runs from the barrier before s (b) to x; a canonical
+  // replay gathers all of its members at once (there are at most Lb).
+  uint64_t b = s, Lb = 0;
+  if (kStatic && pool_words) {
+    long long bb = 0;
+    if (lane == 0) bb = static_cast<long long>(bm_prev_set(P, s, t.cap));
+    b = static_cast<uint64_t>(__shfl(bb, 0, 64));
+    Lb = (x - b - 1) & t.mask;
+  }
+  const bool canon = kStatic && pool_words && t.canon_min && b != s && Lb <= kCanonMaxL && 2 * Lb <= pool_words;
+  const uint32_t lcap0 = lcap;
+  if (canon) lcap = static_cast<uint32_t>(Lb);
   uint32_t claimed = 0;
   uint64_t lo = 0, hi = ~0ull;
   bool windowed = false;
@@ -1759,9 +2202,9 @@ __device__ uint32_t replay_arc(const Table& t, const uint8_t* __restrict__ in_ke
       const uint64_t w = (w0 + wi) % words;
       unsigned long long bits = P[w] & ~load_word(&Q[w]);
       while (bits) {
-        const int b = __ffsll(bits) - 1;
+        const int bt = __ffsll(bits) - 1;
         bits &= bits - 1;
-        const uint64_t p = (w << 6) + b;
+        const uint64_t p = (w << 6) + bt;
         if (((p + t.cap - s) & t.mask) >= arc) continue;
         const uint32_t idx = __hip_atomic_load(&t.owner[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (idx < lo || idx >= hi) continue;
@@ -1796,15 +2239,23 @@ __device__ uint32_t replay_arc(const Table& t, const uint8_t* __restrict__ in_ke
         wave_sync();
       }
     }
-    if constexpr (KS > 0 && VS > 0 && KS % 4 == 0 && VS % 4 == 0) {
-      // The whole arc (from the barrier before s to x) in LDS when it fits.
-      long long bb = 0;
-      if (arc_lds && !windowed && lane == 0) bb = static_cast<long long>(bm_prev_set(P, s, t.cap));
-      const uint64_t b = static_cast<uint64_t>(__shfl(bb, 0, 64));
-      const uint64_t L = (x - b - 1) & t.mask;
-      if (arc_lds && !windowed && b != s && L <= arc_lds_cap<KS>()) {
-        claimed += replay_arc_lds<KS, VS>(t, in_keys, in_vals, (b + 1) & t.mask, static_cast<uint32_t>(L), lst, m,
-                                          arc_lds, ctr);
+    if constexpr (kStatic) {
+      bool done = false;
+      if (canon && !windowed && m >= t.canon_min && canon_words(m, static_cast<uint32_t>(Lb)) <= pool_words) {
+        const uint32_t c = replay_arc_canon<KS, VS>(t, in_keys, in_vals, (b + 1) & t.mask, static_cast<uint32_t>(Lb),
+                                                    lst, m, lst);
+        wave_sync();
+        if (c != kNotCanon) {
+          claimed += c;
+          done = true;
+        }
+      }
+      // The arc in LDS when it fits (after the member list).
+      const uint32_t arc_off = max(lcap0, (m + 3u) & ~3u);
+      if (done) {
+      } else if (pool_words && !windowed && b != s && Lb <= arc_lds_cap<KS>() && arc_off + kArcLdsWords <= pool_words) {
+        claimed += replay_arc_lds<KS, VS>(t, in_keys, in_vals, (b + 1) & t.mask, static_cast<uint32_t>(Lb), lst, m,
+                                          lst + arc_off, ctr);
       } else {
         for (uint32_t j = 0; j < m; ++j) {
           const uint64_t k = lst[j];
@@ -1832,39 +2283,18 @@ __device__ uint32_t replay_arc(const Table& t, const uint8_t* __restrict__ in_ke
     lo = hi;
     hi = lo + lcap;
   }
-  if constexpr (kPublish) {
-    __threadfence_block();  // the walks' flag stores, seen by every lane
-    for (uint64_t wi = lane; wi < nwords; wi += 64) {
-      const uint64_t w = (w0 + wi) % words;
-      unsigned long long bits = P[w] & ~load_word(&Q[w]);
-      unsigned long long taken = 0, back = 0;
-      while (bits) {
-        const int b = __ffsll(bits) - 1;
-        bits &= bits - 1;
-        const uint64_t p = (w << 6) + b;
-        if (((p + t.cap - s) & t.mask) >= arc) continue;
-        if ((t.flags[p] & 0xFu) == kOccupied) taken |= 1ull << b;
-        else back |= 1ull << b;
-        __hip_atomic_store(&t.owner[p], kNoOwner, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      }
-      if (taken) {
-        atomicAnd(&t.freebits[w], ~taken);
-        atomicAnd(&t.emptybits[w], ~taken);
-      }
-      if (back) atomicOr(&Q[w], back);
-    }
-  } else if (t.nfree) {  // the arc's real claims, in the next bitmaps
-    __threadfence_block();  // the walks' flag stores, seen by every lane
+  if (t.nfree) {  // the arc's real claims, in the next bitmaps
+    __threadfence_block();  // the replay's flag stores, seen by every lane
     for (uint64_t wi = lane; wi < nwords; wi += 64) {
       const uint64_t w = (w0 + wi) % words;
       unsigned long long bits = P[w] & ~load_word(&Q[w]);
       unsigned long long taken = 0;
       while (bits) {
-        const int b = __ffsll(bits) - 1;
+        const int bt = __ffsll(bits) - 1;
         bits &= bits - 1;
-        const uint64_t p = (w << 6) + b;
+        const uint64_t p = (w << 6) + bt;
         if (((p + t.cap - s) & t.mask) >= arc) continue;
-        if ((t.flags[p] & 0xFu) == kOccupied) taken |= 1ull << b;
+        if ((t.flags[p] & 0xFu) == kOccupied) taken |= 1ull << bt;
       }
       if (taken) {
         atomicAnd(&t.nfree[w], ~taken);
@@ -1885,10 +2315,11 @@ __global__ void __launch_bounds__(64)
 k_ins_big(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __restrict__ in_vals, uint64_t n,
           const unsigned long long* __restrict__ Q, const uint32_t* __restrict__ big, Counters* ctr,
           Counters* host_out, uint32_t seq) {
-  __shared__ uint32_t lst[kBigMax];
+  // static sizes: the member list, then the LDS replays' arrays
+  constexpr uint32_t kPool = (KS > 0 && VS > 0 && KS % 4 == 0 && VS % 4 == 0) ? kBigPoolWords : 0u;
+  __shared__ uint32_t lst[kPool ? kPool : kBigMax];
   __shared__ uint32_t carry[2 * kMaxGenericSize / 4];
   __shared__ uint32_t cnt;
-  __shared__ uint32_t arc[(KS > 0 && VS > 0 && KS % 4 == 0 && VS % 4 == 0) ? kArcLdsWords : 1];
   __shared__ uint32_t cur;
   const uint32_t nbig = __hip_atomic_load(&ctr->nbig, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   // Closures are handed out one at a time (ctr->nsmall is the ticket; their
@@ -1901,8 +2332,8 @@ k_ins_big(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __restric
     const uint32_t c = cur;
     __syncthreads();
     if (c >= nbig) break;
-    const uint32_t claimed = replay_arc<KS, VS, false>(t, in_keys, in_vals, n, const_cast<unsigned long long*>(Q),
-                                                       big[c], lst, kBigMax, carry, &cnt, ctr, arc);
+    const uint32_t claimed = replay_arc<KS, VS>(t, in_keys, in_vals, n, const_cast<unsigned long long*>(Q), big[c],
+                                                lst, kBigMax, carry, &cnt, ctr, kPool);
     if (threadIdx.x == 0 && claimed) atomicAdd(&ctr->count, static_cast<unsigned long long>(claimed));
   }
   // Tickets from the workgroups that took closures only (tickets on one

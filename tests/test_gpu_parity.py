@@ -471,6 +471,48 @@ def test_gpu_long_arcs_in_lds_and_hbm(K, V):
     apply_and_compare(m, o, ops)
 
 
+@pytest.mark.parametrize("canon_min", ["1", "0"])
+@pytest.mark.parametrize("K,V", [(8, 8), (32, 128)])
+def test_gpu_canonical_arc_replay(K, V, canon_min, monkeypatch):
+    """Arcs replayed from their groups of equal homes (canon_min 1: every
+    canonical arc k_ins_big takes; 0: none, the walk replays) against the
+    oracle.  Identity-hashed keys put many keys on few homes, so the arcs are
+    long and their groups large: residents updated by the batch, keys
+    repeated inside a batch (the last value stays), and groups rotated by
+    the inserts before them.  The removes leave tombstones inside runs, and
+    the arcs that hold them are not canonical (walk replay either way)."""
+    say(f"canonical arc replay K={K} V={V} canon_min={canon_min}")
+    monkeypatch.setenv("FM_INSERT_CANON_MIN", canon_min)  # read when a map is built
+    rng = np.random.default_rng(4242 + K)
+    cap = 1 << 14
+
+    def keys_on(homes):
+        ids = rng.permutation(1 << 20)[:len(homes)].astype(np.uint64) + np.uint64(1)
+        first = np.asarray(homes, dtype=np.uint64) | (ids << np.uint64(14))
+        k = rng.integers(0, 256, (len(homes), K), dtype=np.uint8)
+        k[:, :8] = first.view(np.uint8).reshape(-1, 8)[:, :min(8, K)] if K >= 8 else k[:, :8]
+        return k
+
+    def vals(n):
+        return rng.integers(0, 256, (n, V), dtype=np.uint8)
+
+    a = keys_on(rng.integers(0, 12000, 9000))
+    b_new = keys_on(rng.integers(5000, 6000, 1500))
+    b = np.concatenate([b_new, a[rng.integers(0, 9000, 1500)], b_new[rng.integers(0, 1500, 1000)]])
+    b = b[rng.permutation(len(b))]
+    c_new = keys_on(rng.integers(0, cap, 1200))
+    c = np.concatenate([c_new, a[:1500:2], a[rng.integers(0, 9000, 600)], c_new[:300]])
+    c = c[rng.permutation(len(c))]
+    m, o = gpu_make(cap, K, V), OracleAdapter(cap, K, V)
+    ops = [("ins", a, vals(len(a))),
+           ("ins", b, vals(len(b))),
+           ("ins", keys_on(rng.integers(5500, 5600, 40)), vals(40)),   # sparse: leader-queued closures
+           ("rem", a[:3000:2]),
+           ("ins", c, vals(len(c))),
+           ("get", np.concatenate([a, b, c]))]
+    apply_and_compare(m, o, ops)
+
+
 @pytest.mark.parametrize("seed", [0, 1])
 def test_gpu_bitmap_rebuild_and_per_key_batches_alternate(seed):
     """Insert batches alternate across the size where the bits pass switches
