@@ -1096,7 +1096,7 @@ constexpr uint32_t kRecLater = 0x80000000u;   // rec: a later copy of a key
 constexpr uint32_t kRecFlag = 0x8000u;        // rec: first copy: its group needs its rotation count;
                                               //      later copy: its target is a resident
 __host__ __device__ constexpr uint32_t canon_words(uint32_t m, uint32_t L) {
-  return m + m + (m > 128 ? m : 128) + 2 * L + (m + 1) / 2;
+  return m + m + (m > 128 ? m : 128) + L + (L + 1) + (m + 1) / 2 + (L + 1) / 2;
 }
 
 __device__ __forceinline__ uint32_t rd_lane(uint32_t v, uint32_t l) {
@@ -1159,18 +1159,24 @@ __device__ __forceinline__ uint32_t rot_index(uint32_t i, uint32_t r, uint32_t s
 template <int KS, int VS>
 __device__ uint32_t replay_arc_canon(const Table& t, const uint8_t* __restrict__ in_keys,
                                      const uint8_t* __restrict__ in_vals, uint64_t r0, uint32_t L,
-                                     const uint32_t* lst, uint32_t m, uint32_t* pool) {
+                                     const uint32_t* lst, uint32_t m_in, uint32_t* pool) {
   static_assert(KS % 4 == 0 && VS % 4 == 0 && KS > 0 && VS > 0, "static word sizes");
   const uint32_t lane = threadIdx.x & 63u;
+  // (wave-uniform: held in scalar registers, so the loops below are uniform)
+  const uint32_t m = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(m_in)));
+  L = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(L)));
+  r0 = static_cast<uint64_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(r0))) |
+       (static_cast<uint64_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(r0 >> 32))) << 32);
   uint32_t* rec = pool + m;
   uint32_t* fp = rec + m;
   uint32_t* hb = fp + (m > 128 ? m : 128);  // residents (high 16) | members' cursor (low 16)
-  uint32_t* hg = hb + L;                    // elements, then first position (high 16) | rotations (low 16)
-  uint16_t* srt = reinterpret_cast<uint16_t*>(hg + L);
+  uint32_t* hg = hb + L;                    // Fenwick tree over homes 1..L: walks (low 16) | ends (high 16)
+  uint16_t* srt = reinterpret_cast<uint16_t*>(hg + L + 1);
+  uint16_t* fcn = srt + ((m + 1) & ~1u);    // per home: elements, then the first one's final position
   uint16_t* rec16 = reinterpret_cast<uint16_t*>(rec);
   const uint32_t nch = (L + 63) >> 6;
-  for (uint32_t h = lane; h < L; h += 64) {
-    hb[h] = 0;
+  for (uint32_t h = lane; h <= L; h += 64) {
+    if (h < L) hb[h] = 0;
     hg[h] = 0;
   }
   wave_sync();
@@ -1215,8 +1221,8 @@ __device__ uint32_t replay_arc_canon(const Table& t, const uint8_t* __restrict__
   }
   if (__ballot(bad)) return kNotCanon;
   wave_sync();
-  // 3. Members sorted by home, batch order kept (srt: batch ranks); after
-  //    it hb's low half is the end of each home's range.
+  // 3. Members by home (srt: batch ranks); after it hb's low half is the
+  //    end of each home's range.
   {
     uint32_t sum = 0;
     for (uint32_t c = 0; c < nch; ++c) {
@@ -1229,24 +1235,13 @@ __device__ uint32_t replay_arc_canon(const Table& t, const uint8_t* __restrict__
     }
   }
   wave_sync();
-  for (uint32_t base = 0; base < m; base += 64) {
-    const uint32_t j = base + lane;
-    const bool in = j < m;
-    const uint32_t h = in ? (rec[j] & 0x7FFFu) : 0xFFFFu;
-    const uint32_t cur = in ? (hb[h] & 0xFFFFu) : 0u;
-    uint32_t lower = 0;
-    for (uint32_t o = 0; o < 63; ++o) lower += (o < lane && rd_lane(h, o) == h) ? 1u : 0u;
-    wave_sync();
-    if (in) {
-      srt[cur + lower] = static_cast<uint16_t>(j);
-      atomicAdd(&hb[h], 1u);
-    }
-    wave_sync();
-  }
-  // 4. Later copies (one lane per home): each member's key against the
-  //    group's residents and earlier first copies; the group's element
-  //    count; first copies of groups of two or more elements need their
-  //    rotation counts.
+  for (uint32_t j = lane; j < m; j += 64)  // (a home's range in any order; sorted below)
+    srt[atomicAdd(&hb[rec[j] & 0x7FFFu], 1u) & 0xFFFFu] = static_cast<uint16_t>(j);
+  wave_sync();
+  // 4. Later copies (one lane per home): the home's members put in batch
+  //    order, each member's key against the group's residents and earlier
+  //    first copies; the group's element count; first copies of groups of
+  //    two or more elements need their rotation counts.
   {
     uint32_t rsum = 0;
     int rmx = INT_MIN;
@@ -1257,6 +1252,15 @@ __device__ uint32_t replay_arc_canon(const Table& t, const uint8_t* __restrict__
       const uint32_t rc = w >> 16;
       const uint32_t rf = park_first(h, rc, rsum, rmx);
       const uint32_t mend = w & 0xFFFFu, mbeg = (in && h > 0) ? (hb[h - 1] & 0xFFFFu) : 0u;
+      for (uint32_t i = mbeg + 1; i < mend; ++i) {  // the home's members in batch order
+        const uint16_t v = srt[i];
+        uint32_t k = i;
+        while (k > mbeg && srt[k - 1] > v) {
+          srt[k] = srt[k - 1];
+          --k;
+        }
+        srt[k] = v;
+      }
       uint32_t firsts = 0;
       for (uint32_t y = mbeg; y < mend; ++y) {
         const uint32_t j = srt[y];
@@ -1282,7 +1286,7 @@ __device__ uint32_t replay_arc_canon(const Table& t, const uint8_t* __restrict__
         if (tgt != kNoOwner) rec[j] |= kRecLater | (tres ? kRecFlag : 0u) | (tgt << 16);
         else ++firsts;
       }
-      if (in) hg[h] = (rc + firsts) << 16;
+      if (in) fcn[h] = static_cast<uint16_t>(rc + firsts);
       if (rc + firsts >= 2)
         for (uint32_t y = mbeg; y < mend; ++y) {
           const uint32_t j = srt[y];
@@ -1291,78 +1295,97 @@ __device__ uint32_t replay_arc_canon(const Table& t, const uint8_t* __restrict__
     }
   }
   wave_sync();
-  // 5. First copies in batch order: F (parking), rotation counts.
-  uint32_t blk0 = 0, blk1 = 0;  // rotations of 64-slot blocks lane and lane + 64
-  for (uint32_t base = 0; base < m && !bad; base += 64) {
+  // 5. F of every first copy: its walk's end, a parking of the first
+  //    copies in batch order (one at a time; the arc's occupancy in
+  //    registers), kept in rec's high half.
+  bool lost = false;  // (wave-uniform)
+  for (uint32_t base = 0; base < m; base += 64) {
     const uint32_t jl = base + lane;
     const uint32_t recl = jl < m ? rec[jl] : kRecLater;
     const uint32_t cnt = min(64u, m - base);
+    uint32_t Fv = 0;
+    uint32_t r = rd_lane(recl, 0);  // (the chunk's records are in registers before the steps)
     for (uint32_t u = 0; u < cnt; ++u) {
-      const uint32_t r = rd_lane(recl, u);
-      if (r & kRecLater) continue;
-      const uint32_t h = r & 0x7FFFu;
-      const uint32_t w = h >> 6;
-      const unsigned long long word = w < 64 ? rd_lane64(occ0, w) : rd_lane64(occ1, w - 64);
-      const unsigned long long fr = ~word & (~0ull << (h & 63));
-      uint32_t F;
-      if (fr) {
-        F = (w << 6) + static_cast<uint32_t>(__ffsll(fr) - 1);
-      } else {
-        const unsigned long long f0 = lane > w ? ~occ0 : 0ull;
-        const unsigned long long f1 = lane + 64 > w ? ~occ1 : 0ull;
-        const unsigned long long b0 = __ballot(f0 != 0);
-        if (b0) {
-          const uint32_t l = static_cast<uint32_t>(__ffsll(b0) - 1);
-          F = (l << 6) + static_cast<uint32_t>(__ffsll(rd_lane64(f0, l)) - 1);
-        } else {
+      const uint32_t rn = rd_lane(recl, u < 63 ? u + 1 : 63u);
+      if (!(r & kRecLater)) {
+        const uint32_t h = r & 0x7FFFu;
+        const uint32_t w = h >> 6;
+        unsigned long long word = w < 64 ? rd_lane64(occ0, w) : rd_lane64(occ1, w - 64);
+        const unsigned long long fr = ~word & (~0ull << (h & 63));
+        uint32_t F = (w << 6) + static_cast<uint32_t>(__ffsll(fr) - 1);
+        uint32_t fw = w;
+        if (!fr) {  // past the home's word: the first word after it with a free slot
+          const unsigned long long f0 = lane > w ? ~occ0 : 0ull;
+          const unsigned long long f1 = lane + 64 > w ? ~occ1 : 0ull;
+          const unsigned long long b0 = __ballot(f0 != 0);
           const unsigned long long b1 = __ballot(f1 != 0);
-          if (!b1) {
-            bad = true;
+          if (!b0 && !b1) {
+            lost = true;
             break;
           }
-          const uint32_t l = static_cast<uint32_t>(__ffsll(b1) - 1);
-          F = ((l + 64) << 6) + static_cast<uint32_t>(__ffsll(rd_lane64(f1, l)) - 1);
+          const uint32_t l = static_cast<uint32_t>(__ffsll(b0 ? b0 : b1) - 1);
+          fw = l + (b0 ? 0u : 64u);
+          word = fw < 64 ? rd_lane64(occ0, fw) : rd_lane64(occ1, fw - 64);
+          F = (fw << 6) + static_cast<uint32_t>(__ffsll(~word) - 1);
         }
+        word |= 1ull << (F & 63);
+        const bool mine = lane == (fw & 63u);
+        if (fw < 64) occ0 = mine ? word : occ0;
+        else occ1 = mine ? word : occ1;
+        Fv = lane == u ? F : Fv;
       }
-      const uint32_t fw = F >> 6;
-      if (lane == (fw & 63u)) {
-        if (fw < 64) occ0 |= 1ull << (F & 63);
-        else occ1 |= 1ull << (F & 63);
-      }
-      const uint32_t lo = h + 1, hi = F;  // groups this insert rotates: homes lo .. hi-1
-      if (lo < hi) {
-        const uint32_t bl = lo >> 6, bh = hi >> 6;
-        const uint32_t x = (bl << 6) + lane;
-        if (bl == bh) {
-          if (x >= lo && x < hi) atomicAdd(&hg[x], 1u);
-        } else {
-          if (x >= lo) atomicAdd(&hg[x], 1u);
-          const uint32_t x2 = (bh << 6) + lane;
-          if (x2 < hi) atomicAdd(&hg[x2], 1u);
-          blk0 += (lane > bl && lane < bh) ? 1u : 0u;
-          blk1 += (lane + 64 > bl && lane + 64 < bh) ? 1u : 0u;
-        }
-      }
-      if (r & kRecFlag) {  // the rotations its group had before it
-        const uint32_t b = h >> 6;
-        const uint32_t cnt_h = (hg[h] & 0xFFFFu) + (b < 64 ? rd_lane(blk0, b) : rd_lane(blk1, b - 64));
-        if (lane == 0) rec16[2 * (base + u) + 1] = static_cast<uint16_t>(cnt_h);
-      }
+      r = rn;
     }
+    if (lost) break;
+    if (jl < m && !(recl & kRecLater)) rec16[2 * jl + 1] = static_cast<uint16_t>(Fv);
   }
-  if (bad) return kNotCanon;  // (wave-uniform)
-  fp[lane] = blk0;  // the fingerprints are done with
-  fp[lane + 64] = blk1;
+  if (lost) return kNotCanon;
   wave_sync();
+  // 5b. Rotation counts, 64 first copies at a time in batch order: a walk y
+  //     rotates the homes h with home_y < h < F_y.  A Fenwick tree over the
+  //     homes counts, for the walks of earlier chunks, those with
+  //     home_y < h (low half, added at home_y + 1) and those with F_y <= h
+  //     (high half, added at F_y); their difference at h is the walks over
+  //     h.  Walks earlier in the same chunk are counted from the lanes.  A
+  //     first copy of a group of two or more elements keeps its count (the
+  //     rotations its group had before it) in rec's high half.
+  for (uint32_t base = 0; base < m; base += 64) {
+    const uint32_t x = base + lane;
+    const uint32_t r = x < m ? rec[x] : kRecLater;
+    const bool first = !(r & kRecLater);
+    const uint32_t h = r & 0x7FFFu, F = (r >> 16) & 0x7FFFu;
+    const bool ask = first && (r & kRecFlag);
+    uint32_t c = 0;
+    if (ask) {
+      uint32_t lo_sum = 0, hi_sum = 0;
+      for (uint32_t i = h; i > 0; i -= i & (0u - i)) {
+        const uint32_t v = hg[i];
+        lo_sum += v & 0xFFFFu;
+        hi_sum += v >> 16;
+      }
+      c = lo_sum - hi_sum;
+    }
+    for (uint32_t o = 0; o < 63; ++o) {
+      const uint32_t ro = rd_lane(r, o);
+      c += (o < lane && !(ro & kRecLater) && (ro & 0x7FFFu) < h && h < ((ro >> 16) & 0x7FFFu)) ? 1u : 0u;
+    }
+    wave_sync();
+    if (first && F > h) {
+      for (uint32_t i = h + 1; i <= L; i += i & (0u - i)) atomicAdd(&hg[i], 1u);
+      for (uint32_t i = F; i <= L; i += i & (0u - i)) atomicAdd(&hg[i], 0x10000u);
+    }
+    if (ask) rec16[2 * x + 1] = static_cast<uint16_t>(c);
+    wave_sync();
+  }
   // 6. Each home's first final position (parking of the final groups).
   {
     uint32_t sum = 0;
     int mx = INT_MIN;
     for (uint32_t c = 0; c < nch; ++c) {
       const uint32_t h = (c << 6) + lane;
-      const uint32_t w = h < L ? hg[h] : 0u;
-      const uint32_t p = park_first(h, w >> 16, sum, mx);
-      if (h < L) hg[h] = (p << 16) | (w & 0xFFFFu);
+      const uint32_t f = h < L ? fcn[h] : 0u;
+      const uint32_t p = park_first(h, f, sum, mx);
+      if (h < L) fcn[h] = static_cast<uint16_t>(p);
     }
   }
   for (uint32_t q = lane; q < L; q += 64) t.owner[(r0 + q) & t.mask] = kNoOwner;
@@ -1384,9 +1407,14 @@ __device__ uint32_t replay_arc_canon(const Table& t, const uint8_t* __restrict__
       const uint32_t rf = park_first(h, rc, rsum, rmx);
       const uint32_t mend = w & 0xFFFFu, mbeg = (in && h > 0) ? (hb[h - 1] & 0xFFFFu) : 0u;
       if (!in) continue;
-      const uint32_t g = hg[h];
-      const uint32_t gpos = g >> 16;
-      const uint32_t cfin = (g & 0xFFFFu) + fp[h >> 6];
+      const uint32_t gpos = fcn[h];
+      uint32_t lo_sum = 0, hi_sum = 0;  // the walks over h of the whole batch
+      for (uint32_t i = h; i > 0; i -= i & (0u - i)) {
+        const uint32_t v = hg[i];
+        lo_sum += v & 0xFFFFu;
+        hi_sum += v >> 16;
+      }
+      const uint32_t cfin = lo_sum - hi_sum;
       uint32_t nf = 0;
       for (uint32_t y = mbeg; y < mend; ++y) nf += (rec[srt[y]] & kRecLater) ? 0u : 1u;
       claimed += nf;
@@ -2151,6 +2179,50 @@ k_ins_leader(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __rest
   }
 }
 
+// Sorts lst[0..m) ascending (distinct values below n) with one wave:
+// counts per bucket of the values' top bits (B buckets, a power of two),
+// offsets, a scatter into tmp[m], then each bucket (one lane each) by
+// insertion, and the result copied back.
+__device__ inline void sort_by_buckets(uint32_t* lst, uint32_t m, uint64_t n, uint32_t* tmp, uint32_t* bc,
+                                       uint32_t B) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t nb = 64u - static_cast<uint32_t>(__clzll(static_cast<unsigned long long>(n > 2 ? n - 1 : 1)));
+  const uint32_t lb = static_cast<uint32_t>(__ffs(B) - 1);
+  const uint32_t sh = nb > lb ? nb - lb : 0u;
+  for (uint32_t b = lane; b < B; b += 64) bc[b] = 0;
+  wave_sync();
+  for (uint32_t j = lane; j < m; j += 64) atomicAdd(&bc[lst[j] >> sh], 1u);
+  wave_sync();
+  uint32_t sum = 0;
+  for (uint32_t b0 = 0; b0 < B; b0 += 64) {
+    const uint32_t v = bc[b0 + lane];
+    const uint32_t incl = wave_incl_add(v);
+    bc[b0 + lane] = sum + incl - v;
+    sum += rd_lane(incl, 63);
+  }
+  wave_sync();
+  for (uint32_t j = lane; j < m; j += 64) {
+    const uint32_t v = lst[j];
+    tmp[atomicAdd(&bc[v >> sh], 1u)] = v;
+  }
+  wave_sync();
+  for (uint32_t b = lane; b < B; b += 64) {  // bc[b]: the end of bucket b
+    const uint32_t beg = b ? bc[b - 1] : 0u, end = bc[b];
+    for (uint32_t i = beg + 1; i < end; ++i) {
+      const uint32_t v = tmp[i];
+      uint32_t k = i;
+      while (k > beg && tmp[k - 1] > v) {
+        tmp[k] = tmp[k - 1];
+        --k;
+      }
+      tmp[k] = v;
+    }
+  }
+  wave_sync();
+  for (uint32_t j = lane; j < m; j += 64) lst[j] = tmp[j];
+  wave_sync();
+}
+
 // Replays one closure with a whole wave (64 active lanes).  The arc runs
 // from `s`, the dry slot of the closure's first member, up to its barrier
 // (the next bit still set in Q).  The members' batch indices are gathered
@@ -2196,6 +2268,45 @@ __device__ uint32_t replay_arc(const Table& t, const uint8_t* __restrict__ in_ke
   uint64_t lo = 0, hi = ~0ull;
   bool windowed = false;
   while (lo < n) {
+    uint32_t m = 0;
+    // (the words that slots s .. s+arc-1 fall in, counted from w0 without
+    // wrapping: a word can appear twice when the arc wraps the table)
+    const uint64_t nwc = ((s & 63) + arc + 63) >> 6;
+    if (kStatic && pool_words && nwc <= 128 && t.cap >= 64 && !windowed) {
+      // The arc's claim words in registers (lane l: words l and l + 64 from
+      // w0), then four 64-slot chunks at a time: the owners of the claimed
+      // slots read together, positions from the ballots.
+      const auto claim_word = [&](uint64_t i) -> unsigned long long {
+        const uint64_t w = (w0 + i) % words;
+        return i < nwc ? (P[w] & ~load_word(&Q[w])) : 0ull;
+      };
+      const unsigned long long cb0 = claim_word(lane), cb1 = claim_word(lane + 64);
+      const uint32_t o = static_cast<uint32_t>(s & 63);
+      const auto word_at = [&](uint32_t i) -> unsigned long long {
+        return i < 64 ? rd_lane64(cb0, i) : (i < 128 ? rd_lane64(cb1, i - 64) : 0ull);
+      };
+      for (uint64_t c0 = 0; c0 < arc; c0 += 256) {
+        uint32_t idx[4];
+        bool take[4];
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u) {
+          const uint32_t c = static_cast<uint32_t>(c0 >> 6) + u;
+          const unsigned long long bits = o ? ((word_at(c) >> o) | (word_at(c + 1) << (64 - o))) : word_at(c);
+          const uint64_t q = (static_cast<uint64_t>(c) << 6) + lane;
+          take[u] = q < arc && ((bits >> lane) & 1ull);
+          idx[u] = take[u] ? __hip_atomic_load(&t.owner[(s + q) & t.mask], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                           : 0u;
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u) {
+          const unsigned long long bal = __ballot(take[u]);
+          const uint32_t k = m + static_cast<uint32_t>(__popcll(bal & ((1ull << lane) - 1ull)));
+          if (take[u] && k < lcap) lst[k] = idx[u];
+          m += static_cast<uint32_t>(__popcll(bal));
+        }
+      }
+      wave_sync();
+    } else {
     if (lane == 0) *cnt = 0;
     wave_sync();
     for (uint64_t wi = lane; wi < nwords; wi += 64) {
@@ -2213,8 +2324,9 @@ __device__ uint32_t replay_arc(const Table& t, const uint8_t* __restrict__ in_ke
       }
     }
     wave_sync();
-    const uint32_t m = *cnt;
+    m = *cnt;
     wave_sync();
+    }
     if (m > lcap) {  // too many for one pass: windows of lcap indices
       windowed = true;
       hi = lo + lcap;
@@ -2222,6 +2334,10 @@ __device__ uint32_t replay_arc(const Table& t, const uint8_t* __restrict__ in_ke
     }
     uint32_t p2 = 1;
     while (p2 < m) p2 <<= 1;
+    if (kStatic && pool_words && m > 64 && 2 * ((m + 3) & ~3u) + p2 <= pool_words) {
+      sort_by_buckets(lst, m, n, lst + ((m + 3) & ~3u), lst + 2 * ((m + 3) & ~3u), p2 < 64 ? 64u : p2);
+      p2 = 0;  // (sorted)
+    }
     for (uint32_t k = m + lane; k < p2; k += 64) lst[k] = 0xFFFFFFFFu;
     wave_sync();
     for (uint32_t size = 2; size <= p2; size <<= 1) {
