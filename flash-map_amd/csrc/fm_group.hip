@@ -50,11 +50,15 @@ struct OpDesc {  // device-resident description of one op over the group
   const uint32_t* n_chunks;
 };
 
-// nbig[0]: closures queued for kg_ins_big, nbig[1]: its work ticket,
-// nbig[2 + b]: queued closures in size class b, nbig[2 + kBigClasses + b]:
-// the order pass's fill cursor of class b.
+// nbig[0]: closures queued for the big replays, nbig[1]: kg_ins_big's work
+// ticket, nbig[2 + b]: queued closures in size class b,
+// nbig[2 + kBigClasses + b]: the order pass's fill cursor of class b,
+// nbig[kSmallTicket]: kg_ins_big_small's work ticket, nbig[kOverflow]: the
+// closures it passed on to kg_ins_big.
 constexpr uint32_t kBigClasses = 32;
-constexpr uint32_t kNbigWords = 2 + 2 * kBigClasses;
+constexpr uint32_t kSmallTicket = 2 + 2 * kBigClasses;
+constexpr uint32_t kOverflow = kSmallTicket + 1;
+constexpr uint32_t kNbigWords = kOverflow + 1;
 
 __global__ void kg_zero(Counters* a, Counters* b, uint32_t n, uint32_t* nbig) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -252,19 +256,126 @@ kg_ins_leader(const Table* __restrict__ tables, OpDesc op, const uint8_t* __rest
   }
 }
 
+// The closures of fewer than 2^kSmallClasses members whose arcs fit
+// kSmallArcWords of LDS: one wave each, many workgroups per CU (the big
+// replay's LDS leaves room for one).  They come from the back of the
+// ordered queue (the small classes); a closure that does not fit is passed
+// on to kg_ins_big through `overflow`.  No workgroup waits for another.
+constexpr uint32_t kSmallClasses = 9;
+constexpr uint32_t kSmallLst = 1u << kSmallClasses;
+constexpr uint32_t kSmallArcWords = 4096;  // 16 KiB
+
+__device__ inline uint32_t large_closures(const uint32_t* nbig) {
+  uint32_t n = 0;
+  for (uint32_t b = kSmallClasses; b < kBigClasses; ++b) n += nbig[2 + b];
+  return n;
+}
+
+template <int KS, int VS>
+__global__ void __launch_bounds__(64)
+kg_ins_big_small(const Table* __restrict__ tables, OpDesc op, const uint8_t* __restrict__ keys,
+                 const uint8_t* __restrict__ vals, const unsigned long long* __restrict__ Q,
+                 const unsigned long long* __restrict__ qoff, const unsigned long long* __restrict__ ordered,
+                 unsigned long long* __restrict__ overflow, uint32_t* nbig_p, Counters* ctr) {
+  static_assert(KS > 0 && VS > 0 && KS % 4 == 0 && VS % 4 == 0, "static sizes");
+  __shared__ uint32_t lst[kSmallLst];
+  __shared__ uint32_t arc_lds[kSmallArcWords];
+  __shared__ uint32_t cnt, cur;
+  __shared__ unsigned long long barrier_at, barrier_before;
+  const uint32_t nbig = nbig_p[0];
+  const uint32_t n_large = large_closures(nbig_p);
+  if (n_large + blockIdx.x >= nbig) return;
+  for (;;) {
+    if (threadIdx.x == 0) cur = atomicAdd(&nbig_p[kSmallTicket], 1u);
+    __syncthreads();
+    const uint32_t c = n_large + cur;
+    __syncthreads();
+    if (c >= nbig) break;
+    const unsigned long long e = ordered[c];
+    const uint32_t table = static_cast<uint32_t>(e >> 32);
+    const uint64_t s = e & 0xFFFFFFFFull;
+    const Table t = tables[table];
+    const TableOp to = op.tops[table];
+    const unsigned long long* Qt = Q + qoff[table];
+    const unsigned long long* P = t.freebits;
+    const uint64_t words = (t.cap + 63) >> 6;
+    if (threadIdx.x == 0) {
+      barrier_at = bm_next_set(Qt, s, t.cap);
+      barrier_before = bm_prev_set(P, s, t.cap);
+      cnt = 0;
+    }
+    __syncthreads();
+    const uint64_t arc = (barrier_at + t.cap - s) & t.mask;
+    const uint64_t b = barrier_before;
+    const uint64_t L = (barrier_at - b - 1) & t.mask;
+    bool fits = b != s && (2 + KS / 4) * L <= kSmallArcWords;
+    if (fits) {
+      const uint64_t w0 = s >> 6;
+      const uint64_t nwords = min(((s & 63) + arc + 63) >> 6, words);
+      for (uint64_t wi = threadIdx.x; wi < nwords; wi += blockDim.x) {
+        const uint64_t w = (w0 + wi) % words;
+        unsigned long long bits = P[w] & ~Qt[w];
+        while (bits) {
+          const int bt = __ffsll(bits) - 1;
+          bits &= bits - 1;
+          const uint64_t p = (w << 6) + bt;
+          if (((p + t.cap - s) & t.mask) >= arc) continue;
+          const uint32_t k = atomicAdd(&cnt, 1u);
+          if (k < kSmallLst) lst[k] = t.owner[p];
+        }
+      }
+      __syncthreads();
+      fits = cnt <= kSmallLst;
+    }
+    if (!fits) {
+      if (threadIdx.x == 0) overflow[atomicAdd(&nbig_p[kOverflow], 1u)] = e;
+      __syncthreads();
+      continue;
+    }
+    const uint32_t m = cnt;
+    uint32_t p2 = 1;
+    while (p2 < m) p2 <<= 1;
+    for (uint32_t k = m + threadIdx.x; k < p2; k += blockDim.x) lst[k] = 0xFFFFFFFFu;
+    __syncthreads();
+    for (uint32_t size = 2; size <= p2; size <<= 1) {
+      for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+        for (uint32_t k = threadIdx.x; k < p2; k += blockDim.x) {
+          const uint32_t j = k ^ stride;
+          if (j > k) {
+            const uint32_t a = lst[k], b2 = lst[j];
+            if ((a > b2) == ((k & size) == 0)) {
+              lst[k] = b2;
+              lst[j] = a;
+            }
+          }
+        }
+        __syncthreads();
+      }
+    }
+    const uint32_t claimed = replay_arc_lds<KS, VS>(t, keys + to.begin * KS, vals + to.begin * VS, (b + 1) & t.mask,
+                                                    static_cast<uint32_t>(L), lst, m, arc_lds, &ctr[table]);
+    if (threadIdx.x == 0 && claimed) atomicAdd(&ctr[table].count, static_cast<unsigned long long>(claimed));
+    __syncthreads();
+  }
+}
+
 template <int KS, int VS>
 __global__ void __launch_bounds__(64)
 kg_ins_big(const Table* __restrict__ tables, OpDesc op, const uint8_t* __restrict__ keys,
            const uint8_t* __restrict__ vals, const unsigned long long* __restrict__ Q,
-           const unsigned long long* __restrict__ qoff, const unsigned long long* __restrict__ big,
-           uint32_t* nbig_p, Counters* ctr) {
-  __shared__ uint32_t lst[kBigMax];
+           const unsigned long long* __restrict__ qoff, const unsigned long long* __restrict__ ordered,
+           const unsigned long long* __restrict__ overflow, uint32_t* nbig_p, Counters* ctr) {
+  // static sizes: the member list, then the LDS replays' arrays
+  constexpr uint32_t kPool = (KS > 0 && VS > 0 && KS % 4 == 0 && VS % 4 == 0) ? kBigPoolWords : 0u;
+  __shared__ uint32_t lst[kPool ? kPool : kBigMax];
   __shared__ uint32_t carry[2 * kMaxGenericSize / 4];
   __shared__ uint32_t cnt;
   __shared__ unsigned long long barrier_at, barrier_before;
-  __shared__ uint32_t arc_lds[(KS > 0 && VS > 0 && KS % 4 == 0 && VS % 4 == 0) ? kArcLdsWords : 1];
   __shared__ uint32_t cur;
-  const uint32_t nbig = nbig_p[0];
+  // The closures of the large classes (the front of the ordered queue),
+  // then those kg_ins_big_small passed on (its launch has ended).
+  const uint32_t n_large = kPool ? large_closures(nbig_p) : nbig_p[0];  // (no small replay for runtime sizes)
+  const uint32_t nbig = n_large + nbig_p[kOverflow];
   // Closures are handed out one at a time (their replay times differ by
   // orders of magnitude, so a fixed assignment would leave workgroups idle).
   // Workgroups beyond the queue's length leave at once.
@@ -275,8 +386,9 @@ kg_ins_big(const Table* __restrict__ tables, OpDesc op, const uint8_t* __restric
     const uint32_t c = cur;
     __syncthreads();
     if (c >= nbig) break;
-    const uint32_t table = static_cast<uint32_t>(big[c] >> 32);
-    const uint64_t s = big[c] & 0xFFFFFFFFull;
+    const unsigned long long e = c < n_large ? ordered[c] : overflow[c - n_large];
+    const uint32_t table = static_cast<uint32_t>(e >> 32);
+    const uint64_t s = e & 0xFFFFFFFFull;
     const Table t = tables[table];
     const TableOp to = op.tops[table];
     const uint32_t K = KS ? KS : t.K;
@@ -284,11 +396,20 @@ kg_ins_big(const Table* __restrict__ tables, OpDesc op, const uint8_t* __restric
     const unsigned long long* Qt = Q + qoff[table];
     const unsigned long long* P = t.freebits;
     const uint64_t words = (t.cap + 63) >> 6;
-    if (threadIdx.x == 0) barrier_at = bm_next_set(Qt, s, t.cap);
+    if (threadIdx.x == 0) {
+      barrier_at = bm_next_set(Qt, s, t.cap);
+      barrier_before = kPool ? bm_prev_set(P, s, t.cap) : s;
+    }
     __syncthreads();
     const uint64_t arc = (barrier_at + t.cap - s) & t.mask;
     const uint64_t w0 = s >> 6;
     const uint64_t nwords = min(((s & 63) + arc + 63) >> 6, words);
+    // the whole arc, from the barrier before s; a canonical replay takes
+    // all of its members at once
+    const uint64_t b = barrier_before;
+    const uint64_t L = (barrier_at - b - 1) & t.mask;
+    const bool canon = kPool && t.canon_min && b != s && L <= kCanonMaxL && 2 * L <= kPool;
+    const uint32_t lcap = canon ? static_cast<uint32_t>(L) : kBigMax;
     uint64_t lo = 0, hi = ~0ull;
     bool windowed = false;
     while (lo < to.count) {
@@ -305,19 +426,23 @@ kg_ins_big(const Table* __restrict__ tables, OpDesc op, const uint8_t* __restric
           const uint32_t idx = t.owner[p];
           if (idx < lo || idx >= hi) continue;
           const uint32_t k = atomicAdd(&cnt, 1u);
-          if (k < kBigMax) lst[k] = idx;
+          if (k < lcap) lst[k] = idx;
         }
       }
       __syncthreads();
       const uint32_t m = cnt;
-      if (m > kBigMax) {
+      if (m > lcap) {
         windowed = true;
-        hi = lo + kBigMax;
+        hi = lo + lcap;
         __syncthreads();
         continue;
       }
       uint32_t p2 = 1;
       while (p2 < m) p2 <<= 1;
+      if (kPool && m > 64 && 2 * ((m + 3) & ~3u) + p2 <= kPool) {
+        sort_by_buckets(lst, m, to.count, lst + ((m + 3) & ~3u), lst + 2 * ((m + 3) & ~3u), p2);
+        p2 = 0;  // (sorted)
+      }
       for (uint32_t k = m + threadIdx.x; k < p2; k += blockDim.x) lst[k] = 0xFFFFFFFFu;
       __syncthreads();
       for (uint32_t size = 2; size <= p2; size <<= 1) {
@@ -336,16 +461,29 @@ kg_ins_big(const Table* __restrict__ tables, OpDesc op, const uint8_t* __restric
         }
       }
       if constexpr (KS > 0 && VS > 0 && KS % 4 == 0 && VS % 4 == 0) {
-        // The whole arc (from the barrier before s to the one after) in LDS
-        // when it fits; otherwise one wave walks each member in HBM.
-        if (threadIdx.x == 0) barrier_before = windowed ? s : bm_prev_set(P, s, t.cap);
-        __syncthreads();
-        const uint64_t b = barrier_before;
-        const uint64_t L = (barrier_at - b - 1) & t.mask;
+        // A canonical arc from its groups; else the whole arc (from the
+        // barrier before s to the one after) in LDS when it fits; otherwise
+        // one wave walks each member in HBM.
         uint32_t claimed = 0;
-        if (!windowed && b != s && L <= arc_lds_cap<KS>()) {
+        bool done = false;
+        if (canon && !windowed && m >= t.canon_min && canon_words(m, static_cast<uint32_t>(L)) <= kPool) {
+          const uint32_t c = replay_arc_canon<KS, VS>(t, keys + to.begin * K, vals + to.begin * V, (b + 1) & t.mask,
+                                                      static_cast<uint32_t>(L), lst, m, lst);
+          // owner[] is the removes' conflict scratch here: every entry of the
+          // arc back to "none"
+          __threadfence_block();
+          for (uint64_t q = threadIdx.x; q < L; q += blockDim.x) t.owner[(b + 1 + q) & t.mask] = kNoOwner;
+          __syncthreads();
+          if (c != kNotCanon) {
+            claimed = c;
+            done = true;
+          }
+        }
+        const uint32_t arc_off = max(kBigMax, (m + 3u) & ~3u);
+        if (done) {
+        } else if (!windowed && b != s && L <= arc_lds_cap<KS>() && arc_off + kArcLdsWords <= kPool) {
           claimed = replay_arc_lds<KS, VS>(t, keys + to.begin * K, vals + to.begin * V, (b + 1) & t.mask,
-                                           static_cast<uint32_t>(L), lst, m, arc_lds, &ctr[table]);
+                                           static_cast<uint32_t>(L), lst, m, lst + arc_off, &ctr[table]);
         } else {
           for (uint32_t j = 0; j < m; ++j) {  // one wave per workgroup walks each member
             const uint64_t gk = to.begin + lst[j];
@@ -573,8 +711,11 @@ void g_insert(const Table* tables, OpDesc op, const TableOp* tops, uint32_t n_ta
                      keys, vals, Q, qoff, dslot, big, nbig, ctr, coop_arc);
   hipLaunchKernelGGL(dev::kg_big_class, dim3(256), dim3(256), 0, s, tables, Q, qoff, big, nbig, cls);
   hipLaunchKernelGGL(dev::kg_big_order, dim3(256), dim3(256), 0, s, big, cls, nbig, ordered);
+  if constexpr (KS > 0 && VS > 0 && KS % 4 == 0 && VS % 4 == 0)
+    hipLaunchKernelGGL((dev::kg_ins_big_small<KS, VS>), dim3(2048), dim3(64), 0, s, tables, op, keys, vals, Q, qoff,
+                       ordered, big, nbig, ctr);
   hipLaunchKernelGGL((dev::kg_ins_big<KS, VS>), dim3(1024), dim3(64), 0, s, tables, op, keys, vals, Q, qoff, ordered,
-                     nbig, ctr);
+                     big, nbig, ctr);
   hipLaunchKernelGGL((dev::kg_ins_serial<KS, VS>), dim3(gridn(n_tables, 64)), dim3(64), 0, s, tables, tops, n_tables,
                      keys, vals, ctr);
   hipLaunchKernelGGL(dev::kg_ins_bits, dim3(grid), dim3(block), 0, s, tables, op, dslot, ctr);
@@ -826,6 +967,10 @@ fm_status GroupImpl::init(uint32_t n, const uint64_t* caps, uint32_t K, uint32_t
     return FM_ERR_GPU_ALLOC;
   }
   h_tables_.resize(n);
+  // Arcs of at least this many members replayed from their groups when
+  // canonical (0: never).
+  uint32_t canon_min = 32;
+  if (const char* env = std::getenv("FM_INSERT_CANON_MIN")) canon_min = static_cast<uint32_t>(std::strtoul(env, nullptr, 10));
   for (uint32_t i = 0; i < n; ++i) {
     Table& t = h_tables_[i];
     t.keys = keys_all_ + slot_off[i] * K;
@@ -839,6 +984,7 @@ fm_status GroupImpl::init(uint32_t n, const uint64_t* caps, uint32_t K, uint32_t
     t.K = K;
     t.V = V;
     t.mode = mode_;
+    t.canon_min = canon_min;
     hipLaunchKernelGGL(dev::k_clear, dim3(gridn(t.cap, 256)), dim3(256), 0, stream_, t, nullptr);
   }
   hipLaunchKernelGGL(dev::k_fill_u32, dim3(gridn(total_slots_, 256)), dim3(256), 0, stream_, owner_all_, total_slots_,
