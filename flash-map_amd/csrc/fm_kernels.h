@@ -1238,10 +1238,13 @@ __device__ uint32_t replay_arc_canon(const Table& t, const uint8_t* __restrict__
   for (uint32_t j = lane; j < m; j += 64)  // (a home's range in any order; sorted below)
     srt[atomicAdd(&hb[rec[j] & 0x7FFFu], 1u) & 0xFFFFu] = static_cast<uint16_t>(j);
   wave_sync();
-  // 4. Later copies (one lane per home): the home's members put in batch
-  //    order, each member's key against the group's residents and earlier
-  //    first copies; the group's element count; first copies of groups of
-  //    two or more elements need their rotation counts.
+  // 4. Later copies.  (a) One lane per home: the home's members put in
+  //    batch order, and the first slot of its residents (parking of the
+  //    residents by home) kept in fcn.  (b) One lane per member: its key
+  //    against the group's residents, then against the group's earlier
+  //    members (the first equal one is the key's first copy).  (c) One lane
+  //    per home: the group's element count in fcn; first copies of groups
+  //    of two or more elements need their rotation counts.
   {
     uint32_t rsum = 0;
     int rmx = INT_MIN;
@@ -1249,10 +1252,9 @@ __device__ uint32_t replay_arc_canon(const Table& t, const uint8_t* __restrict__
       const uint32_t h = (c << 6) + lane;
       const bool in = h < L;
       const uint32_t w = in ? hb[h] : 0u;
-      const uint32_t rc = w >> 16;
-      const uint32_t rf = park_first(h, rc, rsum, rmx);
+      const uint32_t rf = park_first(h, w >> 16, rsum, rmx);
       const uint32_t mend = w & 0xFFFFu, mbeg = (in && h > 0) ? (hb[h - 1] & 0xFFFFu) : 0u;
-      for (uint32_t i = mbeg + 1; i < mend; ++i) {  // the home's members in batch order
+      for (uint32_t i = mbeg + 1; i < mend; ++i) {
         const uint16_t v = srt[i];
         uint32_t k = i;
         while (k > mbeg && srt[k - 1] > v) {
@@ -1261,38 +1263,50 @@ __device__ uint32_t replay_arc_canon(const Table& t, const uint8_t* __restrict__
         }
         srt[k] = v;
       }
-      uint32_t firsts = 0;
-      for (uint32_t y = mbeg; y < mend; ++y) {
-        const uint32_t j = srt[y];
-        const uint32_t fj = fp[j];
-        uint32_t tgt = kNoOwner;
-        bool tres = false;
-        if (rc > 0) {
-          Blob<KS> kj;
-          kj.load(in_keys + static_cast<uint64_t>(lst[j]) * KS, KS);
-          for (uint32_t i = 0; i < rc && tgt == kNoOwner; ++i)
-            if (kj.eq(t.keys + ((r0 + rf + i) & t.mask) * KS, KS)) {
-              tgt = rf + i;
-              tres = true;
-            }
-        }
-        for (uint32_t y2 = mbeg; y2 < y && tgt == kNoOwner; ++y2) {
-          const uint32_t j2 = srt[y2];
-          if ((rec[j2] & kRecLater) || fp[j2] != fj) continue;
-          Blob<KS> kj;
-          kj.load(in_keys + static_cast<uint64_t>(lst[j]) * KS, KS);
-          if (kj.eq(in_keys + static_cast<uint64_t>(lst[j2]) * KS, KS)) tgt = j2;
-        }
-        if (tgt != kNoOwner) rec[j] |= kRecLater | (tres ? kRecFlag : 0u) | (tgt << 16);
-        else ++firsts;
-      }
-      if (in) fcn[h] = static_cast<uint16_t>(rc + firsts);
-      if (rc + firsts >= 2)
-        for (uint32_t y = mbeg; y < mend; ++y) {
-          const uint32_t j = srt[y];
-          if (!(rec[j] & kRecLater)) rec[j] |= kRecFlag;
+      if (in) fcn[h] = static_cast<uint16_t>(rf);
+    }
+  }
+  wave_sync();
+  for (uint32_t y = lane; y < m; y += 64) {
+    const uint32_t j = srt[y];
+    const uint32_t h = rec[j] & 0x7FFFu;
+    const uint32_t w = hb[h];
+    const uint32_t rc = w >> 16, mbeg = h > 0 ? (hb[h - 1] & 0xFFFFu) : 0u;
+    const uint32_t fj = fp[j];
+    uint32_t tgt = kNoOwner;
+    bool tres = false;
+    if (rc > 0) {
+      const uint32_t rf = fcn[h];
+      Blob<KS> kj;
+      kj.load(in_keys + static_cast<uint64_t>(lst[j]) * KS, KS);
+      for (uint32_t i = 0; i < rc && tgt == kNoOwner; ++i)
+        if (kj.eq(t.keys + ((r0 + rf + i) & t.mask) * KS, KS)) {
+          tgt = rf + i;
+          tres = true;
         }
     }
+    for (uint32_t y2 = mbeg; y2 < y && tgt == kNoOwner; ++y2) {
+      const uint32_t j2 = srt[y2];
+      if (fp[j2] != fj) continue;
+      Blob<KS> kj;
+      kj.load(in_keys + static_cast<uint64_t>(lst[j]) * KS, KS);
+      if (kj.eq(in_keys + static_cast<uint64_t>(lst[j2]) * KS, KS)) tgt = j2;
+    }
+    if (tgt != kNoOwner) rec[j] |= kRecLater | (tres ? kRecFlag : 0u) | (tgt << 16);
+  }
+  wave_sync();
+  for (uint32_t h = lane; h < L; h += 64) {
+    const uint32_t w = hb[h];
+    const uint32_t mend = w & 0xFFFFu, mbeg = h > 0 ? (hb[h - 1] & 0xFFFFu) : 0u;
+    uint32_t firsts = 0;
+    for (uint32_t y = mbeg; y < mend; ++y) firsts += (rec[srt[y]] & kRecLater) ? 0u : 1u;
+    const uint32_t n = (w >> 16) + firsts;
+    fcn[h] = static_cast<uint16_t>(n);
+    if (n >= 2)
+      for (uint32_t y = mbeg; y < mend; ++y) {
+        const uint32_t j = srt[y];
+        if (!(rec[j] & kRecLater)) rec[j] |= kRecFlag;
+      }
   }
   wave_sync();
   // 5. F of every first copy: its walk's end, a parking of the first
@@ -1391,11 +1405,51 @@ __device__ uint32_t replay_arc_canon(const Table& t, const uint8_t* __restrict__
   for (uint32_t q = lane; q < L; q += 64) t.owner[(r0 + q) & t.mask] = kNoOwner;
   __threadfence_block();
   wave_sync();
-  // 7. Each group's final order (one lane per home): a record per element
-  //    at its final slot in owner[]: bit 31 set: key and value of batch rank
-  //    (bits 0-14, the element's last copy), else the resident at slot
-  //    (bits 0-14) unchanged; bits 16-30 the home.
+  // 7. Each group's final order: a record per element at its final slot in
+  //    owner[]: bit 31 set: key and value of batch rank (bits 0-14, the
+  //    element's last copy), else the resident at slot (bits 0-14)
+  //    unchanged; bits 16-30 the home.  An element's index in its group
+  //    follows the rotations after it joined (each first copy of the group
+  //    brings a count; cfin is the count after the batch).  First copies
+  //    one lane each, residents one lane per home.
+  const auto walks_over = [&](uint32_t h) -> uint32_t {  // the walks over h of the whole batch
+    uint32_t lo_sum = 0, hi_sum = 0;
+    for (uint32_t i = h; i > 0; i -= i & (0u - i)) {
+      const uint32_t v = hg[i];
+      lo_sum += v & 0xFFFFu;
+      hi_sum += v >> 16;
+    }
+    return lo_sum - hi_sum;
+  };
   uint32_t claimed = 0;
+  for (uint32_t y = lane; y < m; y += 64) {
+    const uint32_t j = srt[y];
+    const uint32_t r = rec[j];
+    if (r & kRecLater) continue;
+    ++claimed;
+    const uint32_t h = r & 0x7FFFu;
+    const uint32_t w = hb[h];
+    const uint32_t rc = w >> 16, mend = w & 0xFFFFu, mbeg = h > 0 ? (hb[h - 1] & 0xFFFFu) : 0u;
+    uint32_t e = rc;
+    for (uint32_t y2 = mbeg; y2 < y; ++y2) e += (rec[srt[y2]] & kRecLater) ? 0u : 1u;
+    uint32_t idx = e, size = e + 1, cprev = (r >> 16) & 0x7FFFu, vs = j;
+    for (uint32_t y2 = y + 1; y2 < mend; ++y2) {
+      const uint32_t j2 = srt[y2];
+      const uint32_t r2 = rec[j2];
+      if (r2 & kRecLater) {
+        if (!(r2 & kRecFlag) && ((r2 >> 16) & 0x7FFFu) == j) vs = j2;
+      } else {
+        const uint32_t c2 = (r2 >> 16) & 0x7FFFu;
+        idx = rot_index(idx, c2 - cprev, size);
+        cprev = c2;
+        ++size;
+      }
+    }
+    if (size > 1) idx = rot_index(idx, walks_over(h) - cprev, size);
+    const uint32_t pos = fcn[h] + idx;
+    if (pos >= L) bad = true;
+    else t.owner[(r0 + pos) & t.mask] = 0x80000000u | vs | (h << 16);
+  }
   {
     uint32_t rsum = 0;
     int rmx = INT_MIN;
@@ -1405,46 +1459,17 @@ __device__ uint32_t replay_arc_canon(const Table& t, const uint8_t* __restrict__
       const uint32_t w = in ? hb[h] : 0u;
       const uint32_t rc = w >> 16;
       const uint32_t rf = park_first(h, rc, rsum, rmx);
-      const uint32_t mend = w & 0xFFFFu, mbeg = (in && h > 0) ? (hb[h - 1] & 0xFFFFu) : 0u;
-      if (!in) continue;
-      const uint32_t gpos = fcn[h];
-      uint32_t lo_sum = 0, hi_sum = 0;  // the walks over h of the whole batch
-      for (uint32_t i = h; i > 0; i -= i & (0u - i)) {
-        const uint32_t v = hg[i];
-        lo_sum += v & 0xFFFFu;
-        hi_sum += v >> 16;
-      }
-      const uint32_t cfin = lo_sum - hi_sum;
-      uint32_t nf = 0;
-      for (uint32_t y = mbeg; y < mend; ++y) nf += (rec[srt[y]] & kRecLater) ? 0u : 1u;
-      claimed += nf;
-      for (uint32_t e = 0; e < rc + nf; ++e) {
-        // element e: resident e (e < rc) or the (e - rc)-th first copy
-        uint32_t idx, size, cprev = 0, ystart = mbeg, id = 0;
-        if (e < rc) {
-          idx = e;
-          size = rc;
-          id = rf + e;
-        } else {
-          uint32_t k = e - rc, y = mbeg;
-          for (;; ++y) {
-            if (rec[srt[y]] & kRecLater) continue;
-            if (k == 0) break;
-            --k;
-          }
-          id = srt[y];
-          idx = e;
-          size = e + 1;
-          cprev = (rec[id] >> 16) & 0x7FFFu;
-          ystart = y + 1;
-        }
-        uint32_t vs = kNoOwner;
+      if (rc == 0) continue;
+      const uint32_t mend = w & 0xFFFFu, mbeg = h > 0 ? (hb[h - 1] & 0xFFFFu) : 0u;
+      const uint32_t cfin = walks_over(h);
+      for (uint32_t e = 0; e < rc; ++e) {
+        uint32_t idx = e, size = rc, cprev = 0, vs = kNoOwner;
         for (uint32_t y = mbeg; y < mend; ++y) {
           const uint32_t j = srt[y];
           const uint32_t r = rec[j];
           if (r & kRecLater) {
-            if (((r >> 16) & 0x7FFFu) == id && ((r & kRecFlag) != 0) == (e < rc)) vs = j;
-          } else if (y >= ystart) {
+            if ((r & kRecFlag) && ((r >> 16) & 0x7FFFu) == rf + e) vs = j;
+          } else {
             const uint32_t cj = (r >> 16) & 0x7FFFu;
             idx = rot_index(idx, cj - cprev, size);
             cprev = cj;
@@ -1452,13 +1477,12 @@ __device__ uint32_t replay_arc_canon(const Table& t, const uint8_t* __restrict__
           }
         }
         idx = rot_index(idx, cfin - cprev, size);
-        const uint32_t pos = gpos + idx;
+        const uint32_t pos = fcn[h] + idx;
         if (pos >= L) {
           bad = true;
           break;
         }
-        const uint32_t recd = (vs != kNoOwner) ? (0x80000000u | vs) : (e < rc ? id : (0x80000000u | id));
-        t.owner[(r0 + pos) & t.mask] = recd | (h << 16);
+        t.owner[(r0 + pos) & t.mask] = (vs != kNoOwner ? (0x80000000u | vs) : (rf + e)) | (h << 16);
       }
     }
   }
