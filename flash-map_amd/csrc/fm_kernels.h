@@ -1705,25 +1705,33 @@ constexpr uint32_t kVsBatch = 0x100u;  // value source: the batch value of the m
 // the arcs replayed, `starts` their first slots and `mem` their dry-claimed
 // slots.  Every arc replays its j-th member (in batch order) in step j, so
 // a round takes as many steps as its largest arc has members.  Returns the
-// slots claimed (the same in every lane).
-template <int KS, int VS>
+// slots claimed (the same in every lane).  kPacked: the arcs come from
+// anywhere in the table, packed side by side into the 64 lanes, and lane l
+// holds slot `lane_slot` (consecutive within an arc; `base` unused).
+template <int KS, int VS, bool kPacked = false>
 __device__ __forceinline__ uint32_t arcs_round(const Table& t, const uint8_t* __restrict__ in_keys,
                                                const uint8_t* __restrict__ in_vals, uint64_t base,
                                                unsigned long long act, unsigned long long starts,
                                                unsigned long long mem, Counters* ctr,
-                                               unsigned long long& claims) {
+                                               unsigned long long& claims, uint64_t lane_slot = 0) {
   constexpr int KW = KS / 4;
   const uint32_t lane = threadIdx.x & 63u;
   const bool in = (act >> lane) & 1ull;
   const bool is_mem = (mem >> lane) & 1ull;
-  const uint64_t q = (base + lane) & t.mask;
+  const uint64_t q = kPacked ? lane_slot : (base + lane) & t.mask;
   // this lane's arc: first lane a0, length alen, members mcount, and the
   // sorted position of its first member (members of earlier arcs sort first)
   const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
   const unsigned long long sm = starts & upto;
   const uint32_t a0 = sm ? 63u - static_cast<uint32_t>(__clzll(sm)) : 0u;
-  const unsigned long long rest = ~(act >> a0);
-  const uint32_t alen = rest ? static_cast<uint32_t>(__ffsll(rest) - 1) : 64u - a0;
+  uint32_t alen;
+  if constexpr (kPacked) {  // arcs abut: this one ends where the next starts (or the packed lanes end)
+    const unsigned long long nxt = a0 < 63 ? (starts & (~0ull << (a0 + 1))) : 0ull;
+    alen = nxt ? static_cast<uint32_t>(__ffsll(nxt) - 1) - a0 : static_cast<uint32_t>(__popcll(act)) - a0;
+  } else {
+    const unsigned long long rest = ~(act >> a0);
+    alen = rest ? static_cast<uint32_t>(__ffsll(rest) - 1) : 64u - a0;
+  }
   const unsigned long long amask = (alen == 64 ? ~0ull : ((1ull << alen) - 1ull)) << a0;
   const uint32_t mcount = static_cast<uint32_t>(__popcll(mem & amask));
   const uint32_t mst = static_cast<uint32_t>(__popcll(mem & ((1ull << a0) - 1ull)));
@@ -1737,10 +1745,15 @@ __device__ __forceinline__ uint32_t arcs_round(const Table& t, const uint8_t* __
   }
   Blob<KS> mk;  // a member's key, in the lane of its dry slot
   uint32_t hp = 0, vi = 0;
+  // (packed: the slot held by lane a0, the arc's first)
+  const uint64_t org = kPacked ? (static_cast<uint64_t>(static_cast<uint32_t>(__shfl(static_cast<int>(q), a0, 64))) |
+                                  (static_cast<uint64_t>(static_cast<uint32_t>(__shfl(static_cast<int>(q >> 32), a0, 64))) << 32))
+                               : 0ull;
   if (is_mem) {
     mk.load(in_keys + static_cast<uint64_t>(o) * KS, KS);
     vi = static_cast<uint32_t>(dd_value_index(t, o));
-    hp = static_cast<uint32_t>(((mk.hash(KS, t.mode) & t.mask) - base) & t.mask);  // home lane
+    hp = kPacked ? a0 + static_cast<uint32_t>(((mk.hash(KS, t.mode) & t.mask) - org) & t.mask)
+                 : static_cast<uint32_t>(((mk.hash(KS, t.mode) & t.mask) - base) & t.mask);  // home lane
     if (hp > lane || hp < a0) {  // cannot happen: a member's home lies in its arc, at or before its dry slot
       atomicOr(&ctr->status, kStatusBudget);
       hp = lane;
@@ -1849,10 +1862,15 @@ __device__ __forceinline__ uint32_t arcs_round(const Table& t, const uint8_t* __
   const uint32_t vsrc = vs & 63u;
   const uint32_t vi_src = static_cast<uint32_t>(__shfl(static_cast<int>(vi), static_cast<int>(vsrc), 64));
   const bool vmove = in && dirty && vs != lane;
+  uint64_t src_slot = (base + vsrc) & t.mask;
+  if constexpr (kPacked)
+    src_slot = static_cast<uint64_t>(static_cast<uint32_t>(__shfl(static_cast<int>(q), static_cast<int>(vsrc), 64))) |
+               (static_cast<uint64_t>(static_cast<uint32_t>(__shfl(static_cast<int>(q >> 32), static_cast<int>(vsrc), 64)))
+                << 32);
   Blob<VS> v;
   if (vmove) {
     if (vs & kVsBatch) v.load(in_vals + static_cast<uint64_t>(vi_src) * VS, VS);
-    else v.load(t.vals + ((base + vsrc) & t.mask) * VS, VS);
+    else v.load(t.vals + src_slot * VS, VS);
   }
   if (in && dirty) {
     t.flags[q] = make_flag(kOccupied, dist);
@@ -2045,6 +2063,7 @@ k_ins_leader(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __rest
   constexpr bool kCoop = kPlan && VS % 16 == 0;
   __shared__ uint64_t mv_src[kCoop ? 64 * kPlanMoves : 1], mv_dst[kCoop ? 64 * kPlanMoves : 1];
   __shared__ u32x4 mv_stage[kCoop ? 64 * (VS / 16) : 1];
+  __shared__ uint64_t arc_at[kPlan ? 64 : 1];  // packed replay: the first slot of the arc starting in lane l
   const uint32_t K = KS ? KS : t.K;
   const uint32_t V = VS ? VS : t.V;
   const unsigned long long* P = t.freebits;
@@ -2058,6 +2077,9 @@ k_ins_leader(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __rest
     const uint32_t sd = i < n ? dslot[i] : kNoSlot;
     uint64_t mv[2 * kPlanMoves];
     uint32_t nmv = 0;
+    bool packed = false;  // this lane's closure goes to the packed replay below
+    uint64_t arc0 = 0;
+    uint32_t alen = 0;
     if (sd != kNoSlot) {
       const uint64_t s = sd;
       Blob<KS> ck;
@@ -2110,7 +2132,8 @@ k_ins_leader(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __rest
           }
           const uint64_t x = (cw << 6) + static_cast<uint64_t>(__ffsll(bits) - 1);
           if ((cq >> (x & 63)) & 1ull) {  // the closure's barrier
-            long_arc = ((x - b - 1) & t.mask) > coop_arc;
+            alen = static_cast<uint32_t>(min<uint64_t>((x - b - 1) & t.mask, 0xFFFFFFFFull));
+            long_arc = alen > coop_arc;
             break;
           }
           if (m == leader_max) {
@@ -2135,6 +2158,9 @@ k_ins_leader(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __rest
           claimed += c1;
           if (c1 && t.nfree) claim_next(t, at);
           if (t.outcome) t.outcome[i] = static_cast<uint8_t>(c1);
+        } else if (kPlan && alen <= kArcSlots && t.cap >= 256) {
+          packed = true;  // (replayed with the wave's other such closures after this step)
+          arc0 = (b + 1) & t.mask;
         } else {
           for (uint32_t a = 1; a < m; ++a) {  // insertion sort by batch index
             const uint32_t v = my[a * 64];
@@ -2154,6 +2180,52 @@ k_ins_leader(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __rest
             if (t.outcome) t.outcome[k] = static_cast<uint8_t>(c1);
           }
         }
+      }
+    }
+    if constexpr (kPlan) {
+      // Closures that the plan does not cover (several members, or a walk
+      // past the plan's reach) whose arcs hold at most 64 slots: packed side
+      // by side into the 64 lanes (arc after arc in lane order, as many as
+      // fit) and replayed together by wave scans, one member per arc and
+      // step (arcs_round), instead of walked slot by slot.
+      unsigned long long todo = __ballot(packed);
+      while (todo) {
+        const bool mine = (todo >> threadIdx.x) & 1ull;
+        const uint32_t len = mine ? alen : 0u;
+        const uint32_t incl = wave_incl_add(len);
+        const bool take = mine && incl <= 64;
+        const unsigned long long took = __ballot(take);
+        const uint32_t first = incl - len;  // this arc's first lane
+        if (take) arc_at[first] = arc0;
+        const unsigned long long starts = __ballot(take) ? [&] {
+          unsigned long long sm = 0;
+          for (unsigned long long tk = took; tk; tk &= tk - 1) {
+            const int l = __ffsll(tk) - 1;
+            sm |= 1ull << rd_lane(first, static_cast<uint32_t>(l));
+          }
+          return sm;
+        }() : 0ull;
+        const uint32_t total = rd_lane(incl, static_cast<uint32_t>(63 - __clzll(took)));
+        wave_sync();
+        const unsigned long long act = total >= 64 ? ~0ull : ((1ull << total) - 1ull);
+        const uint32_t lane = threadIdx.x & 63u;
+        const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+        const unsigned long long sm = starts & upto;
+        const uint32_t a0 = sm ? 63u - static_cast<uint32_t>(__clzll(sm)) : 0u;
+        const uint64_t slot = (arc_at[a0] + (lane - a0)) & t.mask;
+        const bool inl = lane < total;
+        bool dry = false;
+        if (inl) {
+          const unsigned long long pw = P[slot >> 6], qw = Q[slot >> 6];
+          dry = ((pw & ~qw) >> (slot & 63)) & 1ull;
+        }
+        const unsigned long long mem = __ballot(inl && dry);
+        unsigned long long claims = 0;
+        claimed += arcs_round<KS, VS, true>(t, in_keys, in_vals, 0, act, starts, mem, ctr, claims, slot) *
+                   (lane == 0 ? 1u : 0u);
+        if (((claims >> lane) & 1ull) && t.nfree) claim_next(t, slot);
+        wave_sync();
+        todo &= ~took;
       }
     }
     if constexpr (kPlan && !kCoop) {
