@@ -1106,6 +1106,11 @@ __device__ __forceinline__ unsigned long long rd_lane64(unsigned long long v, ui
   return static_cast<unsigned long long>(rd_lane(static_cast<uint32_t>(v), l)) |
          (static_cast<unsigned long long>(rd_lane(static_cast<uint32_t>(v >> 32), l)) << 32);
 }
+__device__ __forceinline__ unsigned long long rd_any64(unsigned long long v, uint32_t src) {
+  return static_cast<unsigned long long>(static_cast<uint32_t>(__shfl(static_cast<int>(v), static_cast<int>(src), 64))) |
+         (static_cast<unsigned long long>(static_cast<uint32_t>(__shfl(static_cast<int>(v >> 32), static_cast<int>(src), 64)))
+          << 32);
+}
 __device__ __forceinline__ uint32_t wave_incl_add(uint32_t v) {
   const uint32_t lane = threadIdx.x & 63u;
 #pragma unroll
@@ -1554,7 +1559,12 @@ template <int KS, int VS>
 __device__ inline bool insert_planned(const Table& t, const Blob<KS>& ck, const uint8_t* vin, uint64_t home,
                                       uint64_t s, uint32_t L, const uint32_t (&fl)[kPlanSlots], uint32_t* c1,
                                       uint64_t (&mv)[2 * kPlanMoves], uint32_t* nmv) {
-  if (L > kPlanSlots) return false;
+  // A complete plan covers the walk up to s (at most kPlanChecks comparisons
+  // and kPlanEvicts evictions, s among the plan's slots).  Otherwise the
+  // walk can still end at an equal key among the first kPlanChecks
+  // comparisons (an update, rayon_cpu.rs:176-181): the plan succeeds only
+  // when it finds one there.
+  const bool window = L > kPlanSlots;
   uint32_t p = 0, ne = 0, nc = 0, ep0 = 0, ep1 = 0, cmask = 0, emask = 0;
   bool ok = true;
 #pragma unroll
@@ -1578,7 +1588,8 @@ __device__ inline bool insert_planned(const Table& t, const Blob<KS>& ck, const 
       }
     }
   }
-  if (!ok || p > t.mask || nc > kPlanChecks || ne > kPlanEvicts) return false;
+  const bool complete = !window && nc <= kPlanChecks && ne <= kPlanEvicts;
+  if (!ok || p > t.mask || (!complete && nc == 0)) return false;
   const uint32_t K = KS, V = VS;
   auto slot = [&](uint32_t j) { return (home + j) & t.mask; };
   const uint32_t e0 = emask ? static_cast<uint32_t>(__ffs(emask) - 1) : 0u;
@@ -1608,10 +1619,13 @@ __device__ inline bool insert_planned(const Table& t, const Blob<KS>& ck, const 
   const bool q0 = c0 == 0 ? ck.eqb(kc0) : (c0 == 1 ? ke0.eqb(kc0) : ke1.eqb(kc0));
   const bool q1 = c1_ == 0 ? ck.eqb(kc1) : (c1_ == 1 ? ke0.eqb(kc1) : ke1.eqb(kc1));
   const bool q2 = c2_ == 0 ? ck.eqb(kc2) : (c2_ == 1 ? ke0.eqb(kc2) : ke1.eqb(kc2));
+  // (a comparison after more than two evictions would compare a resident
+  // the plan did not load: no stop there or after it)
   uint32_t stop_j = ~0u;
-  if (nc > 0 && q0) stop_j = cj0;
-  else if (nc > 1 && q1) stop_j = cj1;
-  else if (nc > 2 && q2) stop_j = cj2;
+  if (nc > 0 && c0 <= 2 && q0) stop_j = cj0;
+  else if (nc > 1 && c1_ <= 2 && q1) stop_j = cj1;
+  else if (nc > 2 && c2_ <= 2 && q2) stop_j = cj2;
+  if (!complete && stop_j == ~0u) return false;
   const uint32_t applied = stop_j != ~0u ? static_cast<uint32_t>(__popc(emask & ((1u << stop_j) - 1u))) : ne;
   const uint64_t vb = reinterpret_cast<uint64_t>(t.vals);
   const uint64_t src_e0 = vb + slot(e0) * V, src_e1 = vb + slot(e1) * V;
@@ -1707,13 +1721,15 @@ constexpr uint32_t kVsBatch = 0x100u;  // value source: the batch value of the m
 // a round takes as many steps as its largest arc has members.  Returns the
 // slots claimed (the same in every lane).  kPacked: the arcs come from
 // anywhere in the table, packed side by side into the 64 lanes, and lane l
-// holds slot `lane_slot` (consecutive within an arc; `base` unused).
+// holds slot `lane_slot` (consecutive within an arc; `base` unused) and, for
+// a member, its batch index `lane_owner` (owner[] is not read).
 template <int KS, int VS, bool kPacked = false>
 __device__ __forceinline__ uint32_t arcs_round(const Table& t, const uint8_t* __restrict__ in_keys,
                                                const uint8_t* __restrict__ in_vals, uint64_t base,
                                                unsigned long long act, unsigned long long starts,
                                                unsigned long long mem, Counters* ctr,
-                                               unsigned long long& claims, uint64_t lane_slot = 0) {
+                                               unsigned long long& claims, uint64_t lane_slot = 0,
+                                               uint32_t lane_owner = 0) {
   constexpr int KW = KS / 4;
   const uint32_t lane = threadIdx.x & 63u;
   const bool in = (act >> lane) & 1ull;
@@ -1740,7 +1756,7 @@ __device__ __forceinline__ uint32_t arcs_round(const Table& t, const uint8_t* __
   if (in) {
     f = t.flags[q];
     rk.load(t.keys + q * KS, KS);
-    if (is_mem) o = t.owner[q];
+    if (is_mem) o = kPacked ? lane_owner : t.owner[q];
     keep_loaded<KS>(f, rk);
   }
   Blob<KS> mk;  // a member's key, in the lane of its dry slot
@@ -2064,6 +2080,7 @@ k_ins_leader(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __rest
   __shared__ uint64_t mv_src[kCoop ? 64 * kPlanMoves : 1], mv_dst[kCoop ? 64 * kPlanMoves : 1];
   __shared__ u32x4 mv_stage[kCoop ? 64 * (VS / 16) : 1];
   __shared__ uint64_t arc_at[kPlan ? 64 : 1];  // packed replay: the first slot of the arc starting in lane l
+  __shared__ uint8_t arc_src[kPlan ? 64 : 1];   // ... and the lane that leads it
   const uint32_t K = KS ? KS : t.K;
   const uint32_t V = VS ? VS : t.V;
   const unsigned long long* P = t.freebits;
@@ -2080,6 +2097,7 @@ k_ins_leader(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __rest
     bool packed = false;  // this lane's closure goes to the packed replay below
     uint64_t arc0 = 0;
     uint32_t alen = 0;
+    unsigned long long dmask = 0;
     if (sd != kNoSlot) {
       const uint64_t s = sd;
       Blob<KS> ck;
@@ -2091,7 +2109,7 @@ k_ins_leader(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __rest
         home = ck.hash(K, t.mode) & t.mask;
         L = static_cast<uint32_t>(min<uint64_t>((s - home) & t.mask, kPlanSlots + 1));
 #pragma unroll
-        for (uint32_t j = 0; j < kPlanSlots; ++j) fl[j] = (L <= kPlanSlots && j < L) ? t.flags[(home + j) & t.mask] : 0u;
+        for (uint32_t j = 0; j < kPlanSlots; ++j) fl[j] = j < L ? t.flags[(home + j) & t.mask] : 0u;
       }
       // The free-slot and claim words at the dry slot, read together (and
       // with the flags above); most closures begin, and end, inside that word.
@@ -2115,6 +2133,7 @@ k_ins_leader(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __rest
         uint32_t m = 0;
         uint64_t pos = s;
         bool long_arc = false;
+        const uint64_t a_first = (b + 1) & t.mask;
         for (;;) {
           // next pre-batch free slot at/after pos (cyclic), from the cached
           // word pair or the next ones
@@ -2141,6 +2160,8 @@ k_ins_leader(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __rest
             break;
           }
           my[m * 64] = (x == s) ? static_cast<uint32_t>(i) : t.owner[x];  // owner[s] == i (k_ins_dry)
+          const uint64_t dx = (x - a_first) & t.mask;
+          if (dx < 64) dmask |= 1ull << dx;  // the members' dry slots in the arc, in slot order
           ++m;
           pos = (x + 1 == t.cap) ? 0 : x + 1;
         }
@@ -2196,15 +2217,13 @@ k_ins_leader(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __rest
         const bool take = mine && incl <= 64;
         const unsigned long long took = __ballot(take);
         const uint32_t first = incl - len;  // this arc's first lane
-        if (take) arc_at[first] = arc0;
-        const unsigned long long starts = __ballot(take) ? [&] {
-          unsigned long long sm = 0;
-          for (unsigned long long tk = took; tk; tk &= tk - 1) {
-            const int l = __ffsll(tk) - 1;
-            sm |= 1ull << rd_lane(first, static_cast<uint32_t>(l));
-          }
-          return sm;
-        }() : 0ull;
+        if (take) {
+          arc_at[first] = arc0;
+          arc_src[first] = static_cast<uint8_t>(threadIdx.x);
+        }
+        unsigned long long starts = 0;
+        for (unsigned long long tk = took; tk; tk &= tk - 1)
+          starts |= 1ull << rd_lane(first, static_cast<uint32_t>(__ffsll(tk) - 1));
         const uint32_t total = rd_lane(incl, static_cast<uint32_t>(63 - __clzll(took)));
         wave_sync();
         const unsigned long long act = total >= 64 ? ~0ull : ((1ull << total) - 1ull);
@@ -2212,16 +2231,15 @@ k_ins_leader(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __rest
         const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
         const unsigned long long sm = starts & upto;
         const uint32_t a0 = sm ? 63u - static_cast<uint32_t>(__clzll(sm)) : 0u;
-        const uint64_t slot = (arc_at[a0] + (lane - a0)) & t.mask;
-        const bool inl = lane < total;
-        bool dry = false;
-        if (inl) {
-          const unsigned long long pw = P[slot >> 6], qw = Q[slot >> 6];
-          dry = ((pw & ~qw) >> (slot & 63)) & 1ull;
-        }
-        const unsigned long long mem = __ballot(inl && dry);
+        const uint32_t src = arc_src[a0], off = lane - a0;
+        const uint64_t slot = (arc_at[a0] + off) & t.mask;
+        // the arc's members: the leader lane's dry-slot mask and batch indices (slot order)
+        const unsigned long long dm = rd_any64(dmask, src);
+        const bool is_mem = lane < total && ((dm >> off) & 1ull);
+        const uint32_t o = is_mem ? lst[static_cast<uint32_t>(__popcll(dm & ((1ull << off) - 1ull))) * 64 + src] : 0u;
+        const unsigned long long mem = __ballot(is_mem);
         unsigned long long claims = 0;
-        claimed += arcs_round<KS, VS, true>(t, in_keys, in_vals, 0, act, starts, mem, ctr, claims, slot) *
+        claimed += arcs_round<KS, VS, true>(t, in_keys, in_vals, 0, act, starts, mem, ctr, claims, slot, o) *
                    (lane == 0 ? 1u : 0u);
         if (((claims >> lane) & 1ull) && t.nfree) claim_next(t, slot);
         wave_sync();
