@@ -194,14 +194,37 @@ kg_ins_dry(const Table* __restrict__ tables, OpDesc op, const uint8_t* __restric
   }
 }
 
-template <int KS, int VS>
+// LDS of one wave of kg_ins_leader: with the plan (static sizes), the
+// leader step's arrays (leader_wave) for closures of up to
+// kGroupLeaderMax members; without it, the member lists only.
+constexpr uint32_t kGroupLeaderMax = 8;
+template <int KS, int VS, bool PLAN>
+__host__ __device__ constexpr uint32_t group_leader_wave_bytes() {
+  return PLAN ? 2 * 64 * kPlanMoves * 8 + 64 * 8 + 64 * 16 * (VS % 16 == 0 ? VS / 16 : 0) + kGroupLeaderMax * 64 * 4 + 64
+              : kLeaderMax * 64 * 4;
+}
+
+// The leader pass over every chunk of a leader-mode table: each wave runs
+// the single map's leader step (leader_wave) on its 64 keys of the chunk,
+// with the table's claim bitmap; closures past its reach are queued for
+// the big replays with their table.
+template <int KS, int VS, bool PLAN>
 __global__ void __launch_bounds__(512)
 kg_ins_leader(const Table* __restrict__ tables, OpDesc op, const uint8_t* __restrict__ keys,
               const uint8_t* __restrict__ vals, const unsigned long long* __restrict__ Q,
               const unsigned long long* __restrict__ qoff, const uint32_t* __restrict__ dslot,
               unsigned long long* __restrict__ big, uint32_t* nbig, Counters* ctr, uint32_t coop_arc) {
-  extern __shared__ uint32_t lst[];  // kLeaderMax columns of blockDim lanes
-  uint32_t* my = lst + threadIdx.x;
+  extern __shared__ u32x4 gl_lds[];  // per wave: group_leader_wave_bytes<KS, VS, PLAN>() bytes
+  constexpr bool kPlan = PLAN && KS > 0 && KS % 4 == 0 && VS > 0 && VS % 4 == 0;
+  constexpr uint32_t kWaveBytes = group_leader_wave_bytes<KS, VS, kPlan>();
+  unsigned char* w = reinterpret_cast<unsigned char*>(gl_lds) + (threadIdx.x >> 6) * kWaveBytes;
+  uint64_t* mv_src = reinterpret_cast<uint64_t*>(w);
+  uint64_t* mv_dst = mv_src + (kPlan ? 64 * kPlanMoves : 0);
+  uint64_t* arc_at = mv_dst + (kPlan ? 64 * kPlanMoves : 0);
+  u32x4* mv_stage = reinterpret_cast<u32x4*>(arc_at + (kPlan ? 64 : 0));
+  uint32_t* lst = reinterpret_cast<uint32_t*>(mv_stage + (kPlan ? 64 * (VS % 16 == 0 ? VS / 16 : 0) : 0));
+  uint8_t* arc_src = reinterpret_cast<uint8_t*>(lst + 64 * (kPlan ? kGroupLeaderMax : kLeaderMax));
+  const uint32_t leader_max = kPlan ? kGroupLeaderMax : kLeaderMax;
   const uint32_t nch = *op.n_chunks;
   for (uint32_t c = blockIdx.x; c < nch; c += gridDim.x) {
     const Chunk ch = op.chunks[c];
@@ -210,48 +233,12 @@ kg_ins_leader(const Table* __restrict__ tables, OpDesc op, const uint8_t* __rest
     const Table t = tables[ch.table];
     const uint32_t K = KS ? KS : t.K;
     const uint32_t V = VS ? VS : t.V;
-    const unsigned long long* Qt = Q + qoff[ch.table];
-    uint32_t claimed = 0;
-    if (threadIdx.x < ch.count) {
-      const uint64_t s = dslot[ch.begin + threadIdx.x];
-      const uint64_t b = bm_prev_set(t.freebits, s, t.cap);
-      if (b != s && bit_at(Qt, b)) {
-        uint32_t m = 0;
-        uint64_t pos = s;
-        bool long_arc = false;
-        for (;;) {
-          const uint64_t x = bm_next_set(t.freebits, pos, t.cap);
-          if (bit_at(Qt, x)) {
-            long_arc = ((x - b - 1) & t.mask) > coop_arc;
-            break;
-          }
-          if (m == kLeaderMax) {
-            m = kLeaderMax + 1;
-            break;
-          }
-          my[m * blockDim.x] = t.owner[x];
-          ++m;
-          pos = (x + 1 == t.cap) ? 0 : x + 1;
-        }
-        if (m > kLeaderMax || long_arc) {
+    const uint32_t sd = threadIdx.x < ch.count ? dslot[ch.begin + threadIdx.x] : kNoSlot;
+    const unsigned long long claimed = leader_wave<KS, VS, kPlan>(
+        t, keys + to.begin * K, vals + to.begin * V, ch.local + threadIdx.x, sd, Q + qoff[ch.table], &ctr[ch.table],
+        coop_arc, leader_max, lst, mv_src, mv_dst, mv_stage, arc_at, arc_src, [&](uint64_t s) {
           big[atomicAdd(nbig, 1u)] = (static_cast<unsigned long long>(ch.table) << 32) | s;
-        } else {
-          for (uint32_t a = 1; a < m; ++a) {
-            const uint32_t v = my[a * blockDim.x];
-            uint32_t q = a;
-            while (q > 0 && my[(q - 1) * blockDim.x] > v) {
-              my[q * blockDim.x] = my[(q - 1) * blockDim.x];
-              --q;
-            }
-            my[q * blockDim.x] = v;
-          }
-          for (uint32_t j = 0; j < m; ++j) {
-            const uint64_t gk = to.begin + my[j * blockDim.x];
-            claimed += insert_one<KS, VS, false>(t, keys + gk * K, vals + gk * V, K, V, &ctr[ch.table]);
-          }
-        }
-      }
-    }
+        });
     block_add(&ctr[ch.table].count, claimed);  // chunk-uniform table
   }
 }
@@ -707,8 +694,16 @@ void g_insert(const Table* tables, OpDesc op, const TableOp* tops, uint32_t n_ta
   uint8_t* cls = reinterpret_cast<uint8_t*>(big + 2 * big_cap);
   hipLaunchKernelGGL(dev::kg_copy_words, dim3(gridn(total_words, 256)), dim3(256), 0, s, Q, freebits_all, total_words);
   hipLaunchKernelGGL((dev::kg_ins_dry<KS>), dim3(grid), dim3(block), 0, s, tables, op, keys, Q, qoff, dslot, ctr);
-  hipLaunchKernelGGL((dev::kg_ins_leader<KS, VS>), dim3(grid), dim3(block), dev::kLeaderMax * block * 4, s, tables, op,
-                     keys, vals, Q, qoff, dslot, big, nbig, ctr, coop_arc);
+  // the planned leader step when its LDS fits 64 KiB per workgroup
+  constexpr uint32_t kPlanBytes = dev::group_leader_wave_bytes<KS, VS, true>();
+  constexpr uint32_t kListBytes = dev::group_leader_wave_bytes<KS, VS, false>();
+  const uint32_t plan_lds = (block / 64) * kPlanBytes, list_lds = (block / 64) * kListBytes;
+  if (KS > 0 && VS > 0 && KS % 4 == 0 && VS % 4 == 0 && plan_lds <= 65536)
+    hipLaunchKernelGGL((dev::kg_ins_leader<KS, VS, true>), dim3(grid), dim3(block), plan_lds, s, tables, op, keys, vals,
+                       Q, qoff, dslot, big, nbig, ctr, coop_arc);
+  else
+    hipLaunchKernelGGL((dev::kg_ins_leader<KS, VS, false>), dim3(grid), dim3(block), list_lds, s, tables, op, keys,
+                       vals, Q, qoff, dslot, big, nbig, ctr, coop_arc);
   hipLaunchKernelGGL(dev::kg_big_class, dim3(256), dim3(256), 0, s, tables, Q, qoff, big, nbig, cls);
   hipLaunchKernelGGL(dev::kg_big_order, dim3(256), dim3(256), 0, s, big, cls, nbig, ordered);
   if constexpr (KS > 0 && VS > 0 && KS % 4 == 0 && VS % 4 == 0)

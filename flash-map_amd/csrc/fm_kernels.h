@@ -1580,7 +1580,7 @@ __device__ inline bool insert_planned(const Table& t, const Blob<KS>& ck, const 
       } else if (p > d) {  // Robin Hood eviction: the carried pair settles, the resident is carried on
         emask |= 1u << j;
         if (ne == 0) ep0 = p;
-        else ep1 = p;
+        else if (ne == 1) ep1 = p;  // (an incomplete plan counts more evictions than it applies)
         ++ne;
         p = d + 1;
       } else {
@@ -2067,219 +2067,244 @@ __device__ inline void hand_early(Counters* ctr, Counters* host_out, uint32_t se
 // bitmap words, so a one-member closure is decided and applied with two
 // further rounds of memory traffic (insert_planned).  The instance without
 // it (dense batches when k_ins_arcs is not used) holds no plan registers.
+// One wave's step of the leader pass: lane i's key (dry slot sd, kNoSlot
+// for none) leads its closure when the slot before it (in the free-slot
+// bitmap) is a barrier, and then inserts the closure's members: a planned
+// one-member insert, the packed replay, or walks; closures past the
+// leader's reach go to push_big(s).  LDS: lst (kLeaderMax x 64 words,
+// column per lane), mv_src/mv_dst (64 x kPlanMoves), mv_stage
+// (64 x VS/16 16-byte parts), arc_at/arc_src (64 each): the wave's own.
+// Returns the free slots this lane claimed (packed replays count on lane 0).
+template <int KS, int VS, bool PLAN, typename PushBig>
+__device__ inline unsigned long long leader_wave(const Table& t, const uint8_t* __restrict__ in_keys,
+                                                 const uint8_t* __restrict__ in_vals, uint64_t i, uint32_t sd,
+                                                 const unsigned long long* __restrict__ Q, Counters* ctr,
+                                                 uint32_t coop_arc, uint32_t leader_max, uint32_t* lst,
+                                                 uint64_t* mv_src, uint64_t* mv_dst, u32x4* mv_stage,
+                                                 uint64_t* arc_at, uint8_t* arc_src, PushBig push_big) {
+  // planned one-member inserts for static sizes; values of 16-byte multiples
+  // are copied by the whole wave, smaller ones by each lane
+  constexpr bool kPlan = PLAN && KS > 0 && KS % 4 == 0 && VS > 0 && VS % 4 == 0;
+  constexpr bool kCoop = kPlan && VS % 16 == 0;
+  const uint32_t K = KS ? KS : t.K;
+  const uint32_t V = VS ? VS : t.V;
+  const unsigned long long* P = t.freebits;
+  const uint64_t words = (t.cap + 63) >> 6;
+  const uint32_t lane = threadIdx.x & 63u;
+  uint32_t* my = lst + lane;
+  unsigned long long claimed = 0;
+  uint64_t mv[2 * kPlanMoves];
+  uint32_t nmv = 0;
+  bool packed = false;  // this lane's closure goes to the packed replay below
+  uint64_t arc0 = 0;
+  uint32_t alen = 0;
+  unsigned long long dmask = 0;
+  if (sd != kNoSlot) {
+    const uint64_t s = sd;
+    Blob<KS> ck;
+    uint64_t home = 0;
+    uint32_t L = ~0u;
+    uint32_t fl[kPlanSlots];
+    if (kPlan) {
+      ck.load(in_keys + i * K, K);
+      home = ck.hash(K, t.mode) & t.mask;
+      L = static_cast<uint32_t>(min<uint64_t>((s - home) & t.mask, kPlanSlots + 1));
+#pragma unroll
+      for (uint32_t j = 0; j < kPlanSlots; ++j) fl[j] = j < L ? t.flags[(home + j) & t.mask] : 0u;
+    }
+    // The free-slot and claim words at the dry slot, read together (and
+    // with the flags above); most closures begin, and end, inside that word.
+    uint64_t cw = s >> 6;
+    unsigned long long cp = P[cw], cq = Q[cw];
+    if (kPlan) {
+#pragma unroll
+      for (uint32_t j = 0; j < kPlanSlots; ++j) asm volatile("" ::"v"(fl[j]));
+    }
+    const unsigned long long below = cp & ((1ull << (s & 63)) - 1ull);
+    uint64_t b;
+    unsigned long long qb;
+    if (below) {
+      b = (cw << 6) + static_cast<uint64_t>(63 - __clzll(below));
+      qb = cq;
+    } else {
+      b = bm_prev_set(P, s, t.cap);
+      qb = Q[b >> 6];
+    }
+    if (b != s && ((qb >> (b & 63)) & 1ull)) {
+      uint32_t m = 0;
+      uint64_t pos = s;
+      bool long_arc = false;
+      const uint64_t a_first = (b + 1) & t.mask;
+      for (;;) {
+        // next pre-batch free slot at/after pos (cyclic), from the cached
+        // word pair or the next ones
+        if ((pos >> 6) != cw) {
+          cw = pos >> 6;
+          cp = P[cw];
+          cq = Q[cw];
+        }
+        unsigned long long bits = cp & (~0ull << (pos & 63));
+        for (uint64_t it = 0; !bits && it <= words; ++it) {  // b is set: found within a lap
+          cw = (cw + 1 == words) ? 0 : cw + 1;
+          cp = P[cw];
+          cq = Q[cw];
+          bits = cp;
+        }
+        const uint64_t x = (cw << 6) + static_cast<uint64_t>(__ffsll(bits) - 1);
+        if ((cq >> (x & 63)) & 1ull) {  // the closure's barrier
+          alen = static_cast<uint32_t>(min<uint64_t>((x - b - 1) & t.mask, 0xFFFFFFFFull));
+          long_arc = alen > coop_arc;
+          break;
+        }
+        if (m == leader_max) {
+          m = kLeaderMax + 1;
+          break;
+        }
+        my[m * 64] = (x == s) ? static_cast<uint32_t>(i) : t.owner[x];  // owner[s] == i (k_ins_dry)
+        const uint64_t dx = (x - a_first) & t.mask;
+        if (dx < 64) dmask |= 1ull << dx;  // the members' dry slots in the arc, in slot order
+        ++m;
+        pos = (x + 1 == t.cap) ? 0 : x + 1;
+      }
+      uint32_t c1 = 0;
+      if (m > kLeaderMax || long_arc) {
+        push_big(s);
+      } else if (kPlan && m == 1 &&
+                 insert_planned<KS, VS>(t, ck, in_vals + dd_value_index(t, i) * V, home, s, L, fl, &c1, mv, &nmv)) {
+        claimed += c1;
+        if (c1 && t.nfree) claim_next(t, s);
+        if (t.outcome) t.outcome[i] = static_cast<uint8_t>(c1);
+      } else if (m == 1 && !(kPlan)) {  // key i alone, dense batch: walked from its own pair
+        uint64_t at = 0;
+        c1 = insert_one<KS, VS, false>(t, in_keys + i * K, in_vals + dd_value_index(t, i) * V, K, V, ctr, &at);
+        claimed += c1;
+        if (c1 && t.nfree) claim_next(t, at);
+        if (t.outcome) t.outcome[i] = static_cast<uint8_t>(c1);
+      } else if (kPlan && alen <= kArcSlots && t.cap >= 256) {
+        packed = true;  // (replayed with the wave's other such closures after this step)
+        arc0 = (b + 1) & t.mask;
+      } else {
+        for (uint32_t a = 1; a < m; ++a) {  // insertion sort by batch index
+          const uint32_t v = my[a * 64];
+          uint32_t c = a;
+          while (c > 0 && my[(c - 1) * 64] > v) {
+            my[c * 64] = my[(c - 1) * 64];
+            --c;
+          }
+          my[c * 64] = v;
+        }
+        for (uint32_t j = 0; j < m; ++j) {
+          const uint64_t k = my[j * 64];
+          uint64_t at = 0;
+          c1 = insert_one<KS, VS, false>(t, in_keys + k * K, in_vals + dd_value_index(t, k) * V, K, V, ctr, &at);
+          claimed += c1;
+          if (c1 && t.nfree) claim_next(t, at);
+          if (t.outcome) t.outcome[k] = static_cast<uint8_t>(c1);
+        }
+      }
+    }
+  }
+  if constexpr (kPlan) {
+    // Closures that the plan does not cover (several members, or a walk
+    // past the plan's reach) whose arcs hold at most 64 slots: packed side
+    // by side into the 64 lanes (arc after arc in lane order, as many as
+    // fit) and replayed together by wave scans, one member per arc and
+    // step (arcs_round), instead of walked slot by slot.
+    unsigned long long todo = __ballot(packed);
+    while (todo) {
+      const bool mine = (todo >> lane) & 1ull;
+      const uint32_t len = mine ? alen : 0u;
+      const uint32_t incl = wave_incl_add(len);
+      const bool take = mine && incl <= 64;
+      const unsigned long long took = __ballot(take);
+      const uint32_t first = incl - len;  // this arc's first lane
+      if (take) {
+        arc_at[first] = arc0;
+        arc_src[first] = static_cast<uint8_t>(lane);
+      }
+      unsigned long long starts = 0;
+      for (unsigned long long tk = took; tk; tk &= tk - 1)
+        starts |= 1ull << rd_lane(first, static_cast<uint32_t>(__ffsll(tk) - 1));
+      const uint32_t total = rd_lane(incl, static_cast<uint32_t>(63 - __clzll(took)));
+      wave_sync();
+      const unsigned long long act = total >= 64 ? ~0ull : ((1ull << total) - 1ull);
+      const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+      const unsigned long long sm = starts & upto;
+      const uint32_t a0 = sm ? 63u - static_cast<uint32_t>(__clzll(sm)) : 0u;
+      const uint32_t src = arc_src[a0], off = lane - a0;
+      const uint64_t slot = (arc_at[a0] + off) & t.mask;
+      // the arc's members: the leader lane's dry-slot mask and batch indices (slot order)
+      const unsigned long long dm = rd_any64(dmask, src);
+      const bool is_mem = lane < total && ((dm >> off) & 1ull);
+      const uint32_t o = is_mem ? lst[static_cast<uint32_t>(__popcll(dm & ((1ull << off) - 1ull))) * 64 + src] : 0u;
+      const unsigned long long mem = __ballot(is_mem);
+      unsigned long long claims = 0;
+      claimed += arcs_round<KS, VS, true>(t, in_keys, in_vals, 0, act, starts, mem, ctr, claims, slot, o) *
+                 (lane == 0 ? 1u : 0u);
+      if (((claims >> lane) & 1ull) && t.nfree) claim_next(t, slot);
+      wave_sync();
+      todo &= ~took;
+    }
+  }
+  if constexpr (kPlan && !kCoop) {
+    // the planned inserts' value copies, each lane its own: every source
+    // read before the first write
+    Blob<VS> a, b, c;
+    if (nmv > 0) a.load(reinterpret_cast<const uint8_t*>(mv[0]), V);
+    if (nmv > 1) b.load(reinterpret_cast<const uint8_t*>(mv[2]), V);
+    if (nmv > 2) c.load(reinterpret_cast<const uint8_t*>(mv[4]), V);
+    if (nmv > 0) a.store(reinterpret_cast<uint8_t*>(mv[1]), V);
+    if (nmv > 1) b.store(reinterpret_cast<uint8_t*>(mv[3]), V);
+    if (nmv > 2) c.store(reinterpret_cast<uint8_t*>(mv[5]), V);
+  }
+  if constexpr (kCoop) {
+    // the planned inserts' value copies, by the whole wave (wave-uniform)
+    uint32_t incl = nmv;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t y = static_cast<uint32_t>(__shfl_up(static_cast<int>(incl), off, 64));
+      if (lane >= static_cast<uint32_t>(off)) incl += y;
+    }
+    const uint32_t total = static_cast<uint32_t>(__shfl(static_cast<int>(incl), 63, 64));
+    if (total) {
+      const uint32_t first = incl - nmv;
+#pragma unroll
+      for (uint32_t k = 0; k < kPlanMoves; ++k) {
+        if (k < nmv) {
+          mv_src[first + k] = mv[2 * k];
+          mv_dst[first + k] = mv[2 * k + 1];
+        }
+      }
+      wave_sync();
+      wave_copy_values<VS>(mv_src, mv_dst, total, mv_stage);
+      wave_sync();
+    }
+  }
+  return claimed;
+}
+
 template <int KS, int VS, bool PLAN>
 __global__ void __launch_bounds__(64)
 k_ins_leader(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __restrict__ in_vals, uint64_t n,
              const unsigned long long* __restrict__ Q, const uint32_t* __restrict__ dslot, uint32_t* __restrict__ big,
              Counters* ctr, uint32_t coop_arc, uint32_t leader_max, Counters* host_out, uint32_t seq) {
-  __shared__ uint32_t lst[kLeaderMax * 64];  // column per lane: lst[j * 64 + lane]
-  // planned one-member inserts for static sizes; values of 16-byte multiples
-  // are copied by the whole wave, smaller ones by each lane
   constexpr bool kPlan = PLAN && KS > 0 && KS % 4 == 0 && VS > 0 && VS % 4 == 0;
   constexpr bool kCoop = kPlan && VS % 16 == 0;
+  __shared__ uint32_t lst[kLeaderMax * 64];  // column per lane: lst[j * 64 + lane]
   __shared__ uint64_t mv_src[kCoop ? 64 * kPlanMoves : 1], mv_dst[kCoop ? 64 * kPlanMoves : 1];
   __shared__ u32x4 mv_stage[kCoop ? 64 * (VS / 16) : 1];
   __shared__ uint64_t arc_at[kPlan ? 64 : 1];  // packed replay: the first slot of the arc starting in lane l
   __shared__ uint8_t arc_src[kPlan ? 64 : 1];   // ... and the lane that leads it
-  const uint32_t K = KS ? KS : t.K;
-  const uint32_t V = VS ? VS : t.V;
-  const unsigned long long* P = t.freebits;
-  const uint64_t words = (t.cap + 63) >> 6;
-  uint32_t* my = lst + threadIdx.x;
   const bool failed = (ctr->status & kStatusDryFail) != 0;  // block-uniform
   unsigned long long claimed = 0;
   for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * blockDim.x;; base += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
     if (failed || base >= n) break;
     const uint64_t i = base + threadIdx.x;
     const uint32_t sd = i < n ? dslot[i] : kNoSlot;
-    uint64_t mv[2 * kPlanMoves];
-    uint32_t nmv = 0;
-    bool packed = false;  // this lane's closure goes to the packed replay below
-    uint64_t arc0 = 0;
-    uint32_t alen = 0;
-    unsigned long long dmask = 0;
-    if (sd != kNoSlot) {
-      const uint64_t s = sd;
-      Blob<KS> ck;
-      uint64_t home = 0;
-      uint32_t L = ~0u;
-      uint32_t fl[kPlanSlots];
-      if (kPlan) {
-        ck.load(in_keys + i * K, K);
-        home = ck.hash(K, t.mode) & t.mask;
-        L = static_cast<uint32_t>(min<uint64_t>((s - home) & t.mask, kPlanSlots + 1));
-#pragma unroll
-        for (uint32_t j = 0; j < kPlanSlots; ++j) fl[j] = j < L ? t.flags[(home + j) & t.mask] : 0u;
-      }
-      // The free-slot and claim words at the dry slot, read together (and
-      // with the flags above); most closures begin, and end, inside that word.
-      uint64_t cw = s >> 6;
-      unsigned long long cp = P[cw], cq = Q[cw];
-      if (kPlan) {
-#pragma unroll
-        for (uint32_t j = 0; j < kPlanSlots; ++j) asm volatile("" ::"v"(fl[j]));
-      }
-      const unsigned long long below = cp & ((1ull << (s & 63)) - 1ull);
-      uint64_t b;
-      unsigned long long qb;
-      if (below) {
-        b = (cw << 6) + static_cast<uint64_t>(63 - __clzll(below));
-        qb = cq;
-      } else {
-        b = bm_prev_set(P, s, t.cap);
-        qb = Q[b >> 6];
-      }
-      if (b != s && ((qb >> (b & 63)) & 1ull)) {
-        uint32_t m = 0;
-        uint64_t pos = s;
-        bool long_arc = false;
-        const uint64_t a_first = (b + 1) & t.mask;
-        for (;;) {
-          // next pre-batch free slot at/after pos (cyclic), from the cached
-          // word pair or the next ones
-          if ((pos >> 6) != cw) {
-            cw = pos >> 6;
-            cp = P[cw];
-            cq = Q[cw];
-          }
-          unsigned long long bits = cp & (~0ull << (pos & 63));
-          for (uint64_t it = 0; !bits && it <= words; ++it) {  // b is set: found within a lap
-            cw = (cw + 1 == words) ? 0 : cw + 1;
-            cp = P[cw];
-            cq = Q[cw];
-            bits = cp;
-          }
-          const uint64_t x = (cw << 6) + static_cast<uint64_t>(__ffsll(bits) - 1);
-          if ((cq >> (x & 63)) & 1ull) {  // the closure's barrier
-            alen = static_cast<uint32_t>(min<uint64_t>((x - b - 1) & t.mask, 0xFFFFFFFFull));
-            long_arc = alen > coop_arc;
-            break;
-          }
-          if (m == leader_max) {
-            m = kLeaderMax + 1;
-            break;
-          }
-          my[m * 64] = (x == s) ? static_cast<uint32_t>(i) : t.owner[x];  // owner[s] == i (k_ins_dry)
-          const uint64_t dx = (x - a_first) & t.mask;
-          if (dx < 64) dmask |= 1ull << dx;  // the members' dry slots in the arc, in slot order
-          ++m;
-          pos = (x + 1 == t.cap) ? 0 : x + 1;
-        }
-        uint32_t c1 = 0;
-        if (m > kLeaderMax || long_arc) {
-          big[atomicAdd(&ctr->nbig, 1u)] = static_cast<uint32_t>(s);
-        } else if (kPlan && m == 1 &&
-                   insert_planned<KS, VS>(t, ck, in_vals + dd_value_index(t, i) * V, home, s, L, fl, &c1, mv, &nmv)) {
-          claimed += c1;
-          if (c1 && t.nfree) claim_next(t, s);
-          if (t.outcome) t.outcome[i] = static_cast<uint8_t>(c1);
-        } else if (m == 1 && !(kPlan)) {  // key i alone, dense batch: walked from its own pair
-          uint64_t at = 0;
-          c1 = insert_one<KS, VS, false>(t, in_keys + i * K, in_vals + dd_value_index(t, i) * V, K, V, ctr, &at);
-          claimed += c1;
-          if (c1 && t.nfree) claim_next(t, at);
-          if (t.outcome) t.outcome[i] = static_cast<uint8_t>(c1);
-        } else if (kPlan && alen <= kArcSlots && t.cap >= 256) {
-          packed = true;  // (replayed with the wave's other such closures after this step)
-          arc0 = (b + 1) & t.mask;
-        } else {
-          for (uint32_t a = 1; a < m; ++a) {  // insertion sort by batch index
-            const uint32_t v = my[a * 64];
-            uint32_t c = a;
-            while (c > 0 && my[(c - 1) * 64] > v) {
-              my[c * 64] = my[(c - 1) * 64];
-              --c;
-            }
-            my[c * 64] = v;
-          }
-          for (uint32_t j = 0; j < m; ++j) {
-            const uint64_t k = my[j * 64];
-            uint64_t at = 0;
-            c1 = insert_one<KS, VS, false>(t, in_keys + k * K, in_vals + dd_value_index(t, k) * V, K, V, ctr, &at);
-            claimed += c1;
-            if (c1 && t.nfree) claim_next(t, at);
-            if (t.outcome) t.outcome[k] = static_cast<uint8_t>(c1);
-          }
-        }
-      }
-    }
-    if constexpr (kPlan) {
-      // Closures that the plan does not cover (several members, or a walk
-      // past the plan's reach) whose arcs hold at most 64 slots: packed side
-      // by side into the 64 lanes (arc after arc in lane order, as many as
-      // fit) and replayed together by wave scans, one member per arc and
-      // step (arcs_round), instead of walked slot by slot.
-      unsigned long long todo = __ballot(packed);
-      while (todo) {
-        const bool mine = (todo >> threadIdx.x) & 1ull;
-        const uint32_t len = mine ? alen : 0u;
-        const uint32_t incl = wave_incl_add(len);
-        const bool take = mine && incl <= 64;
-        const unsigned long long took = __ballot(take);
-        const uint32_t first = incl - len;  // this arc's first lane
-        if (take) {
-          arc_at[first] = arc0;
-          arc_src[first] = static_cast<uint8_t>(threadIdx.x);
-        }
-        unsigned long long starts = 0;
-        for (unsigned long long tk = took; tk; tk &= tk - 1)
-          starts |= 1ull << rd_lane(first, static_cast<uint32_t>(__ffsll(tk) - 1));
-        const uint32_t total = rd_lane(incl, static_cast<uint32_t>(63 - __clzll(took)));
-        wave_sync();
-        const unsigned long long act = total >= 64 ? ~0ull : ((1ull << total) - 1ull);
-        const uint32_t lane = threadIdx.x & 63u;
-        const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
-        const unsigned long long sm = starts & upto;
-        const uint32_t a0 = sm ? 63u - static_cast<uint32_t>(__clzll(sm)) : 0u;
-        const uint32_t src = arc_src[a0], off = lane - a0;
-        const uint64_t slot = (arc_at[a0] + off) & t.mask;
-        // the arc's members: the leader lane's dry-slot mask and batch indices (slot order)
-        const unsigned long long dm = rd_any64(dmask, src);
-        const bool is_mem = lane < total && ((dm >> off) & 1ull);
-        const uint32_t o = is_mem ? lst[static_cast<uint32_t>(__popcll(dm & ((1ull << off) - 1ull))) * 64 + src] : 0u;
-        const unsigned long long mem = __ballot(is_mem);
-        unsigned long long claims = 0;
-        claimed += arcs_round<KS, VS, true>(t, in_keys, in_vals, 0, act, starts, mem, ctr, claims, slot, o) *
-                   (lane == 0 ? 1u : 0u);
-        if (((claims >> lane) & 1ull) && t.nfree) claim_next(t, slot);
-        wave_sync();
-        todo &= ~took;
-      }
-    }
-    if constexpr (kPlan && !kCoop) {
-      // the planned inserts' value copies, each lane its own: every source
-      // read before the first write
-      Blob<VS> a, b, c;
-      if (nmv > 0) a.load(reinterpret_cast<const uint8_t*>(mv[0]), V);
-      if (nmv > 1) b.load(reinterpret_cast<const uint8_t*>(mv[2]), V);
-      if (nmv > 2) c.load(reinterpret_cast<const uint8_t*>(mv[4]), V);
-      if (nmv > 0) a.store(reinterpret_cast<uint8_t*>(mv[1]), V);
-      if (nmv > 1) b.store(reinterpret_cast<uint8_t*>(mv[3]), V);
-      if (nmv > 2) c.store(reinterpret_cast<uint8_t*>(mv[5]), V);
-    }
-    if constexpr (kCoop) {
-      // the planned inserts' value copies, by the whole wave (wave-uniform)
-      uint32_t incl = nmv;
-#pragma unroll
-      for (int off = 1; off < 64; off <<= 1) {
-        const uint32_t y = static_cast<uint32_t>(__shfl_up(static_cast<int>(incl), off, 64));
-        if (threadIdx.x >= static_cast<uint32_t>(off)) incl += y;
-      }
-      const uint32_t total = static_cast<uint32_t>(__shfl(static_cast<int>(incl), 63, 64));
-      if (total) {
-        const uint32_t first = incl - nmv;
-#pragma unroll
-        for (uint32_t k = 0; k < kPlanMoves; ++k) {
-          if (k < nmv) {
-            mv_src[first + k] = mv[2 * k];
-            mv_dst[first + k] = mv[2 * k + 1];
-          }
-        }
-        wave_sync();
-        wave_copy_values<VS>(mv_src, mv_dst, total, mv_stage);
-        wave_sync();
-      }
-    }
+    claimed += leader_wave<KS, VS, PLAN>(t, in_keys, in_vals, i, sd, Q, ctr, coop_arc, leader_max, lst, mv_src, mv_dst,
+                                         mv_stage, arc_at, arc_src, [&](uint64_t s) {
+                                           big[atomicAdd(&ctr->nbig, 1u)] = static_cast<uint32_t>(s);
+                                         });
   }
   if (!host_out) {
     block_add(&ctr->count, claimed);

@@ -513,6 +513,34 @@ def test_gpu_canonical_arc_replay(K, V, canon_min, monkeypatch):
     apply_and_compare(m, o, ops)
 
 
+@pytest.mark.parametrize("K,V", [(8, 8), (32, 128)])
+def test_gpu_sparse_updates_after_removes(K, V):
+    """Sparse insert batches (n * 16 < capacity: planned one-member inserts,
+    packed replays) that are half updates, alternating with removes on a
+    crowded table.  Removes leave runs that are no longer sorted by home, so
+    an update's walk can evict residents before it reaches its equal key:
+    the plan then ends at that key with evictions applied before it, each
+    evicted slot with its own distance.  The whole slot array is compared
+    with the oracle after every op."""
+    say(f"sparse updates after removes K={K} V={V}")
+    rng = np.random.default_rng(1234 + K)
+    cap = 1 << 14
+    m, o = gpu_make(cap, K, V), OracleAdapter(cap, K, V)
+    keys = rng.integers(0, 256, (cap, K), dtype=np.uint8)
+    live = 12_000
+    ops = [("ins", keys[:live], rng.integers(0, 256, (live, V), dtype=np.uint8))]
+    nxt = live
+    for _ in range(12):
+        ops.append(("rem", keys[rng.integers(0, nxt, 400)]))
+        upd = keys[rng.integers(0, nxt, 250)]
+        new = keys[nxt:nxt + 250]
+        nxt += 250
+        batch = np.concatenate([upd, new])[rng.permutation(500)]
+        ops.append(("ins", batch, rng.integers(0, 256, (500, V), dtype=np.uint8)))
+    ops.append(("get", keys[:nxt]))
+    apply_and_compare(m, o, ops)
+
+
 @pytest.mark.parametrize("seed", [0, 1])
 def test_gpu_bitmap_rebuild_and_per_key_batches_alternate(seed):
     """Insert batches alternate across the size where the bits pass switches
