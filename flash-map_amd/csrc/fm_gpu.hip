@@ -522,6 +522,25 @@ class GpuBackend final : public Backend {
   uint64_t h_out_bytes_ = 0;
   // Grows the pinned buffers; false when a call should keep the pageable path.
   bool ensure_host_staging(uint64_t in, uint64_t out);
+  // Staged transfers move in pieces, so the copy pool fills (or empties) one
+  // piece of the pinned buffer while the DMA engine moves the one before it:
+  // a call costs the DMA time plus one piece of host copying instead of the
+  // two one after the other.
+  struct HostXfer {
+    uint8_t* dev;
+    uint8_t* host;  // source of stage_in, destination of stage_out
+    uint64_t bytes;
+  };
+  static constexpr unsigned kStageXfers = 2;  // transfers per staged call at most
+  static constexpr unsigned kStagePieces = 16;
+  static constexpr uint64_t kStagePieceMin = 4ull << 20;
+  hipEvent_t stage_ev_[kStagePieces + kStageXfers] = {};  // one per piece of stage_out
+  static uint64_t stage_piece(const HostXfer* xs, unsigned nx);
+  // Host -> h_in_ -> device on stream_; returns once the last piece is queued.
+  fm_status stage_in(const HostXfer* xs, unsigned nx);
+  // Device -> h_out_ -> host after the work queued on stream_; returns when
+  // every byte is in place.
+  fm_status stage_out(const HostXfer* xs, unsigned nx);
 };
 
 template <typename T>
@@ -570,6 +589,10 @@ void GpuBackend::release() {
   if (h_out_) (void)hipHostFree(h_out_);
   h_in_ = h_out_ = nullptr;
   h_in_bytes_ = h_out_bytes_ = 0;
+  for (hipEvent_t& e : stage_ev_) {
+    if (e) (void)hipEventDestroy(e);
+    e = nullptr;
+  }
   copies_.reset();
   pending_.clear();
   for (hipEvent_t e : ev_pool_) (void)hipEventDestroy(e);
@@ -746,8 +769,65 @@ bool GpuBackend::ensure_host_staging(uint64_t in, uint64_t out) {
     return true;
   };
   if (!grow(h_in_, h_in_bytes_, in) || !grow(h_out_, h_out_bytes_, out)) return false;
+  for (hipEvent_t& e : stage_ev_)
+    if (!e && hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+      e = nullptr;
+      return false;
+    }
   if (!copies_) copies_ = std::make_unique<CopyPool>(kCopyWorkers);
   return true;
+}
+
+uint64_t GpuBackend::stage_piece(const HostXfer* xs, unsigned nx) {
+  uint64_t total = 0;
+  for (unsigned i = 0; i < nx; ++i) total += xs[i].bytes;
+  const uint64_t even = (total + kStagePieces - 1) / kStagePieces;
+  return (std::max(kStagePieceMin, even) + 0xFFFF) & ~0xFFFFull;  // nx transfers: at most kStagePieces + nx pieces
+}
+
+fm_status GpuBackend::stage_in(const HostXfer* xs, unsigned nx) {
+  const uint64_t piece = stage_piece(xs, nx);
+  uint64_t off = 0;
+  for (unsigned i = 0; i < nx; ++i)
+    for (uint64_t o = 0; o < xs[i].bytes; o += piece) {
+      const uint64_t len = std::min(piece, xs[i].bytes - o);
+      copies_->run({{h_in_ + off, xs[i].host + o, len}});
+      hipError_t e = hipMemcpyAsync(xs[i].dev + o, h_in_ + off, len, hipMemcpyHostToDevice, stream_);
+      if (e != hipSuccess) return fail(FM_ERR_TRANSFER, hip_msg("H2D (staged)", e));
+      off += len;
+    }
+  return FM_OK;
+}
+
+fm_status GpuBackend::stage_out(const HostXfer* xs, unsigned nx) {
+  const uint64_t piece = stage_piece(xs, nx);
+  hipError_t e = hipSuccess;
+  unsigned k = 0;
+  uint64_t off = 0;
+  for (unsigned i = 0; i < nx && e == hipSuccess; ++i)
+    for (uint64_t o = 0; o < xs[i].bytes && e == hipSuccess; o += piece, ++k) {
+      const uint64_t len = std::min(piece, xs[i].bytes - o);
+      if ((e = hipMemcpyAsync(h_out_ + off, xs[i].dev + o, len, hipMemcpyDeviceToHost, stream_)) == hipSuccess)
+        e = hipEventRecord(stage_ev_[k], stream_);
+      off += len;
+    }
+  if (e != hipSuccess) {
+    (void)hipStreamSynchronize(stream_);  // nothing queued here may still write h_out_ after the call
+    return fail(FM_ERR_TRANSFER, hip_msg("D2H (staged)", e));
+  }
+  k = 0;
+  off = 0;
+  for (unsigned i = 0; i < nx; ++i)
+    for (uint64_t o = 0; o < xs[i].bytes; o += piece, ++k) {
+      const uint64_t len = std::min(piece, xs[i].bytes - o);
+      if ((e = hipEventSynchronize(stage_ev_[k])) != hipSuccess) {
+        (void)hipStreamSynchronize(stream_);
+        return fail(FM_ERR_TRANSFER, hip_msg("D2H (staged)", e));
+      }
+      copies_->run({{xs[i].host + o, h_out_ + off, len}});
+      off += len;
+    }
+  return FM_OK;
 }
 
 bool GpuBackend::ensure_next_bits() {
@@ -1237,24 +1317,25 @@ fm_status GpuBackend::bulk_get(const void* keys, uint64_t n, void* out_values, u
   if ((st = ensure_staging(n * t_.K, 0, n * t_.V, n)) != FM_OK) return st;
   const uint64_t kb = n * t_.K, vb = n * t_.V;
   const bool large = kb + vb + n >= kPinnedMin;
-  const bool stage_in = large && !host_pinned(keys, kb) && ensure_host_staging(kb, vb + n);
-  const bool stage_out =
+  const bool staged_in = large && !host_pinned(keys, kb) && ensure_host_staging(kb, vb + n);
+  const bool staged_out =
       large && !(host_pinned(out_values, vb) && host_pinned(out_found, n)) && ensure_host_staging(kb, vb + n);
-  const void* src = keys;
-  if (stage_in) {
-    copies_->run({{h_in_, static_cast<const uint8_t*>(keys), kb}});
-    src = h_in_;
+  hipError_t e;
+  if (staged_in) {
+    const HostXfer x{st_keys_, const_cast<uint8_t*>(static_cast<const uint8_t*>(keys)), kb};
+    if ((st = stage_in(&x, 1)) != FM_OK) return st;
+  } else if ((e = hipMemcpyAsync(st_keys_, keys, kb, hipMemcpyHostToDevice, stream_)) != hipSuccess) {
+    return fail(FM_ERR_TRANSFER, hip_msg("H2D keys", e));
   }
-  hipError_t e = hipMemcpyAsync(st_keys_, src, kb, hipMemcpyHostToDevice, stream_);
-  if (e != hipSuccess) return fail(FM_ERR_TRANSFER, hip_msg("H2D keys", e));
   if ((st = run_get(st_keys_, n, st_out_, st_found_)) != FM_OK) return st;
-  uint8_t* ov = stage_out ? h_out_ : static_cast<uint8_t*>(out_values);
-  uint8_t* of = stage_out ? h_out_ + vb : out_found;
-  if ((e = hipMemcpyAsync(ov, st_out_, vb, hipMemcpyDeviceToHost, stream_)) != hipSuccess ||
-      (e = hipMemcpyAsync(of, st_found_, n, hipMemcpyDeviceToHost, stream_)) != hipSuccess ||
+  if (staged_out) {
+    const HostXfer xs[2] = {{st_out_, static_cast<uint8_t*>(out_values), vb}, {st_found_, out_found, n}};
+    return stage_out(xs, 2);
+  }
+  if ((e = hipMemcpyAsync(out_values, st_out_, vb, hipMemcpyDeviceToHost, stream_)) != hipSuccess ||
+      (e = hipMemcpyAsync(out_found, st_found_, n, hipMemcpyDeviceToHost, stream_)) != hipSuccess ||
       (e = hipStreamSynchronize(stream_)) != hipSuccess)
     return fail(FM_ERR_TRANSFER, hip_msg("D2H results", e));
-  if (stage_out) copies_->run({{static_cast<uint8_t*>(out_values), h_out_, vb}, {out_found, h_out_ + vb, n}});
   return FM_OK;
 }
 
@@ -1271,17 +1352,15 @@ fm_status GpuBackend::bulk_insert(const void* keys, const void* values, uint64_t
   if ((st = use_device()) != FM_OK) return st;
   if ((st = ensure_staging(n * t_.K, n * t_.V, 0, 0)) != FM_OK) return st;
   const uint64_t kb = n * t_.K, vb = n * t_.V;
-  const void* ks = keys;
-  const void* vs = values;
-  if (kb + vb >= kPinnedMin && !(host_pinned(keys, kb) && host_pinned(values, vb)) && ensure_host_staging(kb + vb, 0)) {
-    copies_->run({{h_in_, static_cast<const uint8_t*>(keys), kb}, {h_in_ + kb, static_cast<const uint8_t*>(values), vb}});
-    ks = h_in_;
-    vs = h_in_ + kb;
-  }
   hipError_t e;
-  if ((e = hipMemcpyAsync(st_keys_, ks, kb, hipMemcpyHostToDevice, stream_)) != hipSuccess ||
-      (e = hipMemcpyAsync(st_vals_, vs, vb, hipMemcpyHostToDevice, stream_)) != hipSuccess)
+  if (kb + vb >= kPinnedMin && !(host_pinned(keys, kb) && host_pinned(values, vb)) && ensure_host_staging(kb + vb, 0)) {
+    const HostXfer xs[2] = {{st_keys_, const_cast<uint8_t*>(static_cast<const uint8_t*>(keys)), kb},
+                            {st_vals_, const_cast<uint8_t*>(static_cast<const uint8_t*>(values)), vb}};
+    if ((st = stage_in(xs, 2)) != FM_OK) return st;
+  } else if ((e = hipMemcpyAsync(st_keys_, keys, kb, hipMemcpyHostToDevice, stream_)) != hipSuccess ||
+             (e = hipMemcpyAsync(st_vals_, values, vb, hipMemcpyHostToDevice, stream_)) != hipSuccess) {
     return fail(FM_ERR_TRANSFER, hip_msg("H2D pairs", e));
+  }
   if ((st = run_insert(st_keys_, st_vals_, n, nullptr, true)) != FM_OK) return st;
   Counters c;
   if ((st = read_insert_counters(&c)) != FM_OK) return st;
@@ -1301,13 +1380,13 @@ fm_status GpuBackend::bulk_remove(const void* keys, uint64_t n, uint64_t* out_re
   if ((st = use_device()) != FM_OK) return st;
   if ((st = ensure_staging(n * t_.K, 0, 0, 0)) != FM_OK) return st;
   const uint64_t kb = n * t_.K;
-  const void* src = keys;
+  hipError_t e;
   if (kb >= kPinnedMin && !host_pinned(keys, kb) && ensure_host_staging(kb, 0)) {
-    copies_->run({{h_in_, static_cast<const uint8_t*>(keys), kb}});
-    src = h_in_;
+    const HostXfer x{st_keys_, const_cast<uint8_t*>(static_cast<const uint8_t*>(keys)), kb};
+    if ((st = stage_in(&x, 1)) != FM_OK) return st;
+  } else if ((e = hipMemcpyAsync(st_keys_, keys, kb, hipMemcpyHostToDevice, stream_)) != hipSuccess) {
+    return fail(FM_ERR_TRANSFER, hip_msg("H2D keys", e));
   }
-  hipError_t e = hipMemcpyAsync(st_keys_, src, kb, hipMemcpyHostToDevice, stream_);
-  if (e != hipSuccess) return fail(FM_ERR_TRANSFER, hip_msg("H2D keys", e));
   uint64_t removed = 0;
   if ((st = run_remove(st_keys_, n, &removed)) != FM_OK) return st;
   len_ -= std::min(len_, removed);

@@ -251,6 +251,18 @@ def _as_bytes(a, size: int, what: str) -> Tuple[np.ndarray, int]:
     return arr, arr.size // size
 
 
+def _result_array(a, shape, dtype, what: str) -> np.ndarray:
+    """A fresh zeroed result array, or the caller's `a` once it is checked to
+    be a writable C-contiguous array of that shape and dtype."""
+    if a is None:
+        return np.zeros(shape, dtype=dtype)
+    if not isinstance(a, np.ndarray) or a.dtype != dtype or a.shape != tuple(shape) \
+            or not a.flags.c_contiguous or not a.flags.writeable:
+        raise ValueError(f"{what}: want a writable C-contiguous {np.dtype(dtype).name} array of shape "
+                         f"{tuple(shape)}")
+    return a
+
+
 def _ptr(arr: np.ndarray) -> ctypes.c_void_p:
     return ctypes.c_void_p(arr.ctypes.data if arr.size else 0)
 
@@ -308,14 +320,18 @@ class FlashMap:
         self.close()
 
     # -- host-facing API (lib.rs:123-207) --
-    def bulk_get(self, keys) -> Tuple[np.ndarray, np.ndarray]:
-        """Returns (values: (n, V) uint8, found: (n,) bool); misses are zeroed."""
+    def bulk_get(self, keys, out=None, found=None) -> Tuple[np.ndarray, np.ndarray]:
+        """Returns (values: (n, V) uint8, found: (n,) bool); misses are zeroed.
+
+        `out` ((n, V) uint8) and `found` ((n,) bool) may be caller arrays that
+        receive the results, for instance carved from a PinnedArena, which the
+        DMA engines fill without a host copy; they are returned."""
         k, n = _as_bytes(keys, self.key_size, "keys")
-        vals = np.zeros((n, self.value_size), dtype=np.uint8)
-        found = np.zeros(n, dtype=np.uint8)
+        vals = _result_array(out, (n, self.value_size), np.uint8, "out")
+        hits = _result_array(found, (n,), np.bool_, "found")
         if n:
-            _check(lib().fm_bulk_get(self._h, _ptr(k), n, _ptr(vals), _ptr(found)))
-        return vals, found.astype(bool)
+            _check(lib().fm_bulk_get(self._h, _ptr(k), n, _ptr(vals), _ptr(hits)))
+        return vals, hits
 
     def bulk_get_options(self, keys) -> List[Optional[bytes]]:
         """Vec<Option<V>> form of bulk_get (lib.rs:124)."""

@@ -104,6 +104,25 @@ def test_error_paths_cpu():
     assert "FlashMap" in repr(m) and 'backend: "CPU"' in repr(m)
 
 
+def test_bulk_get_into_caller_arrays_cpu():
+    m = cpu_make(64, 8, 8, 0)
+    ks = np.arange(20, dtype=np.uint64)
+    m.bulk_insert(ks[:10], ks[:10] * 3)
+    out = np.full((20, 8), 9, dtype=np.uint8)
+    found = np.zeros(20, dtype=np.bool_)
+    gv, gf = m.bulk_get(ks, out=out, found=found)
+    assert gv is out and gf is found
+    ev, ef = m.bulk_get(ks)
+    assert np.array_equal(out, ev) and np.array_equal(found, ef)
+    assert found[:10].all() and not found[10:].any() and not out[10:].any()  # misses zeroed
+    for bad in (np.zeros((20, 8), dtype=np.uint16), np.zeros((19, 8), dtype=np.uint8),
+                np.zeros((8, 20), dtype=np.uint8).T):
+        with pytest.raises(ValueError):
+            m.bulk_get(ks, out=bad)
+    with pytest.raises(ValueError):
+        m.bulk_get(ks, found=np.zeros(20, dtype=np.uint8))
+
+
 def test_table_full_all_update_batch_cpu():
     # TableFull fires on len + n > cap even when every key is an update
     # (rayon_cpu.rs:134-142)

@@ -541,6 +541,47 @@ def test_gpu_sparse_updates_after_removes(K, V):
     apply_and_compare(m, o, ops)
 
 
+@pytest.mark.parametrize("pin", ["pageable", "pinned_keys", "pinned_results"])
+def test_gpu_host_calls_in_staged_pieces(pin):
+    """Host-buffer calls of several MiB move through the pinned staging
+    buffers in pieces of at least 4 MiB that the host copy and the DMA engine
+    work on in turn.  Sizes are odd, so each transfer's last piece is short;
+    the second insert updates earlier keys, the gets hold misses (zeroed
+    values), and one side of a call may be pinned (fm_host_alloc) while the
+    other is staged.  Counts, values, found flags and the slot array against
+    the oracle."""
+    say(f"staged host calls {pin}")
+    rng = np.random.default_rng(77)
+    cap, K, V, n = 1 << 19, 32, 128, 400_009
+    m, o = gpu_make(cap, K, V), OracleAdapter(cap, K, V)
+    keys = rng.integers(0, 256, (n, K), dtype=np.uint8)
+    vals = rng.integers(0, 256, (n, V), dtype=np.uint8)
+    arena = fm.PinnedArena(3 * n * (K + V) + n + (1 << 16)) if pin != "pageable" else None
+    host = (lambda a: arena.copy_of(a)) if pin == "pinned_keys" else (lambda a: a)
+
+    def get_and_compare(ks, where):
+        outs = {}
+        if pin == "pinned_results":
+            outs = {"out": arena.take((len(ks), V), fill=7), "found": arena.take((len(ks),), fill=1).view(np.bool_)}
+        gv, gf = m.bulk_get(host(ks), **outs)
+        ev, ef = o.bulk_get(ks)
+        assert np.array_equal(gf, ef), f"{where}: found flags differ"
+        assert np.array_equal(gv, ev), f"{where}: values differ"
+
+    assert m.bulk_insert(host(keys[:300_007]), host(vals[:300_007])) == o.bulk_insert(keys[:300_007], vals[:300_007])
+    batch = np.concatenate([keys[100_000:150_001], keys[300_007:350_009]])
+    bvals = rng.integers(0, 256, (len(batch), V), dtype=np.uint8)
+    assert m.bulk_insert(host(batch), host(bvals)) == o.bulk_insert(batch, bvals)
+    get_and_compare(keys, "after inserts")
+    gone = keys[::3]
+    assert m.bulk_remove(host(gone)) == o.bulk_remove(gone)
+    get_and_compare(keys, "after remove")
+    assert len(m) == len(o)
+    assert_same_slots(m, o, "slots")
+    if arena is not None:
+        arena.close()
+
+
 @pytest.mark.parametrize("seed", [0, 1])
 def test_gpu_bitmap_rebuild_and_per_key_batches_alternate(seed):
     """Insert batches alternate across the size where the bits pass switches
