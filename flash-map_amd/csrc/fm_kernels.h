@@ -1713,6 +1713,20 @@ __device__ inline void wave_copy_values(const uint64_t* lst_src, const uint64_t*
 // written.  Requires cap >= 256 (the window's three bitmap words differ).
 // ---------------------------------------------------------------------------
 constexpr uint32_t kArcSlots = 64;
+
+// Bitonic sort of one 64-bit key per lane across the wave, ascending by lane.
+__device__ __forceinline__ void wave_sort64(unsigned long long& sk, uint32_t lane) {
+#pragma unroll
+  for (uint32_t size = 2; size <= 64; size <<= 1) {
+#pragma unroll
+    for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+      const unsigned long long other = __shfl_xor(sk, static_cast<int>(stride), 64);
+      const bool keep_min = ((lane & stride) == 0) == ((lane & size) == 0);
+      sk = keep_min ? (other < sk ? other : sk) : (other > sk ? other : sk);
+    }
+  }
+}
+
 constexpr uint32_t kVsBatch = 0x100u;  // value source: the batch value of the member in lane (vs & 63)
 
 // One round: lanes are the slots base .. base+63, `act` marks the slots of
@@ -1751,16 +1765,24 @@ __device__ __forceinline__ uint32_t arcs_round(const Table& t, const uint8_t* __
   const unsigned long long amask = (alen == 64 ? ~0ull : ((1ull << alen) - 1ull)) << a0;
   const uint32_t mcount = static_cast<uint32_t>(__popcll(mem & amask));
   const uint32_t mst = static_cast<uint32_t>(__popcll(mem & ((1ull << a0) - 1ull)));
+  // fresh: every slot of the round's arcs was free before the batch (the
+  // arcs hold only their members' dry claims), so no slot is read
+  const bool fresh = !kPacked && act == mem;
   uint32_t f = 0, o = 0;
   Blob<KS> rk;  // the slot's key
+#pragma unroll
+  for (int w = 0; w < KW; ++w) rk.w[w] = 0;
   if (in) {
-    f = t.flags[q];
-    rk.load(t.keys + q * KS, KS);
+    if (!fresh) {
+      f = t.flags[q];
+      rk.load(t.keys + q * KS, KS);
+    }
     if (is_mem) o = kPacked ? lane_owner : t.owner[q];
     keep_loaded<KS>(f, rk);
   }
   Blob<KS> mk;  // a member's key, in the lane of its dry slot
   uint32_t hp = 0, vi = 0;
+  uint64_t mh = 0;  // its hash
   // (packed: the slot held by lane a0, the arc's first)
   const uint64_t org = kPacked ? (static_cast<uint64_t>(static_cast<uint32_t>(__shfl(static_cast<int>(q), a0, 64))) |
                                   (static_cast<uint64_t>(static_cast<uint32_t>(__shfl(static_cast<int>(q >> 32), a0, 64))) << 32))
@@ -1768,11 +1790,91 @@ __device__ __forceinline__ uint32_t arcs_round(const Table& t, const uint8_t* __
   if (is_mem) {
     mk.load(in_keys + static_cast<uint64_t>(o) * KS, KS);
     vi = static_cast<uint32_t>(dd_value_index(t, o));
-    hp = kPacked ? a0 + static_cast<uint32_t>(((mk.hash(KS, t.mode) & t.mask) - org) & t.mask)
-                 : static_cast<uint32_t>(((mk.hash(KS, t.mode) & t.mask) - base) & t.mask);  // home lane
+    mh = mk.hash(KS, t.mode);
+    hp = kPacked ? a0 + static_cast<uint32_t>(((mh & t.mask) - org) & t.mask)
+                 : static_cast<uint32_t>(((mh & t.mask) - base) & t.mask);  // home lane
     if (hp > lane || hp < a0) {  // cannot happen: a member's home lies in its arc, at or before its dry slot
       atomicOr(&ctr->status, kStatusBudget);
       hp = lane;
+    }
+  }
+  if constexpr (!kPacked) {
+    if (fresh) {
+      // Insertion in batch order into free slots leaves an arc's members
+      // sorted by home, one to a slot from the arc's first.  Within one
+      // home's group the order follows the walks: member y parks at the
+      // first slot at or after its home that no earlier member took (F_y),
+      // joins its own home's group at the end (it passes the pairs of its
+      // home, equal distances, and evicts the next group's first), and
+      // rotates every group of a home h with home_y < h < F_y (that group's
+      // first pair is carried past the others of its home to the group's
+      // end).  Groups are replayed as queues of member lanes, 6 bits each,
+      // in the lane of their first pair in (home, batch) order.  A later
+      // copy of a key updates the first copy instead: equal hashes in one
+      // home's group (or a group of more than 8) send the round to the walk
+      // below.
+      unsigned long long fk = is_mem ? (static_cast<unsigned long long>(hp) << 38) |
+                                           (static_cast<unsigned long long>(o) << 6) | lane
+                                     : (~0ull << 6) | lane;
+      wave_sort64(fk, lane);
+      const uint32_t nm = static_cast<uint32_t>(__popcll(mem));
+      const uint32_t shp = static_cast<uint32_t>(fk >> 38) & 63u;
+      const uint64_t smh = static_cast<uint64_t>(__shfl(static_cast<long long>(mh), static_cast<int>(fk & 63u), 64));
+      bool clash = false;
+#pragma unroll
+      for (uint32_t d = 1; d <= 8; ++d) {
+        const uint32_t php = static_cast<uint32_t>(__shfl_up(static_cast<int>(shp), d, 64));
+        const uint64_t pmh = static_cast<uint64_t>(__shfl_up(static_cast<long long>(smh), d, 64));
+        clash = clash || (lane < nm && lane >= d && php == shp && (d == 8 || pmh == smh));
+      }
+      if (!__ballot(clash)) {
+        const uint32_t p1 = static_cast<uint32_t>(__shfl_up(static_cast<int>(shp), 1, 64));
+        const unsigned long long leaders = __ballot(lane < nm && (lane == 0 || p1 != shp));
+        const bool lead = (leaders >> lane) & 1ull;
+        unsigned long long bk = is_mem ? (static_cast<unsigned long long>(o) << 6) | lane : ~0ull;
+        wave_sort64(bk, lane);  // members in batch order
+        unsigned long long gq = 0, taken = 0;
+        uint32_t gn = 0;
+        for (uint32_t j = 0; j < nm; ++j) {
+          const uint32_t yl = rd_lane(static_cast<uint32_t>(bk), j) & 63u;
+          const uint32_t hy = rd_lane(hp, yl);
+          const unsigned long long fr = ~taken & (~0ull << hy);
+          const uint32_t F = fr ? static_cast<uint32_t>(__ffsll(fr) - 1) : 64u;
+          taken |= fr ? (1ull << F) : 0ull;
+          if (lead) {
+            if (shp == hy) {
+              gq |= static_cast<unsigned long long>(yl) << (6u * gn);
+              ++gn;
+            } else if (hy < shp && shp < F && gn > 1) {
+              gq = (gq >> 6) | ((gq & 63ull) << (6u * (gn - 1)));
+            }
+          }
+        }
+        if (taken != act) atomicOr(&ctr->status, kStatusBudget);  // cannot happen: parked outside the arcs
+        // slot lane takes the pair at its rank among the arcs' slots: the
+        // (rank - group start)-th of its group's queue
+        const uint32_t r = static_cast<uint32_t>(__popcll(act & ((1ull << lane) - 1ull)));
+        const unsigned long long below = leaders & ((2ull << r) - 1ull);  // r = 63: every lane
+        const uint32_t g0 = below ? 63u - static_cast<uint32_t>(__clzll(below)) : 0u;
+        const uint32_t gh = static_cast<uint32_t>(__shfl(static_cast<int>(shp), static_cast<int>(r), 64));
+        const unsigned long long gqr = static_cast<unsigned long long>(__shfl(static_cast<long long>(gq), static_cast<int>(g0), 64));
+        const int src = static_cast<int>((gqr >> (6u * ((r - g0) & 7u))) & 63ull);
+        Blob<KS> ck;
+#pragma unroll
+        for (int w = 0; w < KW; ++w) ck.w[w] = static_cast<uint32_t>(__shfl(static_cast<int>(mk.w[w]), src, 64));
+        const uint32_t vj = static_cast<uint32_t>(__shfl(static_cast<int>(vi), src, 64));
+        const uint32_t oj = static_cast<uint32_t>(__shfl(static_cast<int>(o), src, 64));
+        if (in) {
+          Blob<VS> v;
+          v.load(in_vals + static_cast<uint64_t>(vj) * VS, VS);
+          t.flags[q] = make_flag(kOccupied, lane - gh);
+          ck.store(t.keys + q * KS, KS);
+          v.store(t.vals + q * VS, VS);
+          if (t.outcome) t.outcome[oj] = 1;
+        }
+        claims = act;
+        return nm;
+      }
     }
   }
   bool occ = in && (f & 0xFu) == kOccupied;
@@ -1783,15 +1885,7 @@ __device__ __forceinline__ uint32_t arcs_round(const Table& t, const uint8_t* __
   unsigned long long sk = is_mem ? (static_cast<unsigned long long>(a0) << 38) |
                                        (static_cast<unsigned long long>(o) << 6) | lane
                                  : (~0ull << 6) | lane;
-#pragma unroll
-  for (uint32_t size = 2; size <= 64; size <<= 1) {
-#pragma unroll
-    for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
-      const unsigned long long other = __shfl_xor(sk, static_cast<int>(stride), 64);
-      const bool keep_min = ((lane & stride) == 0) == ((lane & size) == 0);
-      sk = keep_min ? (other < sk ? other : sk) : (other > sk ? other : sk);
-    }
-  }
+  wave_sort64(sk, lane);
   // As in replay_arc_lds: with a_k = dist_k - (k - h) per occupied slot k
   // at or after the member's home h, the eviction points are the strict new
   // minima of the a's (the carried pair counting as 0 before slot h), the
