@@ -1743,7 +1743,7 @@ __device__ __forceinline__ uint32_t arcs_round(const Table& t, const uint8_t* __
                                                unsigned long long act, unsigned long long starts,
                                                unsigned long long mem, Counters* ctr,
                                                unsigned long long& claims, uint64_t lane_slot = 0,
-                                               uint32_t lane_owner = 0) {
+                                               uint32_t lane_owner = 0, uint32_t* __restrict__ mk_lds = nullptr) {
   constexpr int KW = KS / 4;
   const uint32_t lane = threadIdx.x & 63u;
   const bool in = (act >> lane) & 1ull;
@@ -1797,6 +1797,10 @@ __device__ __forceinline__ uint32_t arcs_round(const Table& t, const uint8_t* __
       atomicOr(&ctr->status, kStatusBudget);
       hp = lane;
     }
+  }
+  if constexpr (!kPacked) {  // members' keys in this wave's LDS (read back in order by the same wave)
+#pragma unroll
+    for (int w = 0; w < KW; ++w) mk_lds[w * 64 + lane] = mk.w[w];
   }
   if constexpr (!kPacked) {
     if (fresh) {
@@ -1861,7 +1865,10 @@ __device__ __forceinline__ uint32_t arcs_round(const Table& t, const uint8_t* __
         const int src = static_cast<int>((gqr >> (6u * ((r - g0) & 7u))) & 63ull);
         Blob<KS> ck;
 #pragma unroll
-        for (int w = 0; w < KW; ++w) ck.w[w] = static_cast<uint32_t>(__shfl(static_cast<int>(mk.w[w]), src, 64));
+        for (int w = 0; w < KW; ++w) {
+      if constexpr (kPacked) ck.w[w] = static_cast<uint32_t>(__shfl(static_cast<int>(mk.w[w]), src, 64));
+      else ck.w[w] = mk_lds[w * 64 + src];
+    }
         const uint32_t vj = static_cast<uint32_t>(__shfl(static_cast<int>(vi), src, 64));
         const uint32_t oj = static_cast<uint32_t>(__shfl(static_cast<int>(o), src, 64));
         if (in) {
@@ -1911,7 +1918,10 @@ __device__ __forceinline__ uint32_t arcs_round(const Table& t, const uint8_t* __
     const uint32_t oj = (slo >> 6) | (shi << 26);
     Blob<KS> ck;  // the member's key
 #pragma unroll
-    for (int w = 0; w < KW; ++w) ck.w[w] = static_cast<uint32_t>(__shfl(static_cast<int>(mk.w[w]), src, 64));
+    for (int w = 0; w < KW; ++w) {
+      if constexpr (kPacked) ck.w[w] = static_cast<uint32_t>(__shfl(static_cast<int>(mk.w[w]), src, 64));
+      else ck.w[w] = mk_lds[w * 64 + src];
+    }
     const uint32_t h = static_cast<uint32_t>(__shfl(static_cast<int>(hp), src, 64));
     const bool inw = live && lane >= h;
     const int ak = static_cast<int>(dist < 127u ? dist : 127u) - static_cast<int>(lane - h);
@@ -2039,6 +2049,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_num_sgpr(72), amdg
 k_ins_arcs(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __restrict__ in_vals,
            const unsigned long long* __restrict__ Q, uint32_t* __restrict__ big, Counters* ctr) {
   static_assert(KS > 0 && VS > 0 && KS % 4 == 0 && VS % 4 == 0, "static sizes");
+  __shared__ uint32_t mk_lds[1024 / 64 * (KS / 4) * 64];  // per wave: the members' keys, word-major
   const unsigned long long* P = t.freebits;
   const uint64_t words = t.cap >> 6;
   const uint64_t w = (static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;  // wave-uniform
@@ -2092,11 +2103,12 @@ k_ins_arcs(Table t, const uint8_t* __restrict__ in_keys, const uint8_t* __restri
     // still in flight holds up the second round's loads (one counter)
     unsigned long long cl1 = 0, cl2 = 0;
     const bool nb = t.nfree != nullptr;
-    if (act) got += arcs_round<KS, VS>(t, in_keys, in_vals, w << 6, act, starts_in, mem, ctr, cl1);
+    uint32_t* mkl = mk_lds + (threadIdx.x >> 6) * (KS / 4) * 64;
+    if (act) got += arcs_round<KS, VS>(t, in_keys, in_vals, w << 6, act, starts_in, mem, ctr, cl1, 0, 0, mkl);
     if (cross < 64) {
       const unsigned long long a2 = cross_len == 64 ? ~0ull : ((1ull << cross_len) - 1ull);
       const unsigned long long m2 = (cross == 0 ? c1 : ((c1 >> cross) | (c2 << (64u - cross)))) & a2;
-      got += arcs_round<KS, VS>(t, in_keys, in_vals, (w << 6) + cross, a2, 1ull, m2, ctr, cl2);
+      got += arcs_round<KS, VS>(t, in_keys, in_vals, (w << 6) + cross, a2, 1ull, m2, ctr, cl2, 0, 0, mkl);
     }
     if (nb && (threadIdx.x & 63u) == 0) {
       claim_next_mask(t, w << 6, cl1);
